@@ -1,0 +1,640 @@
+// C ABI of the matcher (include/gpsm.h): device context, resident graph and
+// the batched filter -> collect -> candidate-edge -> join pipeline.
+//
+// Phases of one batch (host synchronisations marked *):
+//   A  zero bitmaps, label/degree check of every start vertex, one superstep
+//      per filter step index (explore / refine ops of all queries), row
+//      popcounts                                                   * |C(u)|
+//   B  compaction of every C(u) + rank index, forward |CET| counts * |CET|
+//   -- host: join order per query (heuristic, learned or explicit)
+//   C  candidate-edge tables of the directed edges the plans use
+//   D  persistent join over tasks (slices of first tables)        * results
+// Phases A and C+D may be captured in hipGraphs (opts->use_graph).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "api_internal.h"
+#include "gpsm.h"
+#include "kernels/launch.h"
+#include "plan.h"
+
+using namespace gpsm;
+
+#define HIPCHK(x)                                       \
+  do {                                                  \
+    hipError_t e_ = (x);                                \
+    if (e_ != hipSuccess) {                             \
+      set_error(#x, e_);                                \
+      return e_ == hipErrorOutOfMemory ? GPSM_ENOMEM : GPSM_EDEVICE; \
+    }                                                   \
+  } while (0)
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void set_error(const char* what, hipError_t e) {
+  g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+}
+
+}  // namespace
+
+namespace gpsm {
+
+int DevBuf::ensure(size_t nbytes) {
+  if (nbytes <= bytes) return GPSM_OK;
+  if (ptr) hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+  size_t want = std::max<size_t>(nbytes + nbytes / 4, 4096);
+  if (hipMalloc(&ptr, want) != hipSuccess) {
+    ptr = nullptr;
+    (void)hipGetLastError();
+    return GPSM_ENOMEM;
+  }
+  bytes = want;
+  return GPSM_OK;
+}
+
+void DevBuf::release() {
+  if (ptr) hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+}  // namespace gpsm
+
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const char* gpsm_strerror(int code) {
+  switch (code) {
+    case GPSM_OK: return "ok";
+    case GPSM_EINVAL: return "invalid argument";
+    case GPSM_ENOMEM: return "out of memory";
+    case GPSM_EDEVICE: return "HIP runtime error";
+    case GPSM_ETOOBIG: return "query larger than GPSM_MAX_QV";
+    case GPSM_EOVERFLOW: return "tuple capacity exceeded";
+    case GPSM_ENODEV: return "no HIP device";
+    case GPSM_EDISCONN: return "query not connected or without edges";
+    case GPSM_EFORMAT: return "malformed or unterminated input";
+    default: return "unknown error";
+  }
+}
+
+int gpsm_init(int device, gpsm_ctx** out) {
+  if (!out) return GPSM_EINVAL;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    (void)hipGetLastError();
+    return GPSM_ENODEV;
+  }
+  if (device < 0 || device >= count) return GPSM_EINVAL;
+  HIPCHK(hipSetDevice(device));
+  gpsm_ctx* c = new gpsm_ctx();
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    std::snprintf(c->name, sizeof(c->name), "%s (%s)", prop.name, prop.gcnArchName);
+    c->ncu = prop.multiProcessorCount;
+  } else {
+    c->ncu = 256;
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return GPSM_EDEVICE;
+  }
+  // warm-up: first allocation and launch (match/Match.cu:37-45)
+  if (c->bufs[B_SCRATCH].ensure(1 << 20) != GPSM_OK) {
+    delete c;
+    return GPSM_ENOMEM;
+  }
+  *out = c;
+  return GPSM_OK;
+}
+
+void gpsm_shutdown(gpsm_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  for (auto& b : c->bufs) b.release();
+  hipEventDestroy(c->ev0);
+  hipEventDestroy(c->ev1);
+  hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* gpsm_device_name(const gpsm_ctx* c) { return c ? c->name : ""; }
+
+int gpsm_graph_upload(gpsm_ctx* c, uint32_t n, const uint32_t* row_offset, const uint32_t* col,
+                      const uint32_t* labels, gpsm_graph** out) {
+  if (!c || !out || !row_offset || !labels || n == 0) return GPSM_EINVAL;
+  *out = nullptr;
+  const uint64_t A = row_offset[n];
+  if (A && !col) return GPSM_EINVAL;
+  // validate the CSR: monotone offsets, sorted rows, no self-loops/duplicates
+  if (row_offset[0] != 0) return GPSM_EINVAL;
+  for (uint32_t v = 0; v < n; ++v) {
+    if (row_offset[v + 1] < row_offset[v]) return GPSM_EINVAL;
+    for (uint32_t e = row_offset[v]; e < row_offset[v + 1]; ++e) {
+      if (col[e] >= n || col[e] == v) return GPSM_EINVAL;
+      if (e > row_offset[v] && col[e] <= col[e - 1]) return GPSM_EINVAL;
+    }
+    if (labels[v] == 0) return GPSM_EINVAL;
+  }
+  gpsm_graph* g = new gpsm_graph();
+  g->ctx = c;
+  GraphStats& s = g->stats;
+  s.n = n;
+  s.adj = A;
+  std::vector<uint32_t> lv(labels, labels + n);
+  std::sort(lv.begin(), lv.end());
+  for (uint32_t i = 0; i < n; ++i) {
+    if (i == 0 || lv[i] != lv[i - 1]) {
+      s.label_value.push_back(lv[i]);
+      s.label_count.push_back(0);
+    }
+    s.label_count.back()++;
+  }
+  if (s.label_value.size() > MAX_LABELS) {
+    delete g;
+    return GPSM_EINVAL;
+  }
+  std::vector<uint32_t> key(n);
+  for (uint32_t v = 0; v < n; ++v) {
+    uint32_t deg = row_offset[v + 1] - row_offset[v];
+    key[v] = (s.dense(labels[v]) << KEY_DEG_BITS) | std::min(deg, KEY_DEG_MASK);
+  }
+  DevGraph& d = g->dg;
+  d.n = n;
+  d.adj = A;
+  d.W = words_of(n);
+  d.NB = nblocks_of(n);
+  hipSetDevice(c->device);
+  auto fail = [&](int code) {
+    hipFree(d.row);
+    hipFree(d.col);
+    hipFree(d.key);
+    delete g;
+    return code;
+  };
+  if (hipMalloc(&d.row, sizeof(uint32_t) * (n + 1)) != hipSuccess ||
+      hipMalloc(&d.col, sizeof(uint32_t) * std::max<uint64_t>(A, 1)) != hipSuccess ||
+      hipMalloc(&d.key, sizeof(uint32_t) * n) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(GPSM_ENOMEM);
+  }
+  if (hipMemcpyAsync(d.row, row_offset, sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice, c->stream) !=
+          hipSuccess ||
+      (A && hipMemcpyAsync(d.col, col, sizeof(uint32_t) * A, hipMemcpyHostToDevice, c->stream) != hipSuccess) ||
+      hipMemcpyAsync(d.key, key.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(GPSM_EDEVICE);
+  }
+  g->bytes = sizeof(uint32_t) * ((uint64_t)n + 1 + A + n);
+  *out = g;
+  return GPSM_OK;
+}
+
+int gpsm_graph_free(gpsm_graph* g) {
+  if (!g) return GPSM_EINVAL;
+  hipSetDevice(g->ctx->device);
+  hipStreamSynchronize(g->ctx->stream);
+  hipFree(g->dg.row);
+  hipFree(g->dg.col);
+  hipFree(g->dg.key);
+  delete g;
+  return GPSM_OK;
+}
+
+uint64_t gpsm_graph_device_bytes(const gpsm_graph* g) { return g ? g->bytes : 0; }
+
+void gpsm_result_free(gpsm_result* r) {
+  if (!r) return;
+  std::free(r->tuples);
+  r->tuples = nullptr;
+  r->n_tuples = 0;
+}
+
+uint64_t gpsm_tuple_hash(const uint32_t* t, uint32_t q);  // gen.cpp
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The batch pipeline.
+
+namespace gpsm {
+
+namespace {
+
+template <typename T>
+T* dptr(gpsm_ctx* c, int b) {
+  return reinterpret_cast<T*>(c->bufs[b].ptr);
+}
+
+template <typename T>
+int upload(gpsm_ctx* c, int b, const std::vector<T>& v) {
+  int rc = c->bufs[b].ensure(sizeof(T) * std::max<size_t>(v.size(), 1));
+  if (rc) return rc;
+  if (!v.empty() && hipMemcpyAsync(c->bufs[b].ptr, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice,
+                                   c->stream) != hipSuccess)
+    return GPSM_EDEVICE;
+  return GPSM_OK;
+}
+
+struct QState {
+  int status = GPSM_OK;
+  bool active = false;  // still matching (valid query, no empty candidate set)
+  HostQuery hq;
+  FilterSchedule fs;
+  uint32_t vbase = 0;
+  OrderPlan plan;
+  std::vector<uint64_t> cet_size;  // forward |CET| per query edge
+  uint32_t first_desc = 0;
+};
+
+int predict_with_ctx(void* user, const float* x, uint32_t B, float* y) {
+  return gpsm_mlp_forward(reinterpret_cast<gpsm_mlp*>(user), x, B, y);
+}
+
+}  // namespace
+
+int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* qs, const gpsm_opts* opts,
+              gpsm_result* out, const BatchHooks* hooks) {
+  const DevGraph& dg = g->dg;
+  const GraphStats& st = g->stats;
+  hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  gpsm_opts o{};
+  if (opts) o = *opts;
+  const uint32_t fblock = o.block ? o.block : c->tune.filter_block;
+  const uint32_t cap = std::max<uint32_t>(o.join_cap ? o.join_cap : c->tune.join_cap, 512);
+  const uint32_t wg_per_cu = c->tune.join_wg_per_cu;
+
+  std::vector<QState> Q(nq);
+  uint32_t nslots = 0, maxsteps = 0, qmax = 2;
+  for (uint32_t b = 0; b < nq; ++b) {
+    std::memset(&out[b], 0, sizeof(gpsm_result));
+    QState& S = Q[b];
+    S.status = S.hq.build(&qs[b]);
+    out[b].q = qs[b].n;
+    if (S.status != GPSM_OK) continue;
+    make_filter_schedule(S.hq, st, S.fs);
+    S.active = true;
+    for (uint32_t u = 0; u < S.hq.n; ++u)
+      if (st.dense(S.hq.lab[u]) == 0) S.active = false;  // a label absent from the data graph
+    S.vbase = nslots;
+    nslots += S.hq.n;
+    maxsteps = std::max<uint32_t>(maxsteps, (uint32_t)S.fs.steps.size());
+    qmax = std::max(qmax, S.hq.n);
+  }
+  const uint32_t W = dg.W, NB = dg.NB;
+  int rc;
+  auto t0 = std::chrono::steady_clock::now();
+  hipEventRecord(c->ev0, s);
+
+  // ---------------- phase A: filter ----------------
+  std::vector<InitOp> init;
+  std::vector<FilterOp> fops;
+  std::vector<uint32_t> step_off(maxsteps + 1, 0), nbr_slot, nbr_lo, nbr_hi;
+  for (uint32_t b = 0; b < nq; ++b)
+    if (Q[b].active) init.push_back({Q[b].vbase + Q[b].fs.start, Q[b].fs.start_lo, Q[b].fs.start_hi});
+  for (uint32_t sidx = 0; sidx < maxsteps; ++sidx) {
+    step_off[sidx] = (uint32_t)fops.size();
+    for (uint32_t b = 0; b < nq; ++b) {
+      const QState& S = Q[b];
+      if (!S.active || sidx >= S.fs.steps.size()) continue;
+      const FilterStep& fsx = S.fs.steps[sidx];
+      FilterOp op;
+      op.type = fsx.type;
+      op.u_slot = S.vbase + fsx.u;
+      op.nv = (uint32_t)fsx.v.size();
+      op.nbr_off = (uint32_t)nbr_slot.size();
+      for (size_t i = 0; i < fsx.v.size(); ++i) {
+        nbr_slot.push_back(S.vbase + fsx.v[i]);
+        nbr_lo.push_back(fsx.type == OP_EXPLORE ? fsx.lo[i] : 0);
+        nbr_hi.push_back(fsx.type == OP_EXPLORE ? fsx.hi[i] : 0);
+      }
+      fops.push_back(op);
+    }
+  }
+  step_off[maxsteps] = (uint32_t)fops.size();
+  if ((rc = c->bufs[B_BITMAPS].ensure(sizeof(uint32_t) * (size_t)std::max<uint32_t>(nslots, 1) * W))) return rc;
+  if ((rc = upload(c, B_INIT, init)) || (rc = upload(c, B_FOPS, fops)) || (rc = upload(c, B_NBR_SLOT, nbr_slot)) ||
+      (rc = upload(c, B_NBR_LO, nbr_lo)) || (rc = upload(c, B_NBR_HI, nbr_hi)))
+    return rc;
+  if ((rc = c->bufs[B_COUNTS].ensure(sizeof(uint32_t) * std::max<uint32_t>(nslots, 1)))) return rc;
+  uint32_t* bitmaps = dptr<uint32_t>(c, B_BITMAPS);
+  {
+    bool captured = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (o.use_graph && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) captured = true;
+    launch_fill_u32(bitmaps, 0u, (uint64_t)nslots * W, s);
+    launch_init(dg, dptr<InitOp>(c, B_INIT), (uint32_t)init.size(), bitmaps, s);
+    for (uint32_t sidx = 0; sidx < maxsteps; ++sidx)
+      launch_filter_step(dg, dptr<FilterOp>(c, B_FOPS) + step_off[sidx], step_off[sidx + 1] - step_off[sidx],
+                         dptr<uint32_t>(c, B_NBR_SLOT), dptr<uint32_t>(c, B_NBR_LO), dptr<uint32_t>(c, B_NBR_HI),
+                         bitmaps, fblock, s);
+    launch_row_popcount(dg, bitmaps, nslots, dptr<uint32_t>(c, B_COUNTS), s);
+    if (captured) {
+      if (hipStreamEndCapture(s, &graph) != hipSuccess || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) !=
+                                                              hipSuccess)
+        return GPSM_EDEVICE;
+      HIPCHK(hipGraphLaunch(exec, s));
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<uint32_t>& counts = c->h_counts;
+    counts.assign(std::max<uint32_t>(nslots, 1), 0);
+    HIPCHK(hipMemcpyAsync(counts.data(), dptr<uint32_t>(c, B_COUNTS), sizeof(uint32_t) * nslots,
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+  }
+  const std::vector<uint32_t>& counts = c->h_counts;
+  if (hooks && hooks->after_filter) {
+    rc = hooks->after_filter(hooks->user, c, bitmaps, counts.data(), nslots, W);
+    if (rc) return rc;
+  }
+  for (uint32_t b = 0; b < nq; ++b) {
+    QState& S = Q[b];
+    if (S.status != GPSM_OK) continue;
+    for (uint32_t u = 0; u < S.hq.n && u < GPSM_MAX_QV; ++u) out[b].cand[u] = S.active ? counts[S.vbase + u] : 0;
+    for (uint32_t u = 0; u < S.hq.n && S.active; ++u)
+      if (counts[S.vbase + u] == 0) S.active = false;
+  }
+
+  // ---------------- phase B: compaction + forward |CET| ----------------
+  std::vector<uint64_t> coff(std::max<uint32_t>(nslots, 1) + 1, 0);
+  for (uint32_t r = 0; r < nslots; ++r) coff[r + 1] = coff[r] + counts[r];
+  if ((rc = upload(c, B_COFF, coff))) return rc;
+  if ((rc = c->bufs[B_CLIST].ensure(sizeof(uint32_t) * std::max<uint64_t>(coff[nslots], 1)))) return rc;
+  if ((rc = c->bufs[B_BPREFIX].ensure(sizeof(uint32_t) * (size_t)std::max<uint32_t>(nslots, 1) * NB))) return rc;
+  launch_compact_rows(dg, bitmaps, nslots, dptr<uint64_t>(c, B_COFF), dptr<uint32_t>(c, B_CLIST),
+                      dptr<uint32_t>(c, B_BPREFIX), s);
+  std::vector<CetDesc> fdesc;
+  std::vector<uint32_t> fdesc_of_q(nq, 0);
+  for (uint32_t b = 0; b < nq; ++b) {
+    QState& S = Q[b];
+    fdesc_of_q[b] = (uint32_t)fdesc.size();
+    if (!S.active) continue;
+    for (size_t e = 0; e < S.hq.ef.size(); ++e) {
+      CetDesc d{};
+      d.x_slot = S.vbase + S.hq.ef[e];
+      d.w_slot = S.vbase + S.hq.et[e];
+      d.ncand = counts[d.x_slot];
+      d.clist_off = coff[d.x_slot];
+      fdesc.push_back(d);
+    }
+  }
+  const uint32_t cgrid = std::max<uint32_t>(c->ncu * 8, 64);
+  if ((rc = upload(c, B_FDESC, fdesc))) return rc;
+  if ((rc = c->bufs[B_CHUNKS].ensure(sizeof(uint32_t) * (fdesc.size() + 2)))) return rc;
+  launch_cet_count(dg, bitmaps, dptr<uint32_t>(c, B_CLIST), dptr<CetDesc>(c, B_FDESC), (uint32_t)fdesc.size(),
+                   nullptr, 0, dptr<uint32_t>(c, B_CHUNKS), cgrid, s);
+  HIPCHK(hipGetLastError());
+  if (!fdesc.empty())
+    HIPCHK(hipMemcpyAsync(fdesc.data(), dptr<CetDesc>(c, B_FDESC), sizeof(CetDesc) * fdesc.size(),
+                          hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (hooks && hooks->after_cet_count) {
+    rc = hooks->after_cet_count(hooks->user, c, fdesc.data(), (uint32_t)fdesc.size());
+    if (rc) return rc;
+  }
+
+  // ---------------- host: join orders ----------------
+  std::vector<CetDesc> jdesc;
+  std::vector<JoinPlan> plans;
+  std::vector<JoinTask> tasks;
+  std::vector<uint32_t> plan_of_q(nq, ~0u);
+  uint64_t off_words = 0, list_words = 0;
+  for (uint32_t b = 0; b < nq; ++b) {
+    QState& S = Q[b];
+    if (!S.active) continue;
+    const size_t E = S.hq.ef.size();
+    S.cet_size.resize(E);
+    for (size_t e = 0; e < E; ++e) S.cet_size[e] = fdesc[fdesc_of_q[b] + e].total;
+    bool ok = true;
+    if (o.order_mode == GPSM_ORDER_EXPLICIT) {
+      ok = o.explicit_order && order_from_vertices(S.hq, S.cet_size, o.explicit_order, S.plan);
+      if (!ok) S.status = GPSM_EINVAL;
+    } else if (o.order_mode == GPSM_ORDER_LEARNED && c->planner) {
+      std::vector<uint32_t> cand(S.hq.n);
+      for (uint32_t u = 0; u < S.hq.n; ++u) cand[u] = counts[S.vbase + u];
+      PlanContext pc{&S.hq, &st, &cand, &S.cet_size};
+      ok = learned_order(pc, predict_with_ctx, c->planner, S.plan);
+      if (!ok) heuristic_order(S.hq, S.cet_size, st.adj + 1, S.plan);
+      ok = true;
+    } else {
+      heuristic_order(S.hq, S.cet_size, st.adj + 1, S.plan);
+    }
+    if (!ok) {
+      S.active = false;
+      continue;
+    }
+    for (uint32_t p = 0; p < S.hq.n && p < GPSM_MAX_QV; ++p) out[b].order[p] = S.plan.order[p];
+    // any empty forward table means no embedding
+    bool empty = false;
+    for (size_t e = 0; e < E; ++e) empty |= S.cet_size[e] == 0;
+    if (empty) {
+      S.active = false;
+      continue;
+    }
+    JoinPlan jp{};
+    jp.q = S.hq.n;
+    jp.vbase = S.vbase;
+    auto add_desc = [&](uint32_t xq, uint32_t wq, uint32_t edge) {
+      CetDesc d{};
+      d.x_slot = S.vbase + xq;
+      d.w_slot = S.vbase + wq;
+      d.ncand = counts[d.x_slot];
+      d.clist_off = coff[d.x_slot];
+      d.off_base = off_words;
+      d.list_base = list_words;
+      d.list_cap = (uint32_t)S.cet_size[edge];
+      off_words += d.ncand + 1;
+      list_words += S.cet_size[edge];
+      jdesc.push_back(d);
+      return (uint32_t)jdesc.size() - 1;
+    };
+    const OrderPlan& pl = S.plan;
+    jp.first_cet = add_desc(pl.order[0], pl.order[1], pl.first_edge);
+    S.first_desc = jp.first_cet;
+    uint32_t nb = 0;
+    for (uint32_t k = 0; k < S.hq.n; ++k) {
+      jp.order[k] = pl.order[k];
+      jp.back_off[k] = nb;
+      if (k >= 2) {
+        jp.par[k] = pl.par_pos[k];
+        jp.cet[k] = add_desc(pl.order[pl.par_pos[k]], pl.order[k], pl.par_edge[k]);
+        for (uint32_t x : pl.backs[k]) jp.backs[nb++] = (uint8_t)x;
+      }
+    }
+    jp.back_off[S.hq.n] = nb;
+    jp.nback_total = nb;
+    plan_of_q[b] = (uint32_t)plans.size();
+    plans.push_back(jp);
+  }
+  // tasks: slices of the first tables, large enough to amortise a task's
+  // setup, small enough to balance the persistent grid
+  const uint32_t nwg = std::max<uint32_t>(1, c->ncu * wg_per_cu);
+  uint64_t rows_total = 0;
+  for (uint32_t b = 0; b < nq; ++b)
+    if (plan_of_q[b] != ~0u) rows_total += Q[b].cet_size[Q[b].plan.first_edge];
+  const uint64_t tsize =
+      std::min<uint64_t>(std::max<uint64_t>(rows_total / ((uint64_t)nwg * 8) + 1, c->tune.task_min), 1u << 20);
+  for (uint32_t b = 0; b < nq; ++b) {
+    if (plan_of_q[b] == ~0u) continue;
+    const uint64_t R1 = Q[b].cet_size[Q[b].plan.first_edge];
+    for (uint64_t lo = 0; lo < R1; lo += tsize) tasks.push_back({plan_of_q[b], 0, lo, std::min(R1, lo + tsize)});
+  }
+  // largest queries' tasks first is not needed: tasks are uniform in rows
+
+  // ---------------- phase C: candidate-edge tables ----------------
+  if ((rc = upload(c, B_JDESC, jdesc)) || (rc = upload(c, B_PLANS, plans)) || (rc = upload(c, B_TASKS, tasks)))
+    return rc;
+  if ((rc = c->bufs[B_OFFSETS].ensure(sizeof(uint32_t) * std::max<uint64_t>(off_words, 1))) ||
+      (rc = c->bufs[B_LISTS].ensure(sizeof(uint32_t) * std::max<uint64_t>(list_words, 1))) ||
+      (rc = c->bufs[B_CHUNKS].ensure(sizeof(uint32_t) * (jdesc.size() + 2))))
+    return rc;
+  const uint32_t nplans = (uint32_t)plans.size();
+  if ((rc = c->bufs[B_JOUT].ensure(sizeof(JoinOut) * std::max<uint32_t>(nplans, 1)))) return rc;
+  if ((rc = c->bufs[B_CTR].ensure(256))) return rc;
+  const uint64_t lw = join_level_words(qmax, cap);
+  if ((rc = c->bufs[B_LEVELS].ensure(sizeof(uint32_t) * std::max<uint64_t>(lw * nwg, 1)))) return rc;
+  uint64_t emit_cap = 0;
+  if (o.want_tuples && nq == 1) {
+    emit_cap = o.tuple_cap;
+    if ((rc = c->bufs[B_EMIT].ensure(sizeof(uint32_t) * std::max<uint64_t>(emit_cap * qmax, 1)))) return rc;
+  }
+  {
+    bool captured = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (o.use_graph && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) captured = true;
+    launch_cet_count(dg, bitmaps, dptr<uint32_t>(c, B_CLIST), dptr<CetDesc>(c, B_JDESC), (uint32_t)jdesc.size(),
+                     dptr<uint32_t>(c, B_OFFSETS), 1, dptr<uint32_t>(c, B_CHUNKS), cgrid, s);
+    launch_cet_offsets(dptr<CetDesc>(c, B_JDESC), (uint32_t)jdesc.size(), dptr<uint32_t>(c, B_OFFSETS), s);
+    launch_cet_write(dg, bitmaps, dptr<uint32_t>(c, B_CLIST), dptr<CetDesc>(c, B_JDESC), (uint32_t)jdesc.size(),
+                     dptr<uint32_t>(c, B_OFFSETS), dptr<uint32_t>(c, B_LISTS), dptr<uint32_t>(c, B_CHUNKS), cgrid, s);
+    // ---------------- phase D: join ----------------
+    launch_fill_u32(dptr<uint32_t>(c, B_JOUT), 0u, sizeof(JoinOut) / 4 * (uint64_t)std::max<uint32_t>(nplans, 1), s);
+    launch_fill_u32(dptr<uint32_t>(c, B_CTR), 0u, 64, s);
+    JoinArgs ja{};
+    ja.row = dg.row;
+    ja.col = dg.col;
+    ja.bitmaps = bitmaps;
+    ja.bprefix = dptr<uint32_t>(c, B_BPREFIX);
+    ja.clist = dptr<uint32_t>(c, B_CLIST);
+    ja.offsets = dptr<uint32_t>(c, B_OFFSETS);
+    ja.lists = dptr<uint32_t>(c, B_LISTS);
+    ja.cets = dptr<CetDesc>(c, B_JDESC);
+    ja.plans = dptr<JoinPlan>(c, B_PLANS);
+    ja.tasks = dptr<JoinTask>(c, B_TASKS);
+    ja.ntasks = (uint32_t)tasks.size();
+    ja.W = W;
+    ja.NB = NB;
+    ja.cap = cap;
+    ja.qmax = qmax;
+    ja.levels = dptr<uint32_t>(c, B_LEVELS);
+    ja.level_stride = lw;
+    ja.task_counter = dptr<unsigned int>(c, B_CTR);
+    ja.out = dptr<JoinOut>(c, B_JOUT);
+    if (emit_cap) {
+      ja.emit = dptr<uint32_t>(c, B_EMIT);
+      ja.emit_ctr = reinterpret_cast<unsigned long long*>(dptr<uint32_t>(c, B_CTR) + 16);
+      ja.emit_cap = emit_cap;
+    }
+    if (!tasks.empty()) launch_join(ja, std::min<uint32_t>(nwg, (uint32_t)tasks.size()), 256, s);
+    if (captured) {
+      if (hipStreamEndCapture(s, &graph) != hipSuccess || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) !=
+                                                              hipSuccess)
+        return GPSM_EDEVICE;
+      HIPCHK(hipGraphLaunch(exec, s));
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<JoinOut> jo(std::max<uint32_t>(nplans, 1));
+    HIPCHK(hipMemcpyAsync(jo.data(), dptr<JoinOut>(c, B_JOUT), sizeof(JoinOut) * nplans, hipMemcpyDeviceToHost, s));
+    unsigned long long emitted = 0;
+    if (emit_cap)
+      HIPCHK(hipMemcpyAsync(&emitted, dptr<uint32_t>(c, B_CTR) + 16, sizeof(emitted), hipMemcpyDeviceToHost, s));
+    hipEventRecord(c->ev1, s);
+    HIPCHK(hipStreamSynchronize(s));
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    for (uint32_t b = 0; b < nq; ++b) {
+      out[b].status = Q[b].status;
+      out[b].device_ms = ms;
+      if (plan_of_q[b] == ~0u) continue;
+      const JoinOut& r = jo[plan_of_q[b]];
+      out[b].count = r.count;
+      out[b].hash = r.hash;
+      for (uint32_t k = 1; k < Q[b].hq.n && k < GPSM_MAX_QV; ++k) out[b].level_rows[k] = r.level_rows[k];
+    }
+    if (emit_cap && nq == 1 && plan_of_q[0] != ~0u) {
+      const uint32_t qn = Q[0].hq.n;
+      if (emitted > emit_cap) {
+        out[0].status = GPSM_EOVERFLOW;
+      } else {
+        std::vector<uint32_t> tup(emitted * qn);
+        if (emitted)
+          HIPCHK(hipMemcpy(tup.data(), dptr<uint32_t>(c, B_EMIT), sizeof(uint32_t) * tup.size(),
+                           hipMemcpyDeviceToHost));
+        std::vector<uint64_t> idx(emitted);
+        std::iota(idx.begin(), idx.end(), 0);
+        std::sort(idx.begin(), idx.end(), [&](uint64_t x, uint64_t y) {
+          return std::lexicographical_compare(tup.begin() + x * qn, tup.begin() + (x + 1) * qn, tup.begin() + y * qn,
+                                              tup.begin() + (y + 1) * qn);
+        });
+        out[0].tuples = (uint32_t*)std::malloc(sizeof(uint32_t) * std::max<uint64_t>(emitted * qn, 1));
+        if (!out[0].tuples) return GPSM_ENOMEM;
+        for (uint64_t i = 0; i < emitted; ++i)
+          std::memcpy(out[0].tuples + i * qn, tup.data() + idx[i] * qn, sizeof(uint32_t) * qn);
+        out[0].n_tuples = emitted;
+      }
+    }
+  }
+  if (hooks && hooks->after_join) hooks->after_join(hooks->user, c, &Q[0].first_desc, nq);
+  (void)t0;
+  return GPSM_OK;
+}
+
+}  // namespace gpsm
+
+extern "C" {
+
+int gpsm_match(gpsm_ctx* c, const gpsm_graph* g, const gpsm_query* q, const gpsm_opts* opts, gpsm_result* out) {
+  if (!c || !g || !q || !out) return GPSM_EINVAL;
+  int rc = run_batch(c, g, 1, q, opts, out, nullptr);
+  if (rc) return rc;
+  return out->status;
+}
+
+int gpsm_match_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* qs, const gpsm_opts* opts,
+                     gpsm_result* out) {
+  if (!c || !g || (nq && (!qs || !out))) return GPSM_EINVAL;
+  if (nq == 0) return GPSM_OK;
+  gpsm_opts o{};
+  if (opts) o = *opts;
+  o.want_tuples = 0;
+  return run_batch(c, g, nq, qs, &o, out, nullptr);
+}
+
+}  // extern "C"

@@ -1,0 +1,92 @@
+// Internal state behind the opaque C-ABI handles.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "gpsm.h"
+#include "kernels/launch.h"
+#include "plan.h"
+
+namespace gpsm {
+
+// A growable device buffer, reused across calls of one context.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t nbytes);
+  void release();
+};
+
+enum BufId {
+  B_SCRATCH,
+  B_BITMAPS,
+  B_INIT,
+  B_FOPS,
+  B_NBR_SLOT,
+  B_NBR_LO,
+  B_NBR_HI,
+  B_COUNTS,
+  B_COFF,
+  B_CLIST,
+  B_BPREFIX,
+  B_FDESC,
+  B_JDESC,
+  B_CHUNKS,
+  B_OFFSETS,
+  B_LISTS,
+  B_PLANS,
+  B_TASKS,
+  B_JOUT,
+  B_CTR,
+  B_LEVELS,
+  B_EMIT,
+  B_MLP,
+  B_COUNT_
+};
+
+// Launch parameters chosen by the autotuner (defaults until it runs).
+struct Tuning {
+  uint32_t filter_block = 256;   // workgroup size of the filter supersteps
+  uint32_t join_cap = 4096;      // rows per join level buffer
+  uint32_t join_wg_per_cu = 4;   // persistent join workgroups per CU
+  uint32_t task_min = 256;       // minimum first-table rows per join task
+};
+
+struct BatchHooks {
+  void* user = nullptr;
+  // after phase A; bitmaps are device pointers, counts host.  Nonzero stops.
+  int (*after_filter)(void* user, gpsm_ctx* c, const uint32_t* bitmaps, const uint32_t* counts, uint32_t nslots,
+                      uint32_t W) = nullptr;
+  // after the forward |CET| counts (host descs, totals filled).  Nonzero stops.
+  int (*after_cet_count)(void* user, gpsm_ctx* c, const CetDesc* fdesc, uint32_t nfdesc) = nullptr;
+  void (*after_join)(void* user, gpsm_ctx* c, const uint32_t* unused, uint32_t nq) = nullptr;
+};
+
+int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* qs, const gpsm_opts* opts,
+              gpsm_result* out, const BatchHooks* hooks);
+
+// scan.hip
+void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+
+}  // namespace gpsm
+
+struct gpsm_ctx {
+  int device = 0;
+  int ncu = 256;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  char name[256] = {0};
+  gpsm::DevBuf bufs[gpsm::B_COUNT_];
+  gpsm::Tuning tune;
+  gpsm_mlp* planner = nullptr;
+  std::vector<uint32_t> h_counts;
+};
+
+struct gpsm_graph {
+  gpsm_ctx* ctx = nullptr;
+  gpsm::DevGraph dg;
+  gpsm::GraphStats stats;
+  uint64_t bytes = 0;
+};

@@ -1,0 +1,83 @@
+// Device-side data layout shared by the HIP kernels and their host launchers.
+//
+// Data graph (resident in HBM for the life of a gpsm_graph):
+//   row[n+1], col[A]   undirected CSR, rows ascending
+//   key[n]             (dense label << KEY_DEG_BITS) | min(degree, KEY_DEG_MASK):
+//                      one 4-byte gather answers "label == L and degree >= d"
+//                      for d <= KEY_DEG_MASK as lo <= key <= hi.
+// Per batch of queries (one arena, bump-allocated):
+//   bitmaps[slot][W]   candidate set C(u) of every query vertex ("slot" =
+//                      batch-wide vertex index), W = ceil(n/32) words
+//   bprefix[slot][NB]  set bits before each 256-bit block (rank index)
+//   clist[coff[slot]]  ascending ids of C(u)
+//   per directed query edge x->w used by a join: off[|C(x)|+1] and
+//                      list[|CET|], the candidate-edge table
+#pragma once
+#include <cstdint>
+
+namespace gpsm {
+
+constexpr uint32_t KEY_DEG_BITS = 20;
+constexpr uint32_t KEY_DEG_MASK = (1u << KEY_DEG_BITS) - 1;
+constexpr uint32_t MAX_LABELS = (1u << (32 - KEY_DEG_BITS)) - 1;
+constexpr uint32_t MAXQ = 64;
+constexpr uint32_t BLOCK_BITS = 256;  // rank-index block
+
+inline uint32_t words_of(uint32_t n) { return (n + 31) / 32; }
+inline uint32_t nblocks_of(uint32_t n) { return (words_of(n) + 7) / 8 + 1; }
+
+enum FilterOpType : uint32_t { OP_EXPLORE = 1, OP_REFINE = 2 };
+
+// One filter operation of one query in a superstep.
+struct FilterOp {
+  uint32_t type;
+  uint32_t u_slot;   // C(u) iterated
+  uint32_t nv;       // neighbour entries
+  uint32_t nbr_off;  // into nbr_slot / nbr_lo / nbr_hi
+};
+
+// Initial label/degree check of a query's start vertex.
+struct InitOp {
+  uint32_t slot;
+  uint32_t lo, hi;  // key range
+};
+
+// One directed candidate-edge table x -> w.
+struct CetDesc {
+  uint32_t x_slot, w_slot;
+  uint32_t ncand;      // |C(x)|
+  uint32_t list_cap;   // words allocated for the list (offsets are clamped to it)
+  uint64_t clist_off;  // C(x) ids in clist
+  uint64_t off_base;   // offsets [ncand+1] (u32) in the offsets arena
+  uint64_t list_base;  // lists in the list arena
+  uint64_t total;      // |CET|
+};
+
+// Join plan of one query: vertex order and, per level, how to extend.
+struct JoinPlan {
+  uint32_t q;
+  uint32_t vbase;             // slot of query vertex 0
+  uint32_t first_cet;         // CetDesc index of the first table (pi0 -> pi1)
+  uint32_t nback_total;
+  uint32_t order[MAXQ];       // position -> query id
+  uint32_t par[MAXQ];         // parent position of level k (k >= 2)
+  uint32_t cet[MAXQ];         // CetDesc index used to extend level k (k >= 2)
+  uint32_t back_off[MAXQ + 1];  // back positions of level k in backs[]
+  uint8_t backs[MAXQ * MAXQ / 2];  // back positions (excluding the parent)
+};
+
+// A slice of the first table of one query: rows [lo, hi).
+struct JoinTask {
+  uint32_t query;
+  uint32_t pad;
+  uint64_t lo, hi;
+};
+
+// Per-query accumulators written by the join.
+struct JoinOut {
+  unsigned long long count;
+  unsigned long long hash;
+  unsigned long long level_rows[MAXQ];
+};
+
+}  // namespace gpsm

@@ -1,0 +1,99 @@
+// Host-side planning of one query: the reference's decisions that run on the
+// CPU in Match::match (query edge list, spanning-tree filter order, refine
+// schedule, join order), restated for the batched device pipeline.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "gpsm.h"
+
+namespace gpsm {
+
+// Data-graph statistics planning needs: the inverse label list of
+// Graph::transformToCSR (graph/Graph.cpp:67-101) and the sizes.
+struct GraphStats {
+  uint32_t n = 0;
+  uint64_t adj = 0;
+  std::vector<uint32_t> label_value;  // distinct labels, ascending
+  std::vector<uint32_t> label_count;  // vertices per label
+  // dense label id (1-based index into label_value) or 0 when absent
+  uint32_t dense(uint32_t label) const;
+  uint32_t count(uint32_t label) const;
+};
+
+// A query in the reference's representation (Graph of io/IO.cpp:46-100 plus
+// Graph::transformToCSR): in/out lists in file order, sorted undirected CSR.
+struct HostQuery {
+  uint32_t n = 0;
+  std::vector<uint32_t> lab;
+  std::vector<std::vector<uint32_t>> in, out;
+  std::vector<uint32_t> ro, ci;
+  std::vector<uint32_t> ef, et;  // edge list of match/Match.cu:1735-1754 (from < to)
+  uint32_t deg(uint32_t u) const { return ro[u + 1] - ro[u]; }
+  // GPSM_OK, GPSM_EINVAL, GPSM_ETOOBIG or GPSM_EDISCONN
+  int build(const gpsm_query* q);
+};
+
+struct FilterStep {
+  uint32_t type;  // OP_EXPLORE / OP_REFINE
+  uint32_t u;
+  std::vector<uint32_t> v;       // neighbour query ids, in the reference's order
+  std::vector<uint32_t> lo, hi;  // explore: key range of each neighbour
+};
+
+struct FilterSchedule {
+  std::vector<uint32_t> filter_order;
+  uint32_t start = 0, start_lo = 0, start_hi = 0;
+  std::vector<FilterStep> steps;
+};
+
+// Key range [lo, hi] of "label of u and degree >= deg(u)".
+void key_range(const HostQuery& q, const GraphStats& s, uint32_t u, uint32_t& lo, uint32_t& hi);
+
+// generateSpanningTree (match/Match.cu:338-429), InitializeCandidateVertices
+// (:661-719) and RefineCandidateVertices (:848-890) as a list of device steps.
+void make_filter_schedule(const HostQuery& q, const GraphStats& s, FilterSchedule& out);
+
+// A join order: positions -> query ids, and for every level k >= 2 the
+// parent position (the expand edge) and the other earlier neighbours (the
+// join-mode edges, checked as edge membership).
+struct OrderPlan {
+  std::vector<uint32_t> order;
+  uint32_t first_edge = 0;
+  std::vector<uint32_t> par_pos, par_edge;       // [q], valid for k >= 2
+  std::vector<std::vector<uint32_t>> backs;      // [q] earlier positions, parent excluded
+};
+
+// JoinCandidateEdges (match/Match.cu:1599-1700): first edge = smallest
+// forward |CET| (strict <, lowest index); afterwards every edge with both
+// ends bound is joined before the next expand, which takes the smallest
+// |CET| among edges with exactly one bound end (the reference also admits
+// edges with none, a defect fixed here).
+void heuristic_order(const HostQuery& q, const std::vector<uint64_t>& cet_size, uint64_t big, OrderPlan& out);
+
+// Completes a given vertex order (every prefix of length >= 2 connected):
+// the expand edge of a vertex is its smallest-|CET| edge to an earlier one.
+// Returns false if the order is invalid.
+bool order_from_vertices(const HostQuery& q, const std::vector<uint64_t>& cet_size, const uint32_t* order,
+                         OrderPlan& out);
+
+// Fixed-width planner features of extending a prefix by query edge e
+// (parent bound, child unbound).  Defined in plan.cpp next to the order
+// search that uses them.
+struct PlanContext {
+  const HostQuery* q;
+  const GraphStats* s;
+  const std::vector<uint32_t>* cand;       // |C(u)| per query vertex
+  const std::vector<uint64_t>* cet_size;   // |CET| per query edge
+};
+void step_features(const PlanContext& c, const std::vector<uint8_t>& placed, uint32_t nplaced, uint32_t edge,
+                   uint32_t child, double log_rows, float* f);
+void first_features(const PlanContext& c, uint32_t edge, uint32_t from, float* f);
+
+// Greedy search over predicted log cardinalities: first edge and every next
+// vertex minimise the predicted rows of the next level.  predict(x, B, y)
+// evaluates the network on B feature rows.
+typedef int (*PredictFn)(void* user, const float* x, uint32_t B, float* y);
+bool learned_order(const PlanContext& c, PredictFn predict, void* user, OrderPlan& out);
+
+}  // namespace gpsm

@@ -1,0 +1,181 @@
+// Candidate-edge tables on gfx950 (count_kernel + thrust::exclusive_scan +
+// examine_kernel, match/Match.cu:892-1139).  CET(x -> w) lists, for each
+// mu in C(x) ascending, the data neighbours of mu that are in C(w), in
+// adjacency (ascending) order.
+//
+// One wave per 64 candidates of C(x); the wave walks the concatenated
+// adjacency of its candidates 64 entries per step (balanced across hub and
+// leaf candidates) and tests C(w) membership in the bitmap.  The write pass
+// ranks each hit inside its candidate's run with a ballot, so lists stay in
+// adjacency order without a second scan.
+#include "launch.h"
+#include "wave.h"
+
+namespace gpsm {
+
+namespace {
+
+struct CetLds {
+  uint32_t pre[WAVE + 1];
+  uint32_t off[WAVE];
+  uint32_t cnt[WAVE];
+};
+
+__device__ __forceinline__ void load_chunk(const uint32_t* __restrict__ row, const uint32_t* __restrict__ clist,
+                                           const CetDesc& d, uint32_t c0, CetLds& L, bool& mine, uint32_t& D) {
+  const uint32_t lane = lane_id();
+  mine = c0 + lane < d.ncand;
+  const uint32_t id = mine ? clist[d.clist_off + c0 + lane] : 0u;
+  const uint32_t off = mine ? row[id] : 0u;
+  const uint32_t deg = mine ? row[id + 1] - off : 0u;
+  const uint32_t incl = wave_incl_scan(deg);
+  D = uniform(__shfl(incl, WAVE - 1, WAVE));
+  L.pre[lane] = incl - deg;
+  L.off[lane] = off;
+  L.cnt[lane] = 0;
+  wave_sync();
+}
+
+__global__ void __launch_bounds__(256) cet_count_kernel(const uint32_t* __restrict__ row,
+                                                        const uint32_t* __restrict__ col,
+                                                        const uint32_t* __restrict__ bitmaps, uint32_t W,
+                                                        const uint32_t* __restrict__ clist, CetDesc* descs,
+                                                        const uint32_t* __restrict__ chunk_base, uint32_t ndesc,
+                                                        uint32_t* __restrict__ offsets, int per_cand) {
+  __shared__ CetLds lds[4];
+  CetLds& L = lds[threadIdx.x / WAVE];
+  const uint32_t lane = lane_id();
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) / WAVE;
+  const uint32_t nitems = chunk_base[ndesc];
+  for (uint32_t it = wave; it < nitems; it += nwaves) {
+    const uint32_t di = uniform(seg_search(chunk_base, ndesc + 1, it));
+    const CetDesc d = descs[di];
+    const uint32_t c0 = (it - chunk_base[di]) * WAVE;
+    bool mine;
+    uint32_t D;
+    load_chunk(row, clist, d, c0, L, mine, D);
+    const uint32_t* bm_w = bitmaps + (size_t)d.w_slot * W;
+    uint32_t tot = 0;
+    for (uint32_t f = 0; f < D; f += WAVE) {
+      const uint32_t g = f + lane;
+      if (g < D) {
+        const uint32_t ci = seg_search(L.pre, WAVE, g);
+        const uint32_t mv = col[L.off[ci] + (g - L.pre[ci])];
+        if (bit_of(bm_w, mv)) {
+          ++tot;
+          if (per_cand) atomicAdd(&L.cnt[ci], 1u);
+        }
+      }
+    }
+    wave_sync();
+    if (per_cand && mine) offsets[d.off_base + c0 + lane] = L.cnt[lane];
+    tot = wave_sum(tot);
+    if (lane == 0 && tot) atomicAdd(reinterpret_cast<unsigned long long*>(&descs[di].total), (unsigned long long)tot);
+    wave_sync();
+  }
+}
+
+__global__ void __launch_bounds__(256) cet_offsets_kernel(const CetDesc* __restrict__ descs,
+                                                          uint32_t* __restrict__ offsets) {
+  __shared__ uint32_t tmp[8];
+  const CetDesc d = descs[blockIdx.x];
+  uint32_t* o = offsets + d.off_base;
+  uint32_t carry = 0;
+  for (uint32_t t0 = 0; t0 < d.ncand; t0 += blockDim.x) {
+    const uint32_t i = t0 + threadIdx.x;
+    const uint32_t v = i < d.ncand ? o[i] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_excl_scan(v, tmp, &total);
+    if (i < d.ncand) o[i] = min(carry + ex, d.list_cap);
+    carry += total;
+  }
+  if (threadIdx.x == 0) o[d.ncand] = min(carry, d.list_cap);
+}
+
+__global__ void __launch_bounds__(256) cet_write_kernel(const uint32_t* __restrict__ row,
+                                                        const uint32_t* __restrict__ col,
+                                                        const uint32_t* __restrict__ bitmaps, uint32_t W,
+                                                        const uint32_t* __restrict__ clist,
+                                                        const CetDesc* __restrict__ descs,
+                                                        const uint32_t* __restrict__ chunk_base, uint32_t ndesc,
+                                                        const uint32_t* __restrict__ offsets,
+                                                        uint32_t* __restrict__ lists) {
+  __shared__ CetLds lds[4];
+  CetLds& L = lds[threadIdx.x / WAVE];
+  const uint32_t lane = lane_id();
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) / WAVE;
+  const uint32_t nitems = chunk_base[ndesc];
+  for (uint32_t it = wave; it < nitems; it += nwaves) {
+    const uint32_t di = uniform(seg_search(chunk_base, ndesc + 1, it));
+    const CetDesc d = descs[di];
+    const uint32_t c0 = (it - chunk_base[di]) * WAVE;
+    bool mine;
+    uint32_t D;
+    load_chunk(row, clist, d, c0, L, mine, D);
+    const uint32_t* bm_w = bitmaps + (size_t)d.w_slot * W;
+    const uint32_t* o = offsets + d.off_base + c0;
+    uint32_t* out = lists + d.list_base;
+    for (uint32_t f = 0; f < D; f += WAVE) {
+      const uint32_t g = f + lane;
+      bool hit = false;
+      uint32_t ci = 0, mv = 0;
+      if (g < D) {
+        ci = seg_search(L.pre, WAVE, g);
+        mv = col[L.off[ci] + (g - L.pre[ci])];
+        hit = bit_of(bm_w, mv) != 0;
+      }
+      const unsigned long long hm = __ballot(hit);
+      if (hit) {
+        // lanes of this step that belong to candidate ci start at seg_lo
+        const uint32_t seg_lo = L.pre[ci] > f ? L.pre[ci] - f : 0u;
+        const unsigned long long below = (lane ? (~0ULL >> (WAVE - lane)) : 0ULL) & (~0ULL << seg_lo);
+        const uint32_t rank = __popcll(hm & below);
+        const uint32_t pos = o[ci] + L.cnt[ci] + rank;
+        if (pos < d.list_cap) out[pos] = mv;
+      }
+      wave_sync();
+      if (hit) atomicAdd(&L.cnt[ci], 1u);
+      wave_sync();
+    }
+  }
+}
+
+__global__ void chunk_base_kernel(const CetDesc* __restrict__ descs, uint32_t ndesc, uint32_t* __restrict__ base) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (uint32_t d = 0; d < ndesc; ++d) {
+      base[d] = acc;
+      acc += (descs[d].ncand + WAVE - 1) / WAVE;
+    }
+    base[ndesc] = acc;
+  }
+}
+
+}  // namespace
+
+void launch_cet_count(const DevGraph& g, const uint32_t* bitmaps, const uint32_t* clist, CetDesc* descs,
+                      uint32_t ndesc, uint32_t* offsets, int per_cand, uint32_t* chunk_base, uint32_t grid,
+                      hipStream_t s) {
+  if (!ndesc) return;
+  hipLaunchKernelGGL(chunk_base_kernel, dim3(1), dim3(64), 0, s, descs, ndesc, chunk_base);
+  hipLaunchKernelGGL(cet_count_kernel, dim3(grid), dim3(256), 0, s, g.row, g.col, bitmaps, g.W, clist, descs,
+                     chunk_base, ndesc, offsets, per_cand);
+}
+
+void launch_cet_offsets(CetDesc* descs, uint32_t ndesc, uint32_t* offsets, hipStream_t s) {
+  if (!ndesc) return;
+  hipLaunchKernelGGL(cet_offsets_kernel, dim3(ndesc), dim3(256), 0, s, descs, offsets);
+}
+
+void launch_cet_write(const DevGraph& g, const uint32_t* bitmaps, const uint32_t* clist, const CetDesc* descs,
+                      uint32_t ndesc, const uint32_t* offsets, uint32_t* lists, uint32_t* chunk_base, uint32_t grid,
+                      hipStream_t s) {
+  if (!ndesc) return;
+  hipLaunchKernelGGL(chunk_base_kernel, dim3(1), dim3(64), 0, s, descs, ndesc, chunk_base);
+  hipLaunchKernelGGL(cet_write_kernel, dim3(grid), dim3(256), 0, s, g.row, g.col, bitmaps, g.W, clist, descs,
+                     chunk_base, ndesc, offsets, lists);
+}
+
+}  // namespace gpsm

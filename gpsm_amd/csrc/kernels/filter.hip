@@ -1,0 +1,197 @@
+// Candidate-vertex filtering on gfx950: the start vertex's label/degree check
+// (check_kernel, match/Match.cu:431-461), exploration (explore_kernel,
+// :528-659) and neighbourhood refinement (refine_kernel, :761-846), for any
+// number of queries per launch.
+//
+// Layout: candidate sets are bitmaps (1 bit per data vertex, 32-bit words);
+// the reference used one byte per (query vertex, data vertex).
+// Parallel decomposition: one wave per 64-word bitmap chunk (2048 data
+// vertices) of one op.  The wave compacts the chunk's candidates into LDS
+// (the reference's kernel_collect, fused), then walks the concatenated
+// adjacency lists of 64 candidates at a time, 64 adjacency entries per step
+// (lane t takes entry t; a binary search in the 64 degree prefix sums finds its
+// candidate), so hub and leaf candidates share lanes evenly instead of one
+// 32-lane warp per candidate.
+#include "launch.h"
+#include "wave.h"
+
+namespace gpsm {
+
+namespace {
+
+constexpr int CHUNK_WORDS = 64;  // bitmap words per work item (one per lane)
+constexpr int MAX_CAND = CHUNK_WORDS * 32;
+
+__global__ void init_kernel(const uint32_t* __restrict__ key, uint32_t n, uint32_t W, const InitOp* __restrict__ ops,
+                            uint32_t nops, uint32_t* __restrict__ bitmaps) {
+  const uint32_t pairs = (W + 1) / 2;
+  const uint64_t items = (uint64_t)nops * pairs;
+  const uint32_t lane = lane_id();
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
+  for (uint64_t it = wave; it < items; it += nwaves) {
+    const uint32_t o = (uint32_t)(it / pairs), p = (uint32_t)(it % pairs);
+    const InitOp op = ops[o];
+    const uint32_t v = p * 64 + lane;
+    bool pred = false;
+    if (v < n) {
+      uint32_t k = key[v];
+      pred = k >= op.lo && k <= op.hi;
+    }
+    const unsigned long long m = __ballot(pred);
+    uint32_t* bm = bitmaps + (size_t)op.slot * W;
+    if (lane == 0) bm[2 * p] = (uint32_t)m;
+    if (lane == 1 && 2 * p + 1 < W) bm[2 * p + 1] = (uint32_t)(m >> 32);
+  }
+}
+
+struct WaveLds {
+  uint32_t cand[MAX_CAND];
+  uint32_t pre[WAVE + 1];
+  uint32_t off[WAVE];
+  unsigned long long hit[WAVE];
+  uint32_t lim[WAVE];
+};
+
+__global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __restrict__ row,
+                                                          const uint32_t* __restrict__ col,
+                                                          const uint32_t* __restrict__ key, uint32_t W,
+                                                          const FilterOp* __restrict__ ops, uint32_t nops,
+                                                          const uint32_t* __restrict__ nbr_slot,
+                                                          const uint32_t* __restrict__ nbr_lo,
+                                                          const uint32_t* __restrict__ nbr_hi,
+                                                          uint32_t* __restrict__ bitmaps) {
+  extern __shared__ unsigned char lds_raw[];
+  WaveLds& L = reinterpret_cast<WaveLds*>(lds_raw)[threadIdx.x / WAVE];
+  const uint32_t lane = lane_id();
+  const uint32_t chunks = (W + CHUNK_WORDS - 1) / CHUNK_WORDS;
+  const uint64_t items = (uint64_t)nops * chunks;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
+
+  for (uint64_t it = wave; it < items; it += nwaves) {
+    const uint32_t o = uniform((uint32_t)(it / chunks));
+    const uint32_t c = uniform((uint32_t)(it % chunks));
+    const uint32_t type = uniform(ops[o].type);
+    const uint32_t nv = uniform(ops[o].nv);
+    const uint32_t nb = uniform(ops[o].nbr_off);
+    uint32_t* bm_u = bitmaps + (size_t)uniform(ops[o].u_slot) * W;
+    const unsigned long long full = nv >= 64 ? ~0ULL : ((1ULL << nv) - 1);
+
+    // collect: candidates of this chunk, ascending
+    const uint32_t wi = c * CHUNK_WORDS + lane;
+    const uint32_t w = wi < W ? bm_u[wi] : 0u;
+    const uint32_t pc = __popc(w);
+    const uint32_t incl = wave_incl_scan(pc);
+    const uint32_t C = uniform(__shfl(incl, WAVE - 1, WAVE));
+    {
+      uint32_t pos = incl - pc, x = w;
+      while (x) {
+        uint32_t b = __ffs(x) - 1;
+        x &= x - 1;
+        L.cand[pos++] = wi * 32 + b;
+      }
+    }
+    wave_sync();
+
+    for (uint32_t s0 = 0; s0 < C; s0 += WAVE) {
+      const bool mine = s0 + lane < C;
+      const uint32_t id = mine ? L.cand[s0 + lane] : 0u;
+      const uint32_t off = mine ? row[id] : 0u;
+      const uint32_t deg = mine ? row[id + 1] - off : 0u;
+      const uint32_t dincl = wave_incl_scan(deg);
+      const uint32_t D = uniform(__shfl(dincl, WAVE - 1, WAVE));
+      L.pre[lane] = dincl - deg;
+      L.off[lane] = off;
+      L.hit[lane] = 0;
+      wave_sync();
+
+      // pass 1: which neighbour constraints each candidate satisfies
+      for (uint32_t f = 0; f < D; f += WAVE) {
+        const uint32_t g = f + lane;
+        if (g < D) {
+          const uint32_t ci = seg_search(L.pre, WAVE, g);
+          const uint32_t mv = col[L.off[ci] + (g - L.pre[ci])];
+          unsigned long long bits = 0;
+          if (type == OP_REFINE) {
+            const unsigned long long known = L.hit[ci];
+            for (uint32_t i = 0; i < nv; ++i) {
+              if ((known >> i) & 1ULL) continue;
+              const uint32_t* bm_v = bitmaps + (size_t)nbr_slot[nb + i] * W;
+              if (bit_of(bm_v, mv)) bits |= 1ULL << i;
+            }
+          } else {
+            const uint32_t k = key[mv];
+            for (uint32_t i = 0; i < nv; ++i)
+              if (k >= nbr_lo[nb + i] && k <= nbr_hi[nb + i]) bits |= 1ULL << i;
+          }
+          if (bits) atomicOr(&L.hit[ci], bits);
+        }
+        wave_sync();
+        if (type == OP_REFINE) {
+          // stop once every candidate of the sub-chunk has all its neighbours
+          const bool done = !mine || L.hit[lane] == full;
+          if (__ballot(!done) == 0ULL) break;
+        }
+      }
+      wave_sync();
+      const unsigned long long h = L.hit[lane];
+      if (type == OP_REFINE) {
+        if (mine && h != full) clear_bit(bm_u, id);
+      } else {
+        // first neighbour without a match: mu leaves C(u); neighbours before
+        // it received all of mu's matching data neighbours (match/Match.cu:547-600)
+        const uint32_t ff = h == full ? nv : (uint32_t)(__ffsll((long long)~h) - 1);
+        if (mine && ff < nv) clear_bit(bm_u, id);
+        L.lim[lane] = mine ? ff : 0u;
+        const bool any = __ballot(mine && ff > 0) != 0ULL;
+        wave_sync();
+        if (any) {
+          for (uint32_t f = 0; f < D; f += WAVE) {
+            const uint32_t g = f + lane;
+            if (g < D) {
+              const uint32_t ci = seg_search(L.pre, WAVE, g);
+              const uint32_t lim = L.lim[ci];
+              if (lim) {
+                const uint32_t mv = col[L.off[ci] + (g - L.pre[ci])];
+                const uint32_t k = key[mv];
+                for (uint32_t i = 0; i < lim; ++i)
+                  if (k >= nbr_lo[nb + i] && k <= nbr_hi[nb + i])
+                    set_bit(bitmaps + (size_t)nbr_slot[nb + i] * W, mv);
+              }
+            }
+          }
+        }
+      }
+      wave_sync();
+    }
+  }
+}
+
+}  // namespace
+
+void launch_init(const DevGraph& g, const InitOp* ops, uint32_t nops, uint32_t* bitmaps, hipStream_t s) {
+  if (!nops) return;
+  const uint64_t items = (uint64_t)nops * ((g.W + 1) / 2);
+  const uint32_t block = 256;
+  uint64_t grid = (items * WAVE + block - 1) / block;
+  if (grid > 65536) grid = 65536;
+  hipLaunchKernelGGL(init_kernel, dim3((uint32_t)grid), dim3(block), 0, s, g.key, g.n, g.W, ops, nops, bitmaps);
+}
+
+void launch_filter_step(const DevGraph& g, const FilterOp* ops, uint32_t nops, const uint32_t* nbr_slot,
+                        const uint32_t* nbr_lo, const uint32_t* nbr_hi, uint32_t* bitmaps, uint32_t block,
+                        hipStream_t s) {
+  if (!nops) return;
+  if (block != 64 && block != 128 && block != 256) block = 256;
+  const uint32_t chunks = (g.W + CHUNK_WORDS - 1) / CHUNK_WORDS;
+  const uint64_t items = (uint64_t)nops * chunks;
+  const uint32_t wpb = block / WAVE;
+  uint64_t grid = (items + wpb - 1) / wpb;
+  if (grid > 65536) grid = 65536;
+  const size_t lds = sizeof(WaveLds) * wpb;
+  hipLaunchKernelGGL(filter_step_kernel, dim3((uint32_t)grid), dim3(block), lds, s, g.row, g.col, g.key, g.W, ops,
+                     nops, nbr_slot, nbr_lo, nbr_hi, bitmaps);
+}
+
+}  // namespace gpsm

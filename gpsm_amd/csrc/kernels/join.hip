@@ -1,0 +1,284 @@
+// Join of candidate-edge tables on gfx950 (table_kernel, join_kernel,
+// expand_kernel, link_kernel, filter_kernel; match/Match.cu:1141-1569).
+//
+// The reference materialises every intermediate table breadth-first, finds
+// a row's vertex in the candidate list with a warp-linear scan (quadratic in
+// rows x candidates), and leaves injectivity to the host.  Here:
+//  * a persistent grid pulls tasks (a slice of one query's first table) from
+//    a device counter, so hundreds of uneven queries share one launch;
+//  * each workgroup walks its slice depth-first through levels k = 1..q-1
+//    (k+1 bound vertices) with one bounded column-major buffer per level, so
+//    memory is O(cap * q^2) per workgroup whatever the result size;
+//  * a row's candidate list is found in O(1) through the rank index of C(x)
+//    (set bits before each 256-bit block + popcounts);
+//  * a window of 256 rows is expanded flattened: 256 threads take 256
+//    consecutive (row, candidate) pairs, so rows of any fan-out share lanes;
+//  * join-mode edges become edge-membership checks (binary search in the
+//    shorter adjacency) and injectivity is checked in the same pass;
+//  * the last level is never written: each embedding adds its tuple hash
+//    (additive over vertices, one fmix64 per emit) and 1 to the counters.
+#include "launch.h"
+#include "wave.h"
+
+namespace gpsm {
+
+namespace {
+
+constexpr uint32_t WROWS = 256;  // rows per window = threads per workgroup
+
+__device__ __forceinline__ uint32_t rank_of(const uint32_t* __restrict__ bm, const uint32_t* __restrict__ bp,
+                                            uint32_t x) {
+  const uint32_t blk = x >> 8;
+  uint32_t r = bp[blk];
+  const uint32_t w0 = blk * 8, wx = x >> 5;
+  for (uint32_t w = w0; w < wx; ++w) r += __popc(bm[w]);
+  return r + __popc(bm[wx] & ((1u << (x & 31)) - 1u));
+}
+
+__device__ __forceinline__ bool has_edge(const uint32_t* __restrict__ row, const uint32_t* __restrict__ col,
+                                         uint32_t a, uint32_t b) {
+  uint32_t lo = row[a], hi = row[a + 1];
+  const uint32_t lo2 = row[b], hi2 = row[b + 1];
+  uint32_t key = b;
+  if (hi2 - lo2 < hi - lo) {
+    lo = lo2;
+    hi = hi2;
+    key = a;
+  }
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    const uint32_t v = col[mid];
+    if (v == key) return true;
+    if (v < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return false;
+}
+
+struct JoinShared {
+  uint32_t pre[WROWS + 1];
+  uint32_t start[WROWS];
+  unsigned long long srow[WROWS];
+  uint32_t tmp[8];
+  uint32_t fill[MAXQ], cur_row[MAXQ], cur_cand[MAXQ];
+  unsigned long long lrows[MAXQ];
+  uint32_t task, k;
+  unsigned long long red[4];
+  // followed by the row window: val[c * WROWS + r], c < qmax
+};
+
+__global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
+  extern __shared__ unsigned char lds_raw[];
+  JoinShared& S = *reinterpret_cast<JoinShared*>(lds_raw);
+  uint32_t* val = reinterpret_cast<uint32_t*>(lds_raw + ((sizeof(JoinShared) + 15) & ~(size_t)15));
+  const uint32_t tid = threadIdx.x;
+  uint32_t* lvl_base = a.levels + (uint64_t)blockIdx.x * a.level_stride;
+
+  for (;;) {
+    if (tid == 0) S.task = atomicAdd(a.task_counter, 1u);
+    __syncthreads();
+    const uint32_t t = S.task;
+    if (t >= a.ntasks) return;
+    const JoinTask task = a.tasks[t];
+    const JoinPlan& P = a.plans[task.query];
+    const uint32_t q = P.q;
+    const CetDesc d0 = a.cets[P.first_cet];
+    const uint32_t* off0 = a.offsets + d0.off_base;
+    const uint32_t* cl0 = a.clist + d0.clist_off;
+    const uint32_t* ls0 = a.lists + d0.list_base;
+    unsigned long long my_cnt = 0, my_hash = 0;
+
+    if (tid < MAXQ) {
+      S.lrows[tid] = 0;
+      S.fill[tid] = 0;
+      S.cur_row[tid] = 0;
+      S.cur_cand[tid] = 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      S.fill[1] = (uint32_t)(task.hi - task.lo);
+      S.lrows[1] = task.hi - task.lo;
+      S.k = 1;
+    }
+    __syncthreads();
+
+    if (q == 2) {
+      // the first table is the embedding set
+      for (uint64_t e = task.lo + tid; e < task.hi; e += blockDim.x) {
+        const uint32_t j = seg_search(off0, d0.ncand + 1, (uint32_t)e);
+        const uint32_t mu = cl0[j], mv = ls0[e];
+        my_hash += fmix64(tuple_term(P.order[0], mu) + tuple_term(P.order[1], mv));
+        ++my_cnt;
+        if (a.emit) {
+          const unsigned long long pos = atomicAdd(a.emit_ctr, 1ULL);
+          if (pos < a.emit_cap) {
+            a.emit[pos * 2 + P.order[0]] = mu;
+            a.emit[pos * 2 + P.order[1]] = mv;
+          }
+        }
+      }
+    } else {
+      for (;;) {
+        const uint32_t k = S.k;
+        if (S.cur_row[k] >= S.fill[k]) {
+          __syncthreads();
+          if (k == 1) break;
+          if (tid == 0) S.k = k - 1;
+          __syncthreads();
+          continue;
+        }
+        const uint32_t r0 = S.cur_row[k];
+        const uint32_t nw = min(WROWS, S.fill[k] - r0);
+        const uint32_t nk = k + 1;              // level being produced
+        const bool final = nk == q - 1;
+        const uint32_t p = P.par[nk];
+        const CetDesc dn = a.cets[P.cet[nk]];
+        const uint32_t* bm_x = a.bitmaps + (size_t)dn.x_slot * a.W;
+        const uint32_t* bp_x = a.bprefix + (size_t)dn.x_slot * a.NB;
+        const uint32_t* offn = a.offsets + dn.off_base;
+        const uint32_t bo = P.back_off[nk], be = P.back_off[nk + 1];
+        const uint32_t wn = P.order[nk];
+        const uint32_t cap = a.cap;
+        // level j (j >= 2, j+1 columns) starts at cap * (j(j+1)/2 - 3)
+        const uint32_t* Lk = k >= 2 ? lvl_base + (uint64_t)cap * ((k * (k + 1)) / 2 - 3) : nullptr;
+        uint32_t* Ln = final ? nullptr : lvl_base + (uint64_t)cap * ((nk * (nk + 1)) / 2 - 3);
+
+        // load the window and each row's candidate range
+        uint32_t ub = 0;
+        if (tid < nw) {
+          const uint32_t r = r0 + tid;
+          if (k == 1) {
+            const uint64_t e = task.lo + r;
+            const uint32_t j = seg_search(off0, d0.ncand + 1, (uint32_t)e);
+            val[0 * WROWS + tid] = cl0[j];
+            val[1 * WROWS + tid] = ls0[e];
+          } else {
+            for (uint32_t c = 0; c <= k; ++c) val[c * WROWS + tid] = Lk[(uint64_t)c * cap + r];
+          }
+          unsigned long long s = 0;
+          for (uint32_t c = 0; c <= k; ++c) s += tuple_term(P.order[c], val[c * WROWS + tid]);
+          S.srow[tid] = s;
+          const uint32_t j = rank_of(bm_x, bp_x, val[p * WROWS + tid]);
+          uint32_t st = offn[j];
+          ub = offn[j + 1] - st;
+          if (tid == 0) {
+            ub -= S.cur_cand[k];
+            st += S.cur_cand[k];
+          }
+          S.start[tid] = st;
+        }
+        uint32_t T;
+        const uint32_t ex = block_excl_scan(ub, S.tmp, &T);
+        S.pre[tid] = tid < nw ? ex : T;
+        if (tid == 0 && !final) S.fill[nk] = 0;
+        __syncthreads();
+
+        const uint32_t* lst = a.lists + dn.list_base;
+        uint32_t f = 0;
+        for (; f < T; f += WROWS) {
+          if (!final && S.fill[nk] + WROWS > cap) break;  // uniform: fill is shared
+          const uint32_t g = f + tid;
+          bool ok = false;
+          uint32_t i = 0, mv = 0;
+          if (g < T) {
+            i = seg_search(S.pre, nw, g);
+            mv = lst[S.start[i] + (g - S.pre[i])];
+            ok = true;
+            for (uint32_t c = 0; c <= k && ok; ++c) ok = val[c * WROWS + i] != mv;
+            for (uint32_t b = bo; b < be && ok; ++b) ok = has_edge(a.row, a.col, mv, val[P.backs[b] * WROWS + i]);
+          }
+          if (final) {
+            if (ok) {
+              ++my_cnt;
+              my_hash += fmix64(S.srow[i] + tuple_term(wn, mv));
+              if (a.emit) {
+                const unsigned long long pos = atomicAdd(a.emit_ctr, 1ULL);
+                if (pos < a.emit_cap) {
+                  uint32_t* t = a.emit + pos * q;
+                  for (uint32_t c = 0; c <= k; ++c) t[P.order[c]] = val[c * WROWS + i];
+                  t[wn] = mv;
+                }
+              }
+            }
+          } else {
+            uint32_t tot;
+            const uint32_t pos = block_excl_scan(ok ? 1u : 0u, S.tmp, &tot);
+            const uint32_t base = S.fill[nk];
+            if (ok) {
+              const uint64_t r = base + pos;
+              for (uint32_t c = 0; c <= k; ++c) Ln[(uint64_t)c * cap + r] = val[c * WROWS + i];
+              Ln[(uint64_t)nk * cap + r] = mv;
+            }
+            __syncthreads();
+            if (tid == 0) {
+              S.fill[nk] = base + tot;
+              S.lrows[nk] += tot;
+            }
+            __syncthreads();
+          }
+        }
+        // advance level k's cursor past what was expanded
+        if (tid == 0) {
+          if (f >= T) {
+            S.cur_row[k] = r0 + nw;
+            S.cur_cand[k] = 0;
+          } else {
+            const uint32_t i = seg_search(S.pre, nw, f);
+            const uint32_t extra = (i == 0) ? S.cur_cand[k] : 0u;
+            S.cur_row[k] = r0 + i;
+            S.cur_cand[k] = extra + (f - S.pre[i]);
+          }
+          if (!final && S.fill[nk] > 0) {
+            S.k = nk;
+            S.cur_row[nk] = 0;
+            S.cur_cand[nk] = 0;
+          }
+        }
+        __syncthreads();
+      }
+    }
+
+    // flush this task's counters
+    my_cnt = wave_sum64(my_cnt);
+    my_hash = wave_sum64(my_hash);
+    __syncthreads();
+    if (lane_id() == 0) S.red[tid / WAVE] = my_cnt;
+    __syncthreads();
+    if (tid == 0) {
+      unsigned long long c = 0;
+      for (uint32_t w = 0; w < blockDim.x / WAVE; ++w) c += S.red[w];
+      S.lrows[q - 1] = (q == 2) ? c : S.lrows[q - 1] + c;
+      if (c) atomicAdd(&a.out[task.query].count, c);
+    }
+    __syncthreads();
+    if (lane_id() == 0) S.red[tid / WAVE] = my_hash;
+    __syncthreads();
+    if (tid == 0) {
+      unsigned long long h = 0;
+      for (uint32_t w = 0; w < blockDim.x / WAVE; ++w) h += S.red[w];
+      atomicAdd(&a.out[task.query].hash, h);
+    }
+    if (tid >= 1 && tid < q && S.lrows[tid]) atomicAdd(&a.out[task.query].level_rows[tid], S.lrows[tid]);
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+uint64_t join_level_words(uint32_t qmax, uint32_t cap) {
+  // levels k = 2 .. qmax-2 hold k+1 columns; level k starts at cap*(k(k+1)/2 - 3)
+  if (qmax < 4) return 0;
+  const uint64_t k = qmax - 1;  // one past the last materialised level
+  return (uint64_t)cap * (k * (k + 1) / 2 - 3);
+}
+
+void launch_join(const JoinArgs& a, uint32_t nwg, uint32_t block, hipStream_t s) {
+  (void)block;
+  const size_t lds = ((sizeof(JoinShared) + 15) & ~(size_t)15) + sizeof(uint32_t) * WROWS * a.qmax;
+  if (lds > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(join_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+  hipLaunchKernelGGL(join_kernel, dim3(nwg), dim3(WROWS), lds, s, a);
+}
+
+}  // namespace gpsm

@@ -1,0 +1,86 @@
+// Host launchers of the device kernels (one translation unit per phase).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "gpsm_dev.h"
+
+namespace gpsm {
+
+struct DevGraph {
+  uint32_t n = 0;
+  uint64_t adj = 0;
+  uint32_t W = 0;   // bitmap words per query vertex
+  uint32_t NB = 0;  // rank-index entries per query vertex
+  uint32_t* row = nullptr;
+  uint32_t* col = nullptr;
+  uint32_t* key = nullptr;
+};
+
+// filter.hip ------------------------------------------------------------------
+// Start-vertex candidates: C(slot) = {v : lo <= key[v] <= hi} (check_kernel).
+void launch_init(const DevGraph& g, const InitOp* ops, uint32_t nops, uint32_t* bitmaps, hipStream_t s);
+// One superstep: every op (one per query) explores or refines its C(u).
+void launch_filter_step(const DevGraph& g, const FilterOp* ops, uint32_t nops, const uint32_t* nbr_slot,
+                        const uint32_t* nbr_lo, const uint32_t* nbr_hi, uint32_t* bitmaps, uint32_t block,
+                        hipStream_t s);
+
+// scan.hip ---------------------------------------------------------------------
+// counts[r] = popcount of bitmap row rows[r] (all rows when rows == nullptr).
+void launch_row_popcount(const DevGraph& g, const uint32_t* bitmaps, uint32_t nrows, uint32_t* counts,
+                         hipStream_t s);
+// Compaction of bitmap rows to ascending id lists plus the rank index.
+void launch_compact_rows(const DevGraph& g, const uint32_t* bitmaps, uint32_t nrows, const uint64_t* coff,
+                         uint32_t* clist, uint32_t* bprefix, hipStream_t s);
+// Exclusive scan of n u32 values; out may alias in.  total (device, u64) optional.
+void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, unsigned long long* total,
+                           uint32_t* scratch, uint64_t scratch_words, hipStream_t s);
+uint64_t scan_scratch_words(uint64_t n);
+// Ascending ids of set bits of one bitmap; count written to *count (device u64).
+void launch_compact_bitmap(const uint32_t* bits, uint64_t nbits, uint32_t* ids, unsigned long long* count,
+                           uint32_t* scratch, hipStream_t s);
+
+// cet.hip ----------------------------------------------------------------------
+// Per CetDesc: |CET| into desc.total (atomic); per-candidate counts into
+// offsets (when per_cand != 0) at desc.off_base.
+// chunk_base: device scratch of ndesc+1 words; grid: workgroups (persistent).
+void launch_cet_count(const DevGraph& g, const uint32_t* bitmaps, const uint32_t* clist, CetDesc* descs,
+                      uint32_t ndesc, uint32_t* offsets, int per_cand, uint32_t* chunk_base, uint32_t grid,
+                      hipStream_t s);
+// Segmented exclusive scan of the per-candidate counts of every desc.
+void launch_cet_offsets(CetDesc* descs, uint32_t ndesc, uint32_t* offsets, hipStream_t s);
+// Writes the lists.
+void launch_cet_write(const DevGraph& g, const uint32_t* bitmaps, const uint32_t* clist, const CetDesc* descs,
+                      uint32_t ndesc, const uint32_t* offsets, uint32_t* lists, uint32_t* chunk_base, uint32_t grid,
+                      hipStream_t s);
+
+// join.hip ---------------------------------------------------------------------
+struct JoinArgs {
+  const uint32_t* row;
+  const uint32_t* col;
+  const uint32_t* bitmaps;
+  const uint32_t* bprefix;
+  const uint32_t* clist;
+  const uint32_t* offsets;
+  const uint32_t* lists;
+  const CetDesc* cets;
+  const JoinPlan* plans;
+  const JoinTask* tasks;
+  uint32_t ntasks;
+  uint32_t W, NB;
+  uint32_t cap;       // rows per level buffer
+  uint32_t qmax;      // widest query of the batch
+  uint32_t* levels;   // per workgroup: (qmax-2) buffers of cap rows, column-major
+  uint64_t level_stride;  // words per workgroup
+  unsigned int* task_counter;
+  JoinOut* out;
+  // optional tuple output (query-id order), emit_cap tuples of q words
+  uint32_t* emit;
+  unsigned long long* emit_ctr;
+  uint64_t emit_cap;
+};
+void launch_join(const JoinArgs& a, uint32_t nwg, uint32_t block, hipStream_t s);
+uint64_t join_level_words(uint32_t qmax, uint32_t cap);
+
+}  // namespace gpsm

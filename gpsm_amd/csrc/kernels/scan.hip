@@ -1,0 +1,171 @@
+// Scan and compaction on gfx950: the reference's kernel_collect
+// (transform_kernel + thrust::exclusive_scan + scatter_kernel,
+// match/Match.cu:463-526) and its unbuilt Blelloch prefix scan / stream
+// compaction (util/prefix_scan.cu:158-427, util/compact_scan.cu:7-70).
+//
+// Candidate sets are bitmaps here, so "collect" is a popcount scan: a
+// workgroup walks one bitmap row in tiles of 256 words, each thread
+// popcounts one word, a wave64 shuffle scan plus a per-wave LDS carry gives
+// every word its output position, ids are staged in LDS and written back
+// coalesced.  The same pass writes the rank index (set bits before each
+// 256-bit block) the join uses to find a vertex's candidate-edge list.
+#include "launch.h"
+#include "wave.h"
+
+namespace gpsm {
+
+namespace {
+
+constexpr uint32_t TILE_WORDS = 256;  // one word per thread
+
+__global__ void __launch_bounds__(256) row_popcount_kernel(const uint32_t* __restrict__ bitmaps, uint32_t W,
+                                                           uint32_t* __restrict__ counts) {
+  __shared__ uint32_t part[4];
+  const uint32_t* bm = bitmaps + (size_t)blockIdx.x * W;
+  uint32_t c = 0;
+  for (uint32_t i = threadIdx.x; i < W; i += blockDim.x) c += __popc(bm[i]);
+  c = wave_sum(c);
+  if (lane_id() == 0) part[threadIdx.x / WAVE] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) counts[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+__global__ void __launch_bounds__(256) compact_rows_kernel(const uint32_t* __restrict__ bitmaps, uint32_t W,
+                                                           uint32_t NB, const uint64_t* __restrict__ coff,
+                                                           uint32_t* __restrict__ clist,
+                                                           uint32_t* __restrict__ bprefix) {
+  __shared__ uint32_t stage[TILE_WORDS * 32];
+  __shared__ uint32_t tmp[8];
+  const uint32_t r = blockIdx.x;
+  const uint32_t* bm = bitmaps + (size_t)r * W;
+  uint32_t* bp = bprefix + (size_t)r * NB;
+  uint32_t* out = clist + coff[r];
+  uint32_t carry = 0;
+  for (uint32_t t0 = 0; t0 < W; t0 += TILE_WORDS) {
+    const uint32_t wi = t0 + threadIdx.x;
+    const uint32_t w = wi < W ? bm[wi] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_excl_scan(__popc(w), tmp, &total);
+    if ((wi & 7) == 0 && wi < W) bp[wi >> 3] = carry + ex;
+    uint32_t pos = ex, x = w;
+    while (x) {
+      const uint32_t b = __ffs(x) - 1;
+      x &= x - 1;
+      stage[pos++] = wi * 32 + b;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) out[carry + i] = stage[i];
+    carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bp[NB - 1] = carry;
+}
+
+// Reduce-then-scan for generic u32 arrays (wrapping u32 values, u64 total).
+constexpr uint32_t SCAN_ITEMS = 16;
+constexpr uint32_t SCAN_TILE = 256 * SCAN_ITEMS;
+
+__global__ void __launch_bounds__(256) scan_reduce_kernel(const uint32_t* __restrict__ in, uint64_t n,
+                                                          unsigned long long* __restrict__ sums) {
+  __shared__ unsigned long long part[4];
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+  unsigned long long s = 0;
+  for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
+    const uint64_t i = base + (uint64_t)j * 256 + threadIdx.x;
+    if (i < n) s += in[i];
+  }
+  s = wave_sum64(s);
+  if (lane_id() == 0) part[threadIdx.x / WAVE] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) sums[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+__global__ void scan_sums_kernel(unsigned long long* __restrict__ sums, uint32_t nb,
+                                 unsigned long long* __restrict__ total) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    unsigned long long acc = 0;
+    for (uint32_t i = 0; i < nb; ++i) {
+      unsigned long long t = sums[i];
+      sums[i] = acc;
+      acc += t;
+    }
+    if (total) *total = acc;
+  }
+}
+
+__global__ void __launch_bounds__(256) scan_tile_kernel(const uint32_t* in, uint32_t* out, uint64_t n,
+                                                        const unsigned long long* __restrict__ sums) {
+  __shared__ uint32_t tmp[8];
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  uint32_t s = 0;
+  for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
+    const uint64_t i = base + j;
+    v[j] = i < n ? in[i] : 0u;
+    s += v[j];
+  }
+  uint32_t total;
+  uint32_t ex = block_excl_scan(s, tmp, &total) + (uint32_t)sums[blockIdx.x];
+  for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
+    const uint64_t i = base + j;
+    if (i < n) out[i] = ex;
+    ex += v[j];
+  }
+}
+
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+}  // namespace
+
+// Fill kernel instead of hipMemsetAsync so that captured graphs hold only
+// kernel nodes.
+void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  uint64_t grid = (n + 255) / 256;
+  if (grid > 8192) grid = 8192;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)grid), dim3(256), 0, s, p, v, n);
+}
+
+void launch_row_popcount(const DevGraph& g, const uint32_t* bitmaps, uint32_t nrows, uint32_t* counts,
+                         hipStream_t s) {
+  if (!nrows) return;
+  hipLaunchKernelGGL(row_popcount_kernel, dim3(nrows), dim3(256), 0, s, bitmaps, g.W, counts);
+}
+
+void launch_compact_rows(const DevGraph& g, const uint32_t* bitmaps, uint32_t nrows, const uint64_t* coff,
+                         uint32_t* clist, uint32_t* bprefix, hipStream_t s) {
+  if (!nrows) return;
+  hipLaunchKernelGGL(compact_rows_kernel, dim3(nrows), dim3(256), 0, s, bitmaps, g.W, g.NB, coff, clist, bprefix);
+}
+
+uint64_t scan_scratch_words(uint64_t n) { return 2 * ((n + SCAN_TILE - 1) / SCAN_TILE) + 2; }
+
+void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, unsigned long long* total,
+                           uint32_t* scratch, uint64_t scratch_words, hipStream_t s) {
+  const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (nb == 0 || scratch_words < 2 * nb) return;
+  unsigned long long* sums = reinterpret_cast<unsigned long long*>(scratch);
+  hipLaunchKernelGGL(scan_reduce_kernel, dim3((uint32_t)nb), dim3(256), 0, s, in, n, sums);
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(64), 0, s, sums, (uint32_t)nb, total);
+  hipLaunchKernelGGL(scan_tile_kernel, dim3((uint32_t)nb), dim3(256), 0, s, in, out, n, sums);
+}
+
+void launch_compact_bitmap(const uint32_t* bits, uint64_t nbits, uint32_t* ids, unsigned long long* count,
+                           uint32_t* scratch, hipStream_t s) {
+  // one row of ceil(nbits/32) words through the row compaction; scratch
+  // holds the row offset (2 words) and the rank index.
+  DevGraph g;
+  g.W = (uint32_t)((nbits + 31) / 32);
+  g.NB = (g.W + 7) / 8 + 1;
+  uint64_t* coff = reinterpret_cast<uint64_t*>(scratch);
+  hipMemsetAsync(coff, 0, sizeof(uint64_t), s);
+  uint32_t* bp = scratch + 2;
+  launch_compact_rows(g, bits, 1, coff, ids, bp, s);
+  hipMemsetAsync(count, 0, sizeof(unsigned long long), s);
+  hipMemcpyAsync(count, bp + g.NB - 1, sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+}
+
+}  // namespace gpsm

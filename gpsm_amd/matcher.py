@@ -1,0 +1,174 @@
+"""Python face of the matcher: a device context, a resident data graph and
+the match entry points, all through the C ABI of libgpsm.so.
+
+Mirrors the reference's host interface (bookug/GpSM match/Match.h:80-125:
+Match::initGPU, Match(query, data), Match::match) with an explicit context and
+a graph uploaded once.  There is no CPU path: without a HIP device every call
+raises GpsmError.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, ptr, u32
+
+
+@dataclass
+class MatchResult:
+    count: int
+    hash: int
+    order: list
+    level_rows: list
+    cand: list
+    device_ms: float
+    status: int = 0
+    tuples: np.ndarray = field(default=None)
+
+
+def _result(r, q):
+    return MatchResult(int(r.count), int(r.hash), list(r.order[:q]), [int(x) for x in r.level_rows[:q]],
+                       list(r.cand[:q]), float(r.device_ms), int(r.status))
+
+
+def _opts(order_mode=_lib.ORDER_HEURISTIC, explicit_order=None, want_tuples=False, tuple_cap=0, use_graph=False,
+          block=0, join_cap=0):
+    o = _lib.Opts()
+    o.order_mode = order_mode
+    keep = None
+    if explicit_order is not None:
+        keep = u32(explicit_order)
+        o.explicit_order = ptr(keep)
+    o.want_tuples = int(want_tuples)
+    o.tuple_cap = tuple_cap
+    o.use_graph = int(use_graph)
+    o.block = block
+    o.join_cap = join_cap
+    return o, keep
+
+
+class Context:
+    """One HIP device and stream (Match::initGPU, match/Match.cu:15-70)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(_lib.lib().gpsm_init(device, C.byref(h)), "gpsm_init")
+        self.h = h
+        self.planner = None
+
+    @property
+    def device_name(self):
+        return _lib.lib().gpsm_device_name(self.h).decode()
+
+    def close(self):
+        if self.h:
+            _lib.lib().gpsm_shutdown(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, g):
+        return ResidentGraph(self, g)
+
+    # --- parity entry points -------------------------------------------------
+    def scan(self, a):
+        a = u32(a)
+        out = np.zeros_like(a)
+        tot = C.c_uint64(0)
+        check(_lib.lib().gpsm_scan_u32(self.h, ptr(a), ptr(out), len(a), C.byref(tot)), "gpsm_scan_u32")
+        return out, tot.value
+
+    def compact(self, bits, nbits):
+        bits = u32(bits)
+        ids = np.zeros(max(nbits, 1), dtype=np.uint32)
+        cnt = C.c_uint64(0)
+        check(_lib.lib().gpsm_compact_bitmap(self.h, ptr(bits), nbits, ptr(ids), C.byref(cnt)),
+              "gpsm_compact_bitmap")
+        return ids[:cnt.value].copy()
+
+
+class ResidentGraph:
+    """A data graph uploaded once to HBM (replaces Match::copyGraphToGPU,
+    which re-uploaded it for every query, match/Match.cu:128-143)."""
+
+    def __init__(self, ctx, g):
+        self.ctx = ctx
+        self.g = g
+        h = C.c_void_p()
+        ro, col, lab = u32(g.row_offset), u32(g.col), u32(g.labels)
+        check(_lib.lib().gpsm_graph_upload(ctx.h, len(lab), ptr(ro), ptr(col), ptr(lab), C.byref(h)),
+              "gpsm_graph_upload")
+        self.h = h
+
+    def free(self):
+        if self.h:
+            _lib.lib().gpsm_graph_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def match(self, q, **kw):
+        o, keep = _opts(**kw)
+        qs = q.struct()
+        r = _lib.Result()
+        rc = _lib.lib().gpsm_match(self.ctx.h, self.h, C.byref(qs), C.byref(o), C.byref(r))
+        if rc not in (0,):
+            raise _lib.GpsmError(rc, "gpsm_match")
+        res = _result(r, q.n)
+        if kw.get("want_tuples"):
+            nt = int(r.n_tuples)
+            res.tuples = np.ctypeslib.as_array(r.tuples, shape=(max(nt * q.n, 1),))[:nt * q.n].reshape(
+                nt, q.n).copy() if nt else np.zeros((0, q.n), dtype=np.uint32)
+            _lib.lib().gpsm_result_free(C.byref(r))
+        return res
+
+    def match_batch(self, queries, **kw):
+        o, keep = _opts(**kw)
+        arr = (_lib.QueryS * len(queries))()
+        for i, q in enumerate(queries):
+            arr[i] = q.struct()
+        res = (_lib.Result * len(queries))()
+        check(_lib.lib().gpsm_match_batch(self.ctx.h, self.h, len(queries), arr, C.byref(o), res),
+              "gpsm_match_batch")
+        return [_result(res[i], q.n) for i, q in enumerate(queries)]
+
+    def filter(self, q):
+        W = (self.g.n + 31) // 32
+        bm = np.zeros(q.n * W, dtype=np.uint32)
+        cnt = np.zeros(q.n, dtype=np.uint32)
+        qs = q.struct()
+        check(_lib.lib().gpsm_filter(self.ctx.h, self.h, C.byref(qs), ptr(bm), ptr(cnt)), "gpsm_filter")
+        return bm.reshape(q.n, W), cnt
+
+    def candidate_edges(self, q):
+        """Forward candidate-edge tables in the reference layout
+        [C(u) ids | offsets | lists] (match/Match.cu:1119-1121)."""
+        _, cand = self.filter(q)
+        qs = q.struct()
+        E = q.m
+        ef = np.zeros(E, dtype=np.uint32)
+        et = np.zeros(E, dtype=np.uint32)
+        size = np.zeros(E, dtype=np.uint32)
+        check(_lib.lib().gpsm_candidate_edges(self.ctx.h, self.h, C.byref(qs), ptr(ef), ptr(et), ptr(size), None, 0,
+                                              None), "gpsm_candidate_edges")
+        m = [int(cand[ef[e]]) for e in range(E)]
+        cap_words = sum(2 * m[e] + 1 + int(size[e]) for e in range(E))
+        buf = np.zeros(max(cap_words, 1), dtype=np.uint32)
+        toff = np.zeros(max(E, 1), dtype=np.uint64)
+        check(_lib.lib().gpsm_candidate_edges(self.ctx.h, self.h, C.byref(qs), ptr(ef), ptr(et), ptr(size),
+                                              ptr(buf), cap_words, ptr(toff)), "gpsm_candidate_edges")
+        out = []
+        for e in range(E):
+            t = buf[int(toff[e]):]
+            k, s = m[e], int(size[e])
+            out.append((t[:k].copy(), t[k:2 * k + 1].copy(), t[2 * k + 1:2 * k + 1 + s].copy()))
+        return [(int(ef[i]), int(et[i])) for i in range(E)], out

@@ -1,0 +1,203 @@
+/*
+ * gpsm.h — C ABI of the MI355X (gfx950) GpSM filter-and-join subgraph matcher.
+ *
+ * Semantics: undirected, vertex-labelled, non-induced, injective matching of a
+ * small query graph (at most GPSM_MAX_QV vertices, connected, at least one edge)
+ * against a data graph that stays resident in HBM.  Vertex ids start at 0,
+ * labels at 1.  The result of a query is its embedding set: every injective map
+ * f from query vertices to data vertices that preserves labels and maps every
+ * query edge to a data edge.  It is reported as an exact 64-bit count plus an
+ * order-independent 64-bit set hash (gpsm_tuple_hash summed mod 2^64), and,
+ * on request, as the sorted list of int32 tuples in query-vertex-id order.
+ *
+ * Conventions: every entry point returns GPSM_OK (0) or a negative GPSM_E*
+ * code and never exits the process.  Input arrays are caller-owned and only
+ * read during the call.  Result buffers (gpsm_result.tuples) are library-owned
+ * and released by gpsm_result_free.  One gpsm_ctx per host thread; a context
+ * owns one HIP stream and all device scratch it uses.
+ *
+ * Each declaration names the reference interface it replaces
+ * (bookug/GpSM, paths relative to the reference root).
+ */
+#ifndef GPSM_H
+#define GPSM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPSM_MAX_QV 64 /* maximum query vertices handled by the device join */
+
+/* error codes */
+#define GPSM_OK 0
+#define GPSM_EINVAL (-1)      /* malformed argument or graph */
+#define GPSM_ENOMEM (-2)      /* device or host allocation failed */
+#define GPSM_EDEVICE (-3)     /* HIP runtime error */
+#define GPSM_ETOOBIG (-4)     /* query larger than GPSM_MAX_QV */
+#define GPSM_EOVERFLOW (-5)   /* tuple materialisation exceeded tuple_cap */
+#define GPSM_ENODEV (-6)      /* no HIP device */
+#define GPSM_EDISCONN (-7)    /* query not connected or without edges */
+#define GPSM_EFORMAT (-8)     /* text reader: malformed or unterminated input */
+
+typedef struct gpsm_ctx gpsm_ctx;
+typedef struct gpsm_graph gpsm_graph;
+
+/* A query graph as the reference's text format lists it: vertex labels and the
+ * "e src dst label" lines in file order (io/IO.cpp:46-100).  Edge direction
+ * and edge labels are ignored (undirected semantics). */
+typedef struct gpsm_query {
+  uint32_t n;              /* vertices, ids 0..n-1 */
+  uint32_t m;              /* edges in file order */
+  const uint32_t* labels;  /* [n], >= 1 */
+  const uint32_t* src;     /* [m] */
+  const uint32_t* dst;     /* [m] */
+} gpsm_query;
+
+/* join-order selection */
+#define GPSM_ORDER_HEURISTIC 0 /* the reference's greedy min-|CET| order (match/Match.cu:1599-1661) */
+#define GPSM_ORDER_LEARNED 1   /* greedy over MLP-predicted log cardinalities; needs gpsm_set_planner */
+#define GPSM_ORDER_EXPLICIT 2  /* caller-given vertex order in gpsm_opts.explicit_order */
+
+typedef struct gpsm_opts {
+  int32_t order_mode;             /* GPSM_ORDER_* */
+  const uint32_t* explicit_order; /* EXPLICIT: [n] query ids; every prefix of length >=2 connected */
+  uint32_t want_tuples;           /* gpsm_match only: materialise the sorted tuple list */
+  uint64_t tuple_cap;             /* maximum tuples to materialise (GPSM_EOVERFLOW beyond) */
+  uint32_t use_graph;             /* gpsm_match_batch: capture the phases in hipGraphs */
+  uint32_t block;                 /* 0 = autotuned; else workgroup size of the filter/collect kernels */
+  uint32_t join_block;            /* 0 = autotuned; else workgroup size of the join kernel */
+  uint32_t join_cap;              /* 0 = autotuned; else rows per DFS level buffer of the join */
+} gpsm_opts;
+
+typedef struct gpsm_result {
+  int32_t status;                   /* GPSM_OK or error for this query */
+  uint32_t q;                       /* query vertices */
+  uint64_t count;                   /* embeddings */
+  uint64_t hash;                    /* set hash */
+  uint32_t order[GPSM_MAX_QV];      /* join position -> query vertex id */
+  uint64_t level_rows[GPSM_MAX_QV]; /* rows with k+1 bound vertices after step k (k>=1) */
+  uint32_t cand[GPSM_MAX_QV];       /* |C(u)| after refinement, by query vertex id */
+  uint32_t* tuples;                 /* library-owned, n_tuples x q, sorted */
+  uint64_t n_tuples;
+  double device_ms;                 /* device time of this call (batch: whole batch) */
+} gpsm_result;
+
+/* --- lifecycle ---------------------------------------------------------- */
+
+/* Replaces Match::initGPU (match/Match.cu:15-70): selects the device, creates
+ * the context's stream and warms the runtime.  Error code instead of exit. */
+int gpsm_init(int device, gpsm_ctx** out);
+void gpsm_shutdown(gpsm_ctx* ctx);
+const char* gpsm_strerror(int code);
+/* Name of the HIP device the context runs on (empty when none). */
+const char* gpsm_device_name(const gpsm_ctx* ctx);
+
+/* Replaces Match::copyGraphToGPU (match/Match.cu:128-143), which re-uploaded
+ * the CSR on every query: the graph is uploaded once and stays resident.
+ * Input is the undirected CSR of Graph::transformToCSR (graph/Graph.cpp:30-104):
+ * row_offset[n+1], col[row_offset[n]] with each row sorted ascending, no
+ * self-loops and no duplicates, labels[n] >= 1. */
+int gpsm_graph_upload(gpsm_ctx* ctx, uint32_t n, const uint32_t* row_offset,
+                      const uint32_t* col, const uint32_t* labels, gpsm_graph** out);
+int gpsm_graph_free(gpsm_graph* g);
+/* Bytes of device memory the resident graph occupies. */
+uint64_t gpsm_graph_device_bytes(const gpsm_graph* g);
+
+/* --- matching ----------------------------------------------------------- */
+
+/* Replaces Match::match (match/Match.cu:1710-1966) together with the
+ * injectivity half of IO::verify (io/IO.cpp:146-160): one query, launched
+ * step by step on the context's stream. */
+int gpsm_match(gpsm_ctx* ctx, const gpsm_graph* g, const gpsm_query* q,
+               const gpsm_opts* opts, gpsm_result* out);
+
+/* Batched form of gpsm_match with no reference counterpart (the reference
+ * loops over queries in main/run.cpp:74-93): nq queries of uneven size are
+ * filtered in lock-step supersteps and joined by one persistent launch fed by
+ * a device work queue.  out[nq]. */
+int gpsm_match_batch(gpsm_ctx* ctx, const gpsm_graph* g, uint32_t nq,
+                     const gpsm_query* qs, const gpsm_opts* opts, gpsm_result* out);
+
+void gpsm_result_free(gpsm_result* r);
+
+/* --- pipeline stages (parity entry points) ------------------------------ */
+
+/* InitializeCandidateVertices + RefineCandidateVertices
+ * (match/Match.cu:661-719, 848-890): candidate bitmaps after the filter.
+ * bitmaps: [q->n][words] with words = (n+31)/32, bit v of row u set iff v in C(u);
+ * counts[q->n] = |C(u)|. */
+int gpsm_filter(gpsm_ctx* ctx, const gpsm_graph* g, const gpsm_query* q,
+                uint32_t* bitmaps, uint32_t* counts);
+
+/* FindCandidateEdges (match/Match.cu:1082-1139) for the query edge list in
+ * the reference's order (match/Match.cu:1735-1754, from < to).
+ * edge_from/edge_to[e]: the edges; cet_size[e] = |CET(from,to)|.
+ * If lists != NULL (capacity list_cap entries) the forward tables are written
+ * back to back in the reference layout [C(u) ids | offsets | neighbour lists]
+ * (match/Match.cu:1119-1121), table e starting at lists[table_off[e]]. */
+int gpsm_candidate_edges(gpsm_ctx* ctx, const gpsm_graph* g, const gpsm_query* q,
+                         uint32_t* edge_from, uint32_t* edge_to, uint32_t* cet_size,
+                         uint32_t* lists, uint64_t list_cap, uint64_t* table_off);
+
+/* thrust::exclusive_scan as used by kernel_collect / kernel_join
+ * (match/Match.cu:504-508) and gpsm::scan::prefixSum (util/prefix_scan.cu:410-427):
+ * out[i] = sum(in[0..i)), returns the total in *total. Host buffers. */
+int gpsm_scan_u32(gpsm_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* total);
+
+/* kernel_collect (match/Match.cu:488-526) / gpsm::scan::compact
+ * (util/compact_scan.cu:52-70) on a bitmap: ids of set bits, ascending. */
+int gpsm_compact_bitmap(gpsm_ctx* ctx, const uint32_t* bits, uint64_t nbits,
+                        uint32_t* ids, uint64_t* count);
+
+/* --- learned planner (no reference counterpart) -------------------------- */
+
+#define GPSM_FEAT 32   /* features per candidate join step */
+#define GPSM_HIDDEN 128
+
+/* Per-step features for planning: for the state "vertices in prefix placed",
+ * for each candidate next edge/vertex.  Computed on the host from the filter
+ * statistics the device returns (|C(u)|, |CET(e)|, label frequencies, degrees). */
+typedef struct gpsm_mlp gpsm_mlp;
+
+/* MLP 32 -> 128 -> 128 -> 1 with ReLU, BF16 weights, FP32 accumulation,
+ * trained with Adam on the device. */
+int gpsm_mlp_create(gpsm_ctx* ctx, uint64_t seed, gpsm_mlp** out);
+void gpsm_mlp_free(gpsm_mlp* m);
+/* Parameter count and (de)serialisation of the FP32 master weights in the
+ * order W1[128x32], b1[128], W2[128x128], b2[128], W3[128], b3[1]. */
+uint64_t gpsm_mlp_num_params(void);
+int gpsm_mlp_get_params(gpsm_mlp* m, float* out);
+int gpsm_mlp_set_params(gpsm_mlp* m, const float* in);
+/* Forward: x[B x 32] (host, FP32, rounded to BF16 on the device) -> y[B]. */
+int gpsm_mlp_forward(gpsm_mlp* m, const float* x, uint32_t B, float* y);
+/* Loss 0.5*mean((y-t)^2) and its gradient w.r.t. every parameter (same
+ * order as gpsm_mlp_get_params) for one mini-batch. */
+int gpsm_mlp_grad(gpsm_mlp* m, const float* x, const float* t, uint32_t B, float* loss, float* grad);
+/* One Adam step on a mini-batch; *loss receives the pre-step loss. */
+int gpsm_mlp_train_step(gpsm_mlp* m, const float* x, const float* t, uint32_t B, float lr, float* loss);
+/* Installs (or, with NULL, removes) the network used by GPSM_ORDER_LEARNED. */
+int gpsm_set_planner(gpsm_ctx* ctx, gpsm_mlp* m);
+
+/* Host-side planner features for a query after the filter (no device work):
+ * per query edge e the fixed-width feature row the MLP consumes when e is the
+ * candidate expand edge of the state given by placed[] (1 = vertex bound). */
+int gpsm_plan_features(uint32_t q_n, const gpsm_query* q, const uint32_t* cand,
+                       const uint32_t* cet_size, const uint32_t* label_count,
+                       uint32_t n_labels_total, uint32_t data_n, uint64_t data_adj,
+                       const uint8_t* placed, float log_rows, float* feat_out);
+
+/* --- set hash ------------------------------------------------------------- */
+
+/* Hash of one tuple t[0..q) in query-id order:
+ * fmix64( sum_j fmix64(((uint64)j << 32 | t[j]) ^ 0x9E3779B97F4A7C15) ),
+ * fmix64 = MurmurHash3's 64-bit finaliser.  The set hash is the sum of tuple
+ * hashes mod 2^64, so it is independent of enumeration order. */
+uint64_t gpsm_tuple_hash(const uint32_t* t, uint32_t q);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPSM_H */

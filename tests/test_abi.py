@@ -1,0 +1,120 @@
+"""The C-ABI library: loads, exports every declared symbol, and its host-only
+parts (generators, text writer, tuple hash) behave.  CPU only: no call here
+touches a GPU."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from gpsm_amd import _lib
+from gpsm_amd import graphs as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    names = set()
+    for h in ("gpsm.h", "gpsm_gen.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(gpsm_\w+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = C.CDLL(_lib.LIB_PATH)
+    missing = [n for n in sorted(declared()) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_bindings_cover_the_header():
+    assert declared() <= set(_lib.SIGNATURES), sorted(declared() - set(_lib.SIGNATURES))
+
+
+def test_init_without_device_fails_cleanly():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    h = C.c_void_p()
+    assert _lib.lib().gpsm_init(0, C.byref(h)) == -6  # GPSM_ENODEV, no exit()
+
+
+def _check_csr(g):
+    ro, col = g.row_offset.astype(np.int64), g.col.astype(np.int64)
+    assert ro[0] == 0 and np.all(np.diff(ro) >= 0)
+    src = np.repeat(np.arange(g.n), np.diff(ro))
+    assert np.all(src != col)
+    for v in range(0, g.n, max(1, g.n // 200)):
+        r = col[ro[v]:ro[v + 1]]
+        assert np.all(np.diff(r) > 0)
+    # symmetric
+    fwd = set(zip(src.tolist(), col.tolist()))
+    assert all((b, a) in fwd for a, b in list(fwd)[:5000])
+
+
+def test_generators_deterministic_and_valid():
+    a = G.erdos_renyi(5000, 20000, 7)
+    b = G.erdos_renyi(5000, 20000, 7)
+    assert np.array_equal(a.col, b.col) and a.adj == 40000
+    _check_csr(a)
+    c = G.chung_lu(20000, 80000, 2.1, 3)
+    assert c.adj == 160000
+    _check_csr(c)
+    r = G.rmat(12, 20000, 0.57, 0.19, 0.19, 5)
+    assert r.adj == 40000
+    _check_csr(r)
+    G.set_labels(c, 16, 1.0, 9)
+    cnt = np.bincount(c.labels, minlength=17)[1:]
+    assert cnt[0] > cnt[3] > cnt[15] > 0  # Zipf(1.0)
+
+
+@pytest.mark.parametrize("kind", [_lib.Q_WALK_TREE, _lib.Q_WALK_INDUCED, _lib.Q_CYCLE, _lib.Q_CLIQUE])
+def test_queries_are_data_subgraphs(kind):
+    g = G.small_graph(4, n=3000, m=9000, labels=3, kind="cl")
+    k = 4 if kind == _lib.Q_CLIQUE else 6
+    qs = G.queries(g, 6, k, k, kind, 11)
+    assert qs == qs and len(qs) == 6
+    for q in qs:
+        assert q.n == k
+        # connected, simple
+        assert len(set(zip(q.src.tolist(), q.dst.tolist()))) == q.m
+        # at least one embedding (the walk itself)
+        assert O.o1_enumerate(g, q)["count"] >= 1
+        if kind == _lib.Q_CYCLE:
+            assert q.m == k
+        if kind == _lib.Q_CLIQUE:
+            assert q.m == k * (k - 1) // 2
+
+
+def test_query_sets_do_not_depend_on_thread_count():
+    g = G.small_graph(4, n=3000, m=9000, labels=3, kind="cl")
+    a = G.queries(g, 8, 4, 8, _lib.Q_WALK_HALF, 5, 10 ** 7)
+    os.environ["OMP_NUM_THREADS"] = "1"
+    b = G.queries(g, 8, 4, 8, _lib.Q_WALK_HALF, 5, 10 ** 7)
+    for x, y in zip(a, b):
+        assert np.array_equal(x.labels, y.labels) and np.array_equal(x.src, y.src)
+
+
+def test_tuple_hash_matches_oracle():
+    rng = np.random.default_rng(0)
+    for k in (2, 5, 17):
+        t = rng.integers(0, 1 << 31, size=k).astype(np.uint32)
+        assert _lib.lib().gpsm_tuple_hash(_lib.ptr(t), k) == O.tuple_hash(t)
+
+
+def test_text_writer_format(tmp_path):
+    g = G.complete(4)
+    src, dst = g.edges()
+    p = str(tmp_path / "g.txt").encode()
+    assert _lib.lib().gpsm_io_write_graph(p, 0, 0, g.n, _lib.ptr(g.labels), len(src), _lib.ptr(src),
+                                          _lib.ptr(dst)) == 0
+    assert _lib.lib().gpsm_io_write_end(p) == 0
+    lines = open(p.decode()).read().splitlines()
+    assert lines[0] == "t # 0" and lines[1].split()[:2] == ["4", "6"]
+    assert lines[2] == "v 0 1" and lines[6] == "e 0 1 0" and lines[-1] == "t # -1"

@@ -1,0 +1,204 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle: every
+stage bit-exact (integer work), at sizes the oracle finishes in seconds."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from gpsm_amd import _lib
+from gpsm_amd import graphs as G
+from helpers import small_cases
+from test_oracle import hand_fixture
+
+pytestmark = pytest.mark.gpu
+
+
+def bits_to_sets(bm, n):
+    out = []
+    for row in bm:
+        b = np.unpackbits(row.view(np.uint8), bitorder="little")[:n]
+        out.append(np.nonzero(b)[0])
+    return out
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 1023, 1024, 1025, 4097, (1 << 20) + 3])
+def test_scan(ctx, n):
+    rng = np.random.default_rng(n)
+    a = rng.integers(0, 1 << 20, size=n).astype(np.uint32)
+    out, tot = ctx.scan(a)
+    ref, rtot = O.exclusive_scan(a)
+    assert np.array_equal(out, ref) and tot == rtot
+
+
+def test_scan_wraps_like_u32(ctx):
+    a = np.full(4096 * 3 + 5, 0xFFFFFFF0, dtype=np.uint32)
+    out, tot = ctx.scan(a)
+    ref, rtot = O.exclusive_scan(a)
+    assert np.array_equal(out, ref) and tot == rtot
+
+
+@pytest.mark.parametrize("nbits", [1, 31, 32, 33, 255, 256, 257, 8192, 8193, (1 << 20) + 3])
+@pytest.mark.parametrize("density", [0.0, 0.01, 0.5, 1.0])
+def test_compact(ctx, nbits, density):
+    rng = np.random.default_rng(nbits)
+    words = np.zeros((nbits + 31) // 32, dtype=np.uint32)
+    on = np.nonzero(rng.random(nbits) < density)[0] if density < 1 else np.arange(nbits)
+    for i in on:
+        words[i >> 5] |= np.uint32(1 << (i & 31))
+    words[-1] |= np.uint32(0xFFFFFFFF) << np.uint32(nbits & 31) if nbits & 31 else np.uint32(0)  # garbage past nbits
+    assert np.array_equal(ctx.compact(words, nbits), O.compact_bitmap(words, nbits))
+
+
+def test_hand_fixture(ctx):
+    fx, g, q = hand_fixture()
+    rg = ctx.upload(g)
+    bm, cnt = rg.filter(q)
+    sets = bits_to_sets(bm, g.n)
+    for u, ids in enumerate(fx["cand_refined"]):
+        assert sets[u].tolist() == ids
+    edges, tables = rg.candidate_edges(q)
+    assert edges == [tuple(e) for e in fx["edges"]]
+    for (ids, off, lst), t in zip(tables, fx["cet"]):
+        assert [ids.tolist(), off.tolist(), lst.tolist()] == t
+    r = rg.match(q, want_tuples=True, tuple_cap=100)
+    assert r.order == fx["order"]
+    assert r.level_rows[1:] == fx["step_rows_injective"]
+    assert r.tuples.tolist() == fx["embeddings"]
+
+
+@pytest.mark.parametrize("case", range(16))
+def test_filter_matches_oracle(ctx, case):
+    g, q = small_cases(16, seed=31)[case]
+    st, init, ref, fo = O.o2_filter(g, q)
+    rg = ctx.upload(g)
+    bm, cnt = rg.filter(q)
+    if st != O.OK:
+        # the reference stops at the first empty set; the device finishes the
+        # schedule, so only the emptiness itself is compared
+        assert (cnt == 0).any()
+        return
+    sets = bits_to_sets(bm, g.n)
+    for u in range(q.n):
+        assert np.array_equal(sets[u], np.nonzero(ref[u])[0]), u
+        assert cnt[u] == ref[u].sum()
+
+
+@pytest.mark.parametrize("case", range(12))
+def test_candidate_edges_match_oracle(ctx, case):
+    g, q = small_cases(12, seed=41)[case]
+    ref = O.o2_cet(g, q)
+    rg = ctx.upload(g)
+    if ref is None:
+        return
+    edges, tables = rg.candidate_edges(q)
+    assert edges == ref[0]
+    for (a, b, c), (x, y, z) in zip(tables, ref[1]):
+        assert np.array_equal(a, x) and np.array_equal(b, y) and np.array_equal(c, z)
+
+
+@pytest.mark.parametrize("case", range(24))
+def test_match_matches_oracle(ctx, case):
+    g, q = small_cases(24, seed=51)[case]
+    r2 = O.o2_match(g, q, mode=0)
+    r1 = O.o1_enumerate(g, q)
+    rg = ctx.upload(g)
+    r = rg.match(q)
+    assert (r.count, r.hash) == (r1["count"], r1["hash"]) == (r2["count"], r2["hash"])
+    if r2["status"] == O.OK and r2["count"] > 0:
+        assert r.order == r2["order"]
+        assert r.level_rows[1:] == r2["level_rows"][1:q.n]
+        assert r.cand == r2["cand"]
+
+
+def test_config1_full_tuple_list(ctx):
+    """Config 1: ER 10k/40k, 8 labels, 6-vertex walk query: the sorted
+    int32 tuple list is identical to the independent enumerator's."""
+    g, (q,) = G.config1()
+    r1 = O.o1_enumerate(g, q, tuple_cap=10 ** 6)
+    r2 = O.o2_match(g, q, mode=0, tuple_cap=10 ** 6)
+    r = ctx.upload(g).match(q, want_tuples=True, tuple_cap=10 ** 6)
+    assert r.count == r1["count"] == r2["count"] >= 1
+    assert r.tuples.tobytes() == r1["tuples"].tobytes() == r2["tuples"].tobytes()
+    assert r.hash == r1["hash"]
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 5])
+def test_closed_forms(ctx, k):
+    import math
+    rg = ctx.upload(G.complete(8))
+    assert rg.match(G.clique_query(k)).count == math.perm(8, k)
+    rc = ctx.upload(G.cycle(11))
+    assert rc.match(G.path_query(k + 1)).count == 22
+
+
+def test_absent_label_and_empty_results(ctx):
+    g = G.small_graph(3, n=200, m=600, labels=2)
+    rg = ctx.upload(g)
+    q = G.path_query(3, label=7)  # label absent from the data graph
+    r = rg.match(q)
+    assert r.count == 0 and r.hash == 0
+    # bipartite data graph, odd cycle query
+    a = np.repeat(np.arange(5), 5)
+    b = np.tile(np.arange(5, 10), 5)
+    gb = G.from_edges(10, a, b, np.ones(10))
+    assert ctx.upload(gb).match(G.cycle_query(5)).count == 0
+    assert ctx.upload(gb).match(G.cycle_query(4)).count == 5 * 4 * 5 * 4 * 2
+
+
+def test_invalid_queries_are_rejected(ctx):
+    g = G.small_graph(3, n=200, m=600, labels=2)
+    rg = ctx.upload(g)
+    disconnected = G.Query(np.ones(4, dtype=np.uint32), np.array([0, 2], dtype=np.uint32),
+                           np.array([1, 3], dtype=np.uint32))
+    with pytest.raises(_lib.GpsmError) as e:
+        rg.match(disconnected)
+    assert e.value.code == -7
+    big = G.path_query(65)
+    with pytest.raises(_lib.GpsmError) as e:
+        rg.match(big)
+    assert e.value.code == -4
+
+
+def test_batch_equals_single_and_oracle(ctx):
+    g = G.small_graph(9, n=3000, m=12000, labels=3, kind="cl")
+    qs = G.queries(g, 24, 3, 7, _lib.Q_WALK_HALF, 13, 10 ** 7)
+    rg = ctx.upload(g)
+    batch = rg.match_batch(qs)
+    graph = rg.match_batch(qs, use_graph=True)
+    for q, b, gr in zip(qs, batch, graph):
+        s = rg.match(q)
+        r1 = O.o1_enumerate(g, q)
+        assert (b.count, b.hash) == (s.count, s.hash) == (r1["count"], r1["hash"]) == (gr.count, gr.hash)
+        assert b.level_rows == s.level_rows
+
+
+def test_small_join_buffers_force_depth_first_spills(ctx):
+    """A join buffer of 512 rows forces many partial windows and re-entries."""
+    g = G.small_graph(10, n=2000, m=16000, labels=1, kind="er")
+    rg = ctx.upload(g)
+    for q in G.queries(g, 4, 4, 6, _lib.Q_WALK_TREE, 3):
+        r1 = O.o1_enumerate(g, q)
+        r = rg.match(q, join_cap=512)
+        assert (r.count, r.hash) == (r1["count"], r1["hash"])
+
+
+def test_explicit_orders_agree(ctx):
+    g, q = small_cases(6, seed=61)[4]
+    rg = ctx.upload(g)
+    base = rg.match(q)
+    rng = np.random.default_rng(0)
+    adj = [set() for _ in range(q.n)]
+    for a, b in zip(q.src, q.dst):
+        adj[a].add(int(b))
+        adj[b].add(int(a))
+    for _ in range(5):
+        start = int(rng.integers(q.n))
+        order, seen = [start], {start}
+        while len(order) < q.n:
+            w = int(rng.choice(sorted({x for v in order for x in adj[v]} - seen)))
+            order.append(w)
+            seen.add(w)
+        r = rg.match(q, order_mode=_lib.ORDER_EXPLICIT, explicit_order=order)
+        r2 = O.o2_match(g, q, mode=0, order=order)
+        assert (r.count, r.hash) == (base.count, base.hash) == (r2["count"], r2["hash"])
+        if r2["count"]:
+            assert r.level_rows[1:] == r2["level_rows"][1:q.n]
