@@ -1,0 +1,169 @@
+"""Benchmark of the GpSM filter-and-join matcher on one MI355X.
+
+Workload (configs[1] of the benchmark definition): Chung-Lu power-law data
+graph, 1,000,000 vertices and 8,000,000 undirected edges (gamma 2.1), 16 labels
+with Zipf(1.0) frequencies; 100 random-walk queries drawn with 8-16 vertices,
+half tree-like and half with all induced edges, under the feasibility rule of
+gpsm_gen_queries (embedding-count bound 1e8; slots shrink until feasible).
+All inputs are synthetic and seeded.
+
+A step is one pass of the matcher over the whole query set (every query's
+exact embedding count and set hash), with the data graph resident in HBM.
+The headline path is the batched launch (filter supersteps for all queries,
+one persistent join grid) with the learned join order when a trained planner
+is present, else the reference heuristic order; the plain path (one query at
+a time, heuristic order, the reference's structure) is timed beside it.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--budget", type=float, default=1e8, help="feasibility bound on embeddings per query")
+    ap.add_argument("--queries", type=int, default=100)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline sample length")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on exactly one GPU (the matcher shards by query across GPUs only "
+              "in design); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from gpsm_amd import _lib
+    from gpsm_amd import graphs as G
+    from gpsm_amd import planner
+    from gpsm_amd.matcher import Context
+
+    t0 = time.time()
+    g = G.config2_graph()
+    qs = G.queries(g, args.queries, 8, 16, _lib.Q_WALK_HALF, 2, int(args.budget))
+    t_gen = time.time() - t0
+    sizes = [q.n for q in qs]
+
+    ctx = Context(0)
+    rg = ctx.upload(g)
+    mode = _lib.ORDER_HEURISTIC
+    planner_used = False
+    if planner.install(ctx):
+        mode = _lib.ORDER_LEARNED
+        planner_used = True
+
+    for _ in range(args.warmup):
+        rg.match_batch(qs, order_mode=mode)
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = rg.match_batch(qs, order_mode=mode)
+    t1 = time.perf_counter()
+    stats = ctx.kernel_stats()
+    ctx.profile(False)
+    elapsed = t1 - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    emb = sum(r.count for r in res)
+    qps = len(qs) * args.steps / elapsed
+
+    # the heuristic order in the same batched launch, and the plain path
+    extra = {}
+    if planner_used:
+        t0 = time.perf_counter()
+        for _ in range(max(1, args.steps // 2)):
+            rh = rg.match_batch(qs, order_mode=_lib.ORDER_HEURISTIC)
+        dt = (time.perf_counter() - t0) / max(1, args.steps // 2)
+        extra["batched_heuristic_qps"] = len(qs) / dt
+        assert [(r.count, r.hash) for r in rh] == [(r.count, r.hash) for r in res]
+    if not args.no_plain:
+        rg.match(qs[0])
+        t0 = time.perf_counter()
+        rp = [rg.match(q) for q in qs]
+        dt = time.perf_counter() - t0
+        extra["plain_heuristic_qps"] = len(qs) / dt
+        assert [(r.count, r.hash) for r in rp] == [(r.count, r.hash) for r in res]
+        extra["speedup_vs_plain"] = qps / extra["plain_heuristic_qps"]
+
+    # roofline of the dominant kernel (by time in the timed region)
+    dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
+    name, st = dom
+    avg_s = st["ms"] / 1e3 / st["launches"]
+    bytes_per_launch = st["bytes"] / st["launches"]
+    achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    roofline = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": round(bytes_per_launch),
+                "launches": st["launches"]}
+    kernels = {k: {"ms_per_step": round(v["ms"] / args.steps, 3), "launches_per_step": v["launches"] / args.steps,
+                   "GBps": round(v["bytes"] / (v["ms"] / 1e3) / 1e9, 1) if v["bytes"] and v["ms"] else None}
+               for k, v in stats.items()}
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+        done, t0 = 0, time.perf_counter()
+        for q, r in zip(qs, res):
+            ro = O.o2_match(g, q, mode=0)
+            assert (ro["count"], ro["hash"]) == (r.count, r.hash), "CPU oracle disagrees with the device"
+            done += 1
+            if time.perf_counter() - t0 > args.cpu_seconds:
+                break
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(done / dt, 4), "unit": "queries/s", "cores": 1, "kind": "port",
+               "sample": "O2 host restatement, single thread, first %d of the %d queries (same results checked)"
+                         % (done, len(qs)), "cpu": cpu_model(), "nproc": os.cpu_count()}
+
+    out = {
+        "metric": "queries/sec (exact embedding counts + set hashes), batched %s-order path"
+                  % ("learned" if planner_used else "heuristic"),
+        "value": round(qps, 3),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "chunglu-1M-8M-zipf16/%d-walk-queries" % len(qs), "n": g.n, "undirected_edges": g.adj // 2,
+                   "queries": len(qs), "query_sizes": {str(k): sizes.count(k) for k in sorted(set(sizes))},
+                   "feasibility_budget": args.budget, "order": "learned" if planner_used else "heuristic",
+                   "device": ctx.device_name},
+        "embeddings_per_step": emb,
+        "embeddings_per_sec": round(emb * args.steps / elapsed),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+        "workload_gen_s": round(t_gen, 1),
+    }
+    out.update({k: (round(v, 3) if isinstance(v, float) else v) for k, v in extra.items()})
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

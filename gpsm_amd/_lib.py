@@ -62,6 +62,10 @@ class Result(C.Structure):
                 ("device_ms", C.c_double)]
 
 
+class KStat(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64), ("total_ms", C.c_double), ("bytes", C.c_double)]
+
+
 _LIB = None
 
 # name -> (restype, argtypes); every name here is exported by libgpsm.so
@@ -97,6 +101,8 @@ SIGNATURES = {
     "gpsm_plan_features": (C.c_int, [C.c_uint32, C.POINTER(QueryS), C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_uint32, C.c_uint32, C.c_uint64, C.c_void_p, C.c_float, C.c_void_p]),
     "gpsm_tuple_hash": (C.c_uint64, [C.c_void_p, C.c_uint32]),
+    "gpsm_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "gpsm_profile_read": (C.c_int, [C.c_void_p, C.POINTER(KStat), C.c_uint32, C.POINTER(C.c_uint32)]),
     # generators and text I/O (include/gpsm_gen.h)
     "gpsm_gen_csr_from_edges": (C.c_int, [C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.POINTER(CSR)]),

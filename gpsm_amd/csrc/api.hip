@@ -72,11 +72,54 @@ void DevBuf::release() {
   bytes = 0;
 }
 
+const char* const kProfNames[K_KINDS_] = {"filter_step", "join", "cet_count", "cet_write",
+                                           "compact_rows", "init", "other"};
+
+hipEvent_t Prof::get() {
+  if (!pool.empty()) {
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+
 }  // namespace gpsm
 
 // ---------------------------------------------------------------------------
 
 extern "C" {
+
+int gpsm_profile_enable(gpsm_ctx* c, int on) {
+  if (!c) return GPSM_EINVAL;
+  c->prof.on = on != 0;
+  for (auto& a : c->prof.acc) a = Prof::Acc();
+  for (auto& p : c->prof.pending) {
+    c->prof.pool.push_back(p.a);
+    c->prof.pool.push_back(p.b);
+  }
+  c->prof.pending.clear();
+  return GPSM_OK;
+}
+
+int gpsm_profile_read(gpsm_ctx* c, gpsm_kstat* out, uint32_t max, uint32_t* n) {
+  if (!c || !n || (max && !out)) return GPSM_EINVAL;
+  uint32_t k = 0;
+  for (int i = 0; i < K_KINDS_ && k < max; ++i) {
+    const Prof::Acc& a = c->prof.acc[i];
+    if (!a.launches) continue;
+    std::memset(&out[k], 0, sizeof(gpsm_kstat));
+    std::snprintf(out[k].name, sizeof(out[k].name), "%s", kProfNames[i]);
+    out[k].launches = a.launches;
+    out[k].total_ms = a.ms;
+    out[k].bytes = a.bytes;
+    ++k;
+  }
+  *n = k;
+  return GPSM_OK;
+}
 
 const char* gpsm_strerror(int code) {
   switch (code) {
@@ -304,6 +347,25 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   }
   const uint32_t W = dg.W, NB = dg.NB;
   int rc;
+  // per-launch timing (not inside captured graphs)
+  const bool prof = c->prof.on && !o.use_graph;
+  for (auto& p : c->prof.pending) {
+    c->prof.pool.push_back(p.a);
+    c->prof.pool.push_back(p.b);
+  }
+  c->prof.pending.clear();
+  auto pbegin = [&]() -> hipEvent_t {
+    if (!prof) return nullptr;
+    hipEvent_t e = c->prof.get();
+    hipEventRecord(e, s);
+    return e;
+  };
+  auto pend = [&](int kind, hipEvent_t a, double bytes, int slot) {
+    if (!prof) return;
+    hipEvent_t b = c->prof.get();
+    hipEventRecord(b, s);
+    c->prof.pending.push_back({kind, a, b, bytes, slot});
+  };
   auto t0 = std::chrono::steady_clock::now();
   hipEventRecord(c->ev0, s);
 
@@ -338,18 +400,29 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       (rc = upload(c, B_NBR_LO, nbr_lo)) || (rc = upload(c, B_NBR_HI, nbr_hi)))
     return rc;
   if ((rc = c->bufs[B_COUNTS].ensure(sizeof(uint32_t) * std::max<uint32_t>(nslots, 1)))) return rc;
+  if ((rc = c->bufs[B_FSTATS].ensure(sizeof(unsigned long long) * 5 * (maxsteps + 1)))) return rc;
   uint32_t* bitmaps = dptr<uint32_t>(c, B_BITMAPS);
+  unsigned long long* fstats = dptr<unsigned long long>(c, B_FSTATS);
   {
     bool captured = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     if (o.use_graph && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) captured = true;
     launch_fill_u32(bitmaps, 0u, (uint64_t)nslots * W, s);
-    launch_init(dg, dptr<InitOp>(c, B_INIT), (uint32_t)init.size(), bitmaps, s);
-    for (uint32_t sidx = 0; sidx < maxsteps; ++sidx)
-      launch_filter_step(dg, dptr<FilterOp>(c, B_FOPS) + step_off[sidx], step_off[sidx + 1] - step_off[sidx],
-                         dptr<uint32_t>(c, B_NBR_SLOT), dptr<uint32_t>(c, B_NBR_LO), dptr<uint32_t>(c, B_NBR_HI),
-                         bitmaps, fblock, s);
+    if (prof) launch_fill_u32(reinterpret_cast<uint32_t*>(fstats), 0u, 10ull * (maxsteps + 1), s);
+    {
+      hipEvent_t a = pbegin();
+      launch_init(dg, dptr<InitOp>(c, B_INIT), (uint32_t)init.size(), bitmaps, s);
+      pend(K_INIT, a, (double)init.size() * (4.0 * dg.n + 4.0 * W), -1);
+    }
+    for (uint32_t sidx = 0; sidx < maxsteps; ++sidx) {
+      const uint32_t nops_s = step_off[sidx + 1] - step_off[sidx];
+      hipEvent_t a = pbegin();
+      launch_filter_step(dg, dptr<FilterOp>(c, B_FOPS) + step_off[sidx], nops_s, dptr<uint32_t>(c, B_NBR_SLOT),
+                         dptr<uint32_t>(c, B_NBR_LO), dptr<uint32_t>(c, B_NBR_HI), bitmaps, fblock,
+                         prof ? fstats + 5 * sidx : nullptr, s);
+      pend(K_FILTER, a, 4.0 * W * nops_s, (int)sidx);
+    }
     launch_row_popcount(dg, bitmaps, nslots, dptr<uint32_t>(c, B_COUNTS), s);
     if (captured) {
       if (hipStreamEndCapture(s, &graph) != hipSuccess || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) !=
@@ -385,8 +458,12 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   if ((rc = upload(c, B_COFF, coff))) return rc;
   if ((rc = c->bufs[B_CLIST].ensure(sizeof(uint32_t) * std::max<uint64_t>(coff[nslots], 1)))) return rc;
   if ((rc = c->bufs[B_BPREFIX].ensure(sizeof(uint32_t) * (size_t)std::max<uint32_t>(nslots, 1) * NB))) return rc;
-  launch_compact_rows(dg, bitmaps, nslots, dptr<uint64_t>(c, B_COFF), dptr<uint32_t>(c, B_CLIST),
-                      dptr<uint32_t>(c, B_BPREFIX), s);
+  {
+    hipEvent_t a = pbegin();
+    launch_compact_rows(dg, bitmaps, nslots, dptr<uint64_t>(c, B_COFF), dptr<uint32_t>(c, B_CLIST),
+                        dptr<uint32_t>(c, B_BPREFIX), s);
+    pend(K_COMPACT, a, (double)nslots * 4.0 * (W + NB) + 4.0 * coff[nslots], -1);
+  }
   std::vector<CetDesc> fdesc;
   std::vector<uint32_t> fdesc_of_q(nq, 0);
   for (uint32_t b = 0; b < nq; ++b) {
@@ -405,8 +482,12 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   const uint32_t cgrid = std::max<uint32_t>(c->ncu * 8, 64);
   if ((rc = upload(c, B_FDESC, fdesc))) return rc;
   if ((rc = c->bufs[B_CHUNKS].ensure(sizeof(uint32_t) * (fdesc.size() + 2)))) return rc;
-  launch_cet_count(dg, bitmaps, dptr<uint32_t>(c, B_CLIST), dptr<CetDesc>(c, B_FDESC), (uint32_t)fdesc.size(),
-                   nullptr, 0, dptr<uint32_t>(c, B_CHUNKS), cgrid, s);
+  {
+    hipEvent_t a = pbegin();
+    launch_cet_count(dg, bitmaps, dptr<uint32_t>(c, B_CLIST), dptr<CetDesc>(c, B_FDESC), (uint32_t)fdesc.size(),
+                     nullptr, 0, dptr<uint32_t>(c, B_CHUNKS), cgrid, s);
+    pend(K_CET_COUNT, a, 0.0, -1);
+  }
   HIPCHK(hipGetLastError());
   if (!fdesc.empty())
     HIPCHK(hipMemcpyAsync(fdesc.data(), dptr<CetDesc>(c, B_FDESC), sizeof(CetDesc) * fdesc.size(),
@@ -527,11 +608,15 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     if (o.use_graph && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) captured = true;
+    hipEvent_t pa = pbegin();
     launch_cet_count(dg, bitmaps, dptr<uint32_t>(c, B_CLIST), dptr<CetDesc>(c, B_JDESC), (uint32_t)jdesc.size(),
                      dptr<uint32_t>(c, B_OFFSETS), 1, dptr<uint32_t>(c, B_CHUNKS), cgrid, s);
+    pend(K_CET_COUNT, pa, 0.0, -1);
     launch_cet_offsets(dptr<CetDesc>(c, B_JDESC), (uint32_t)jdesc.size(), dptr<uint32_t>(c, B_OFFSETS), s);
+    pa = pbegin();
     launch_cet_write(dg, bitmaps, dptr<uint32_t>(c, B_CLIST), dptr<CetDesc>(c, B_JDESC), (uint32_t)jdesc.size(),
                      dptr<uint32_t>(c, B_OFFSETS), dptr<uint32_t>(c, B_LISTS), dptr<uint32_t>(c, B_CHUNKS), cgrid, s);
+    pend(K_CET_WRITE, pa, 0.0, -1);
     // ---------------- phase D: join ----------------
     launch_fill_u32(dptr<uint32_t>(c, B_JOUT), 0u, sizeof(JoinOut) / 4 * (uint64_t)std::max<uint32_t>(nplans, 1), s);
     launch_fill_u32(dptr<uint32_t>(c, B_CTR), 0u, 64, s);
@@ -560,7 +645,9 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       ja.emit_ctr = reinterpret_cast<unsigned long long*>(dptr<uint32_t>(c, B_CTR) + 16);
       ja.emit_cap = emit_cap;
     }
+    pa = pbegin();
     if (!tasks.empty()) launch_join(ja, std::min<uint32_t>(nwg, (uint32_t)tasks.size()), 256, s);
+    pend(K_JOIN, pa, 0.0, -1);
     if (captured) {
       if (hipStreamEndCapture(s, &graph) != hipSuccess || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) !=
                                                               hipSuccess)
@@ -579,6 +666,28 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     if (graph) hipGraphDestroy(graph);
     float ms = 0;
     hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    if (prof) {
+      std::vector<unsigned long long> fs(5 * (maxsteps + 1), 0);
+      HIPCHK(hipMemcpy(fs.data(), fstats, sizeof(unsigned long long) * fs.size(), hipMemcpyDeviceToHost));
+      for (auto& pnd : c->prof.pending) {
+        float kms = 0;
+        hipEventElapsedTime(&kms, pnd.a, pnd.b);
+        double bytes = pnd.bytes;
+        if (pnd.stats_slot >= 0) {
+          const unsigned long long* f = &fs[5 * pnd.stats_slot];
+          // candidates: two row offsets; entries: column index; key gathers;
+          // bitmap probes and bit writes count one bit each
+          bytes += 8.0 * f[0] + 4.0 * f[1] + 4.0 * f[2] + (f[3] + f[4]) / 8.0;
+        }
+        Prof::Acc& acc = c->prof.acc[pnd.kind];
+        acc.launches += 1;
+        acc.ms += kms;
+        acc.bytes += bytes;
+        c->prof.pool.push_back(pnd.a);
+        c->prof.pool.push_back(pnd.b);
+      }
+      c->prof.pending.clear();
+    }
     for (uint32_t b = 0; b < nq; ++b) {
       out[b].status = Q[b].status;
       out[b].device_ms = ms;

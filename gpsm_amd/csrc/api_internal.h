@@ -43,6 +43,7 @@ enum BufId {
   B_LEVELS,
   B_EMIT,
   B_MLP,
+  B_FSTATS,
   B_COUNT_
 };
 
@@ -72,7 +73,34 @@ void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 
 }  // namespace gpsm
 
+namespace gpsm {
+
+// Per-launch timing on the context stream while profiling is on.
+struct Prof {
+  bool on = false;
+  struct Pending {
+    int kind;
+    hipEvent_t a, b;
+    double bytes;
+    int stats_slot;  // >= 0: bytes come from the device counters of this slot
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  struct Acc {
+    uint64_t launches = 0;
+    double ms = 0, bytes = 0;
+  };
+  Acc acc[8];
+  hipEvent_t get();
+};
+
+enum ProfKind { K_FILTER = 0, K_JOIN, K_CET_COUNT, K_CET_WRITE, K_COMPACT, K_INIT, K_OTHER, K_KINDS_ };
+extern const char* const kProfNames[K_KINDS_];
+
+}  // namespace gpsm
+
 struct gpsm_ctx {
+  gpsm::Prof prof;
   int device = 0;
   int ncu = 256;
   hipStream_t stream = nullptr;

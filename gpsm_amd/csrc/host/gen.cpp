@@ -449,7 +449,7 @@ int gpsm_gen_queries(const gpsm_csr* g, uint32_t nq, uint32_t kmin, uint32_t kma
     bool done = false;
     // after every 50 rejected draws the slot's size shrinks by one vertex
     for (int attempt = 0; attempt < 50 * 64 && !done; ++attempt) {
-      if (attempt && attempt % 50 == 0 && k > kmin) --k;
+      if (attempt && attempt % 50 == 0 && k > std::min<uint32_t>(kmin, 3)) --k;
       QueryOut q;
       bool ok;
       if (qkind == GPSM_Q_CYCLE) ok = cycle_query(g, rng, k, q);

@@ -60,7 +60,8 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
                                                           const uint32_t* __restrict__ nbr_slot,
                                                           const uint32_t* __restrict__ nbr_lo,
                                                           const uint32_t* __restrict__ nbr_hi,
-                                                          uint32_t* __restrict__ bitmaps) {
+                                                          uint32_t* __restrict__ bitmaps,
+                                                          unsigned long long* __restrict__ stats) {
   extern __shared__ unsigned char lds_raw[];
   WaveLds& L = reinterpret_cast<WaveLds*>(lds_raw)[threadIdx.x / WAVE];
   const uint32_t lane = lane_id();
@@ -77,6 +78,10 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
     const uint32_t nb = uniform(ops[o].nbr_off);
     uint32_t* bm_u = bitmaps + (size_t)uniform(ops[o].u_slot) * W;
     const unsigned long long full = nv >= 64 ? ~0ULL : ((1ULL << nv) - 1);
+
+    // per-lane tallies for the optional byte accounting: candidates,
+    // adjacency entries read, key gathers, bitmap probes, bit sets
+    uint32_t t_ent = 0, t_key = 0, t_probe = 0, t_set = 0;
 
     // collect: candidates of this chunk, ascending
     const uint32_t wi = c * CHUNK_WORDS + lane;
@@ -112,15 +117,18 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
         if (g < D) {
           const uint32_t ci = seg_search(L.pre, WAVE, g);
           const uint32_t mv = col[L.off[ci] + (g - L.pre[ci])];
+          ++t_ent;
           unsigned long long bits = 0;
           if (type == OP_REFINE) {
             const unsigned long long known = L.hit[ci];
             for (uint32_t i = 0; i < nv; ++i) {
               if ((known >> i) & 1ULL) continue;
               const uint32_t* bm_v = bitmaps + (size_t)nbr_slot[nb + i] * W;
+              ++t_probe;
               if (bit_of(bm_v, mv)) bits |= 1ULL << i;
             }
           } else {
+            ++t_key;
             const uint32_t k = key[mv];
             for (uint32_t i = 0; i < nv; ++i)
               if (k >= nbr_lo[nb + i] && k <= nbr_hi[nb + i]) bits |= 1ULL << i;
@@ -155,15 +163,29 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
               if (lim) {
                 const uint32_t mv = col[L.off[ci] + (g - L.pre[ci])];
                 const uint32_t k = key[mv];
+                ++t_ent;
+                ++t_key;
                 for (uint32_t i = 0; i < lim; ++i)
-                  if (k >= nbr_lo[nb + i] && k <= nbr_hi[nb + i])
+                  if (k >= nbr_lo[nb + i] && k <= nbr_hi[nb + i]) {
                     set_bit(bitmaps + (size_t)nbr_slot[nb + i] * W, mv);
+                    ++t_set;
+                  }
               }
             }
           }
         }
       }
       wave_sync();
+    }
+    if (stats) {
+      const uint32_t e = wave_sum(t_ent), k = wave_sum(t_key), p = wave_sum(t_probe), b = wave_sum(t_set);
+      if (lane == 0) {
+        atomicAdd(stats + 0, (unsigned long long)C);
+        atomicAdd(stats + 1, (unsigned long long)e);
+        atomicAdd(stats + 2, (unsigned long long)k);
+        atomicAdd(stats + 3, (unsigned long long)p);
+        atomicAdd(stats + 4, (unsigned long long)b);
+      }
     }
   }
 }
@@ -181,7 +203,7 @@ void launch_init(const DevGraph& g, const InitOp* ops, uint32_t nops, uint32_t* 
 
 void launch_filter_step(const DevGraph& g, const FilterOp* ops, uint32_t nops, const uint32_t* nbr_slot,
                         const uint32_t* nbr_lo, const uint32_t* nbr_hi, uint32_t* bitmaps, uint32_t block,
-                        hipStream_t s) {
+                        unsigned long long* stats, hipStream_t s) {
   if (!nops) return;
   if (block != 64 && block != 128 && block != 256) block = 256;
   const uint32_t chunks = (g.W + CHUNK_WORDS - 1) / CHUNK_WORDS;
@@ -191,7 +213,7 @@ void launch_filter_step(const DevGraph& g, const FilterOp* ops, uint32_t nops, c
   if (grid > 65536) grid = 65536;
   const size_t lds = sizeof(WaveLds) * wpb;
   hipLaunchKernelGGL(filter_step_kernel, dim3((uint32_t)grid), dim3(block), lds, s, g.row, g.col, g.key, g.W, ops,
-                     nops, nbr_slot, nbr_lo, nbr_hi, bitmaps);
+                     nops, nbr_slot, nbr_lo, nbr_hi, bitmaps, stats);
 }
 
 }  // namespace gpsm

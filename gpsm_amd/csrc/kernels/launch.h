@@ -22,9 +22,11 @@ struct DevGraph {
 // Start-vertex candidates: C(slot) = {v : lo <= key[v] <= hi} (check_kernel).
 void launch_init(const DevGraph& g, const InitOp* ops, uint32_t nops, uint32_t* bitmaps, hipStream_t s);
 // One superstep: every op (one per query) explores or refines its C(u).
+// stats (optional, 5 u64): candidates, adjacency entries read, key gathers,
+// bitmap probes, bits set.
 void launch_filter_step(const DevGraph& g, const FilterOp* ops, uint32_t nops, const uint32_t* nbr_slot,
                         const uint32_t* nbr_lo, const uint32_t* nbr_hi, uint32_t* bitmaps, uint32_t block,
-                        hipStream_t s);
+                        unsigned long long* stats, hipStream_t s);
 
 // scan.hip ---------------------------------------------------------------------
 // counts[r] = popcount of bitmap row rows[r] (all rows when rows == nullptr).

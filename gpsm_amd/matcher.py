@@ -75,6 +75,16 @@ class Context:
     def upload(self, g):
         return ResidentGraph(self, g)
 
+    def profile(self, on=True):
+        check(_lib.lib().gpsm_profile_enable(self.h, int(on)), "gpsm_profile_enable")
+
+    def kernel_stats(self):
+        arr = (_lib.KStat * 16)()
+        n = C.c_uint32(0)
+        check(_lib.lib().gpsm_profile_read(self.h, arr, 16, C.byref(n)), "gpsm_profile_read")
+        return {arr[i].name.decode(): {"launches": int(arr[i].launches), "ms": float(arr[i].total_ms),
+                                       "bytes": float(arr[i].bytes)} for i in range(n.value)}
+
     # --- parity entry points -------------------------------------------------
     def scan(self, a):
         a = u32(a)

@@ -189,6 +189,23 @@ int gpsm_plan_features(uint32_t q_n, const gpsm_query* q, const uint32_t* cand,
                        uint32_t n_labels_total, uint32_t data_n, uint64_t data_adj,
                        const uint8_t* placed, float log_rows, float* feat_out);
 
+/* --- profiling (no reference counterpart; the reference times whole
+ * queries with a millisecond wall clock, util/Util.cpp:21-27) ------------- */
+
+typedef struct gpsm_kstat {
+  char name[32];      /* kernel family */
+  uint64_t launches;
+  double total_ms;    /* sum of HIP-event durations around each launch */
+  double bytes;       /* algorithmic bytes of all launches (0 if not accounted) */
+} gpsm_kstat;
+
+/* Enables (1) or disables (0) per-launch HIP-event timing and device byte
+ * counters on the context's stream; enabling clears the statistics.
+ * Launches captured in a hipGraph are not timed. */
+int gpsm_profile_enable(gpsm_ctx* ctx, int on);
+/* Copies up to max kernel-family statistics; *n receives the number. */
+int gpsm_profile_read(gpsm_ctx* ctx, gpsm_kstat* out, uint32_t max, uint32_t* n);
+
 /* --- set hash ------------------------------------------------------------- */
 
 /* Hash of one tuple t[0..q) in query-id order:
