@@ -98,8 +98,8 @@ SIGNATURES = {
     "gpsm_mlp_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float,
                                       C.POINTER(C.c_float)]),
     "gpsm_set_planner": (C.c_int, [C.c_void_p, C.c_void_p]),
-    "gpsm_plan_features": (C.c_int, [C.c_uint32, C.POINTER(QueryS), C.c_void_p, C.c_void_p, C.c_void_p,
-                                     C.c_uint32, C.c_uint32, C.c_uint64, C.c_void_p, C.c_float, C.c_void_p]),
+    "gpsm_plan_features": (C.c_int, [C.c_void_p, C.POINTER(QueryS), C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_void_p]),
     "gpsm_tuple_hash": (C.c_uint64, [C.c_void_p, C.c_uint32]),
     "gpsm_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "gpsm_profile_read": (C.c_int, [C.c_void_p, C.POINTER(KStat), C.c_uint32, C.POINTER(C.c_uint32)]),

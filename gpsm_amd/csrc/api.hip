@@ -512,7 +512,8 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     for (size_t e = 0; e < E; ++e) S.cet_size[e] = fdesc[fdesc_of_q[b] + e].total;
     bool ok = true;
     if (o.order_mode == GPSM_ORDER_EXPLICIT) {
-      ok = o.explicit_order && order_from_vertices(S.hq, S.cet_size, o.explicit_order, S.plan);
+      ok = o.explicit_order &&
+           order_from_vertices(S.hq, S.cet_size, o.explicit_order + (size_t)b * GPSM_MAX_QV, S.plan);
       if (!ok) S.status = GPSM_EINVAL;
     } else if (o.order_mode == GPSM_ORDER_LEARNED && c->planner) {
       std::vector<uint32_t> cand(S.hq.n);

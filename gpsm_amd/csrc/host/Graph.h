@@ -1,0 +1,48 @@
+// Host graph of the matcher's C++ driver.  Same entry points as the
+// reference's Graph (graph/Graph.h:91-158): vertices with in/out neighbour
+// lists as read from the text format, and the undirected CSR plus inverse
+// label list built by transformToCSR.  Differences: rows are deduplicated
+// (A->B plus B->A is one undirected edge), and storage is std::vector.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+struct Neighbor {
+  int vid;
+  int elb;
+  Neighbor(int v = -1, int e = -1) : vid(v), elb(e) {}
+};
+
+struct Vertex {
+  int label = -1;
+  std::vector<Neighbor> in, out;
+  explicit Vertex(int lb = -1) : label(lb) {}
+  unsigned eSize() const { return (unsigned)(in.size() + out.size()); }
+};
+
+class Graph {
+ public:
+  std::vector<Vertex> vertices;
+  void addVertex(int label);
+  void addEdge(int from, int to, int elb);
+
+  // CSR (valid after transformToCSR)
+  unsigned vertex_num = 0;
+  std::vector<unsigned> vertex_value;             // labels
+  std::vector<unsigned> undirected_row_offset;    // [n+1]
+  std::vector<unsigned> undirected_column_index;  // sorted, deduplicated rows
+  unsigned label_num = 0;
+  std::vector<unsigned> inverse_label;            // distinct labels, ascending
+  std::vector<unsigned> inverse_num;              // vertices per label
+
+  void transformToCSR();
+  // directed, edge-labelled membership (graph/Graph.cpp:107-118)
+  bool isEdgeContained(int from, int to, int label) const;
+  unsigned eSize() const { return undirected_row_offset.empty() ? 0 : undirected_row_offset[vertex_num]; }
+  unsigned vSize() const { return vertex_num; }
+  void getNeighbor(unsigned id, unsigned& num, unsigned& offset) const {
+    offset = undirected_row_offset[id];
+    num = undirected_row_offset[id + 1] - offset;
+  }
+  int getVertexDegree(unsigned id) const { return (int)(undirected_row_offset[id + 1] - undirected_row_offset[id]); }
+};

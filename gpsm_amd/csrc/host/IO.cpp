@@ -1,0 +1,99 @@
+#include "IO.h"
+
+#include <cstring>
+
+IO::IO(const std::string& query, const std::string& data, const std::string& file) {
+  qfp = std::fopen(query.c_str(), "r");
+  dfp = std::fopen(data.c_str(), "r");
+  ofp = std::fopen(file.c_str(), "w+");
+  if (!qfp || !dfp || !ofp) std::fprintf(stderr, "IO: cannot open %s\n", !qfp ? query.c_str() : !dfp ? data.c_str() : file.c_str());
+}
+
+IO::~IO() {
+  if (qfp) std::fclose(qfp);
+  if (dfp) std::fclose(dfp);
+  if (ofp) std::fclose(ofp);
+}
+
+Graph* IO::input(FILE* fp) {
+  err = false;
+  if (!fp) {
+    err = true;
+    return nullptr;
+  }
+  Graph* g = nullptr;
+  char buf[512];
+  long pos = std::ftell(fp);
+  while (std::fgets(buf, sizeof buf, fp)) {
+    char* s = buf;
+    while (*s == ' ' || *s == '\t') ++s;
+    if (*s == '\n' || *s == '\r' || *s == 0) {
+      pos = std::ftell(fp);
+      continue;
+    }
+    if (*s == 't') {
+      if (g) {  // next graph starts: rewind to this line
+        std::fseek(fp, pos, SEEK_SET);
+        return g;
+      }
+      int id = 0;
+      if (std::sscanf(s, "t # %d", &id) != 1) break;
+      if (id == -1) return nullptr;
+      g = new Graph;
+      // "n m Lv Le" header line
+      if (!std::fgets(buf, sizeof buf, fp)) break;
+    } else if (*s == 'v') {
+      int id, lb;
+      if (!g || std::sscanf(s, "v %d %d", &id, &lb) != 2 || id != (int)g->vertices.size()) break;
+      g->addVertex(lb);
+    } else if (*s == 'e') {
+      int a, b, lb = 0;
+      if (!g || std::sscanf(s, "e %d %d %d", &a, &b, &lb) < 2) break;
+      if (a < 0 || b < 0 || a >= (int)g->vertices.size() || b >= (int)g->vertices.size()) break;
+      g->addEdge(a, b, lb);
+    } else {
+      break;
+    }
+    pos = std::ftell(fp);
+  }
+  // malformed line, or end of file before "t # -1"
+  delete g;
+  err = true;
+  return nullptr;
+}
+
+bool IO::input(std::vector<Graph*>& query_list) {
+  for (;;) {
+    Graph* g = input(qfp);
+    if (!g) return !err;
+    g->transformToCSR();
+    query_list.push_back(g);
+  }
+}
+
+bool IO::input(Graph*& data_graph) {
+  data_graph = input(dfp);
+  if (!data_graph) return false;
+  ++data_id;
+  data_graph->transformToCSR();
+  return true;
+}
+
+bool IO::output(int qid) {
+  std::fprintf(ofp, "query graph:%d    data graph:%d\n%s\n", qid, data_id, line.c_str());
+  return true;
+}
+
+bool IO::output(const unsigned* final_result, unsigned rows, unsigned cols, const int* id_map, Graph*, Graph*) {
+  for (unsigned i = 0; i < rows; ++i) {
+    const unsigned* ans = final_result + (size_t)i * cols;
+    for (unsigned j = 0; j < cols; ++j) std::fprintf(ofp, "(%u, %u) ", j, ans[id_map[j]]);
+    std::fprintf(ofp, "\n");
+  }
+  std::fprintf(ofp, "\n\n\n");
+  return true;
+}
+
+void IO::flush() {
+  if (ofp) std::fflush(ofp);
+}

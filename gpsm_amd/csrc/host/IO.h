@@ -1,0 +1,39 @@
+// Text input/output of the C++ driver, same entry points as the reference's
+// IO (io/IO.h:15-39).  Format (io/IO.cpp:46-100): "t # id" opens a graph,
+// then "n m Lv Le", "v id label" lines and "e from to label" lines; "t # -1"
+// ends the stream.  Unlike the reference, a missing terminator or a
+// malformed line is an error (the reference loops forever on the former).
+#pragma once
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "Graph.h"
+
+class IO {
+ public:
+  IO() = default;
+  IO(const std::string& query, const std::string& data, const std::string& file);
+  ~IO();
+  bool ok() const { return qfp && dfp && ofp; }
+  // all query graphs (CSR built)
+  bool input(std::vector<Graph*>& query_list);
+  // next data graph (CSR built); false at the terminator or on error
+  bool input(Graph*& data_graph);
+  // next graph of a stream; nullptr at the terminator, sets error() on failure
+  Graph* input(FILE* fp);
+  bool error() const { return err; }
+  bool output(int qid);
+  // one line per embedding: "(query id, data id)" pairs in query-id order
+  bool output(const unsigned* final_result, unsigned rows, unsigned cols, const int* id_map, Graph* query,
+              Graph* data);
+  void flush();
+
+ private:
+  int data_id = -1;
+  FILE* qfp = nullptr;
+  FILE* dfp = nullptr;
+  FILE* ofp = nullptr;
+  bool err = false;
+  std::string line = "============================================================";
+};

@@ -1,0 +1,113 @@
+#include "Match.h"
+
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <vector>
+
+namespace {
+
+gpsm_ctx* g_ctx = nullptr;
+std::map<const Graph*, gpsm_graph*> g_resident;  // data graphs uploaded once
+
+// The query as the reference lists it: every in-edge of every vertex
+// (io/IO.cpp:77-86 builds in/out lists in file order).
+struct QueryArrays {
+  std::vector<uint32_t> labels, src, dst;
+  gpsm_query q{};
+  explicit QueryArrays(const Graph* g) {
+    for (const Vertex& v : g->vertices) labels.push_back((uint32_t)v.label);
+    for (size_t i = 0; i < g->vertices.size(); ++i)
+      for (const Neighbor& nb : g->vertices[i].out) {
+        src.push_back((uint32_t)i);
+        dst.push_back((uint32_t)nb.vid);
+      }
+    q.n = (uint32_t)labels.size();
+    q.m = (uint32_t)src.size();
+    q.labels = labels.data();
+    q.src = src.data();
+    q.dst = dst.data();
+  }
+};
+
+}  // namespace
+
+bool Match::initGPU(int dev) {
+  if (g_ctx) return true;
+  int rc = gpsm_init(dev, &g_ctx);
+  if (rc != GPSM_OK) {
+    std::fprintf(stderr, "initGPU: %s\n", gpsm_strerror(rc));
+    g_ctx = nullptr;
+    return false;
+  }
+  std::printf("Using device %d: %s\n", dev, gpsm_device_name(g_ctx));
+  return true;
+}
+
+void Match::releaseGPU() {
+  for (auto& kv : g_resident) gpsm_graph_free(kv.second);
+  g_resident.clear();
+  gpsm_shutdown(g_ctx);
+  g_ctx = nullptr;
+}
+
+Match::Match(Graph* q, Graph* d) : query(q), data(d) {}
+
+Match::~Match() { delete[] id2pos; }
+
+gpsm_graph* Match::resident() {
+  auto it = g_resident.find(data);
+  if (it != g_resident.end()) return it->second;
+  gpsm_graph* g = nullptr;
+  status = gpsm_graph_upload(g_ctx, data->vertex_num, data->undirected_row_offset.data(),
+                             data->undirected_column_index.data(), data->vertex_value.data(), &g);
+  if (status != GPSM_OK) return nullptr;
+  g_resident[data] = g;
+  return g;
+}
+
+int Match::count(uint64_t& embeddings, uint64_t& hash) {
+  embeddings = hash = 0;
+  if (!g_ctx) return status = GPSM_ENODEV;
+  gpsm_graph* g = resident();
+  if (!g) return status;
+  QueryArrays qa(query);
+  gpsm_result r;
+  status = gpsm_match(g_ctx, g, &qa.q, nullptr, &r);
+  embeddings = r.count;
+  hash = r.hash;
+  return status;
+}
+
+void Match::match(IO&, unsigned*& final_result, unsigned& result_row_num, unsigned& result_col_num, int*& id_map) {
+  final_result = nullptr;
+  result_row_num = 0;
+  result_col_num = query->vertex_num;
+  delete[] id2pos;
+  id2pos = new int[query->vertex_num];
+  for (unsigned i = 0; i < query->vertex_num; ++i) id2pos[i] = (int)i;
+  id_map = id2pos;
+  if (!g_ctx) {
+    status = GPSM_ENODEV;
+    return;
+  }
+  gpsm_graph* g = resident();
+  if (!g) return;
+  QueryArrays qa(query);
+  gpsm_opts o{};
+  o.want_tuples = 1;
+  o.tuple_cap = 1ull << 28;
+  gpsm_result r;
+  status = gpsm_match(g_ctx, g, &qa.q, &o, &r);
+  if (status != GPSM_OK) {
+    std::fprintf(stderr, "match: %s\n", gpsm_strerror(status));
+    gpsm_result_free(&r);
+    return;
+  }
+  result_row_num = (unsigned)r.n_tuples;
+  if (result_row_num) {
+    final_result = new unsigned[(size_t)result_row_num * result_col_num];
+    std::memcpy(final_result, r.tuples, sizeof(unsigned) * (size_t)result_row_num * result_col_num);
+  }
+  gpsm_result_free(&r);
+}

@@ -19,7 +19,7 @@ namespace gpsm {
 
 namespace {
 
-constexpr int CHUNK_WORDS = 64;  // bitmap words per work item (one per lane)
+constexpr int CHUNK_WORDS = 32;  // bitmap words per work item (one per lane)
 constexpr int MAX_CAND = CHUNK_WORDS * 32;
 
 __global__ void init_kernel(const uint32_t* __restrict__ key, uint32_t n, uint32_t W, const InitOp* __restrict__ ops,
@@ -45,13 +45,42 @@ __global__ void init_kernel(const uint32_t* __restrict__ key, uint32_t n, uint32
   }
 }
 
+constexpr int UNROLL = 4;  // adjacency entries in flight per lane
+
 struct WaveLds {
   uint32_t cand[MAX_CAND];
   uint32_t pre[WAVE + 1];
-  uint32_t off[WAVE];
+  uint32_t base[WAVE];
   unsigned long long hit[WAVE];
   uint32_t lim[WAVE];
 };
+
+// Neighbour constraints of one adjacency entry mv: refine probes C(v_i)
+// for the constraints not yet met; explore compares mv's key with v_i's
+// label/degree range.
+__device__ __forceinline__ unsigned long long constraint_bits(uint32_t type, uint32_t nv, uint32_t nb, uint32_t mv,
+                                                              unsigned long long known,
+                                                              const uint32_t* __restrict__ key,
+                                                              const uint32_t* __restrict__ nbr_slot,
+                                                              const uint32_t* __restrict__ nbr_lo,
+                                                              const uint32_t* __restrict__ nbr_hi,
+                                                              const uint32_t* __restrict__ bitmaps, uint32_t W,
+                                                              uint32_t& t_key, uint32_t& t_probe) {
+  unsigned long long bits = 0;
+  if (type == OP_REFINE) {
+    for (uint32_t i = 0; i < nv; ++i) {
+      if ((known >> i) & 1ULL) continue;
+      ++t_probe;
+      if (bit_of(bitmaps + (size_t)nbr_slot[nb + i] * W, mv)) bits |= 1ULL << i;
+    }
+  } else {
+    ++t_key;
+    const uint32_t k = key[mv];
+    for (uint32_t i = 0; i < nv; ++i)
+      if (k >= nbr_lo[nb + i] && k <= nbr_hi[nb + i]) bits |= 1ULL << i;
+  }
+  return bits;
+}
 
 __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __restrict__ row,
                                                           const uint32_t* __restrict__ col,
@@ -78,21 +107,19 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
     const uint32_t nb = uniform(ops[o].nbr_off);
     uint32_t* bm_u = bitmaps + (size_t)uniform(ops[o].u_slot) * W;
     const unsigned long long full = nv >= 64 ? ~0ULL : ((1ULL << nv) - 1);
-
-    // per-lane tallies for the optional byte accounting: candidates,
-    // adjacency entries read, key gathers, bitmap probes, bit sets
+    // per-lane tallies for the optional byte accounting
     uint32_t t_ent = 0, t_key = 0, t_probe = 0, t_set = 0;
 
     // collect: candidates of this chunk, ascending
     const uint32_t wi = c * CHUNK_WORDS + lane;
-    const uint32_t w = wi < W ? bm_u[wi] : 0u;
+    const uint32_t w = (lane < CHUNK_WORDS && wi < W) ? bm_u[wi] : 0u;
     const uint32_t pc = __popc(w);
     const uint32_t incl = wave_incl_scan(pc);
     const uint32_t C = uniform(__shfl(incl, WAVE - 1, WAVE));
     {
       uint32_t pos = incl - pc, x = w;
       while (x) {
-        uint32_t b = __ffs(x) - 1;
+        const uint32_t b = __ffs(x) - 1;
         x &= x - 1;
         L.cand[pos++] = wi * 32 + b;
       }
@@ -100,50 +127,55 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
     wave_sync();
 
     for (uint32_t s0 = 0; s0 < C; s0 += WAVE) {
+      // lane = one candidate of the sub-chunk
       const bool mine = s0 + lane < C;
       const uint32_t id = mine ? L.cand[s0 + lane] : 0u;
       const uint32_t off = mine ? row[id] : 0u;
       const uint32_t deg = mine ? row[id + 1] - off : 0u;
-      const uint32_t dincl = wave_incl_scan(deg);
-      const uint32_t D = uniform(__shfl(dincl, WAVE - 1, WAVE));
-      L.pre[lane] = dincl - deg;
-      L.off[lane] = off;
       L.hit[lane] = 0;
+      uint32_t cur = 0;  // entries of this candidate already examined
+      unsigned long long h = 0;
       wave_sync();
 
-      // pass 1: which neighbour constraints each candidate satisfies
-      for (uint32_t f = 0; f < D; f += WAVE) {
-        const uint32_t g = f + lane;
-        if (g < D) {
-          const uint32_t ci = seg_search(L.pre, WAVE, g);
-          const uint32_t mv = col[L.off[ci] + (g - L.pre[ci])];
-          ++t_ent;
-          unsigned long long bits = 0;
-          if (type == OP_REFINE) {
-            const unsigned long long known = L.hit[ci];
-            for (uint32_t i = 0; i < nv; ++i) {
-              if ((known >> i) & 1ULL) continue;
-              const uint32_t* bm_v = bitmaps + (size_t)nbr_slot[nb + i] * W;
-              ++t_probe;
-              if (bit_of(bm_v, mv)) bits |= 1ULL << i;
-            }
-          } else {
-            ++t_key;
-            const uint32_t k = key[mv];
-            for (uint32_t i = 0; i < nv; ++i)
-              if (k >= nbr_lo[nb + i] && k <= nbr_hi[nb + i]) bits |= 1ULL << i;
+      // pass 1: constraints met by each candidate.  Every step re-flattens
+      // the remaining adjacency of the candidates not yet satisfying all of
+      // them, so a satisfied candidate stops costing lanes.
+      for (;;) {
+        const uint32_t rem = (mine && h != full) ? deg - cur : 0u;
+        const uint32_t rincl = wave_incl_scan(rem);
+        const uint32_t T = uniform(__shfl(rincl, WAVE - 1, WAVE));
+        if (T == 0) break;
+        const uint32_t rpre = rincl - rem;
+        L.pre[lane] = rpre;
+        L.base[lane] = off + cur;
+        wave_sync();
+        uint32_t mvs[UNROLL], cis[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+          const uint32_t g = u * WAVE + lane;
+          cis[u] = WAVE;
+          if (g < T) {
+            const uint32_t ci = seg_search(L.pre, WAVE, g);
+            cis[u] = ci;
+            mvs[u] = col[L.base[ci] + (g - L.pre[ci])];
           }
-          if (bits) atomicOr(&L.hit[ci], bits);
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+          if (cis[u] == WAVE) continue;
+          ++t_ent;
+          const unsigned long long bits =
+              constraint_bits(type, nv, nb, mvs[u], L.hit[cis[u]], key, nbr_slot, nbr_lo, nbr_hi, bitmaps, W, t_key,
+                              t_probe);
+          if (bits) atomicOr(&L.hit[cis[u]], bits);
         }
         wave_sync();
-        if (type == OP_REFINE) {
-          // stop once every candidate of the sub-chunk has all its neighbours
-          const bool done = !mine || L.hit[lane] == full;
-          if (__ballot(!done) == 0ULL) break;
-        }
+        // entries of this lane's candidate covered by the UNROLL*64 examined
+        const uint32_t span = UNROLL * WAVE;
+        cur += rpre >= span ? 0u : min(rem, span - rpre);
+        h = L.hit[lane];
       }
-      wave_sync();
-      const unsigned long long h = L.hit[lane];
+
       if (type == OP_REFINE) {
         if (mine && h != full) clear_bit(bm_u, id);
       } else {
@@ -151,27 +183,39 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
         // it received all of mu's matching data neighbours (match/Match.cu:547-600)
         const uint32_t ff = h == full ? nv : (uint32_t)(__ffsll((long long)~h) - 1);
         if (mine && ff < nv) clear_bit(bm_u, id);
-        L.lim[lane] = mine ? ff : 0u;
-        const bool any = __ballot(mine && ff > 0) != 0ULL;
+        const uint32_t lim = mine ? ff : 0u;
+        const uint32_t len = lim ? deg : 0u;
+        const uint32_t lincl = wave_incl_scan(len);
+        const uint32_t T = uniform(__shfl(lincl, WAVE - 1, WAVE));
+        L.pre[lane] = lincl - len;
+        L.base[lane] = off;
+        L.lim[lane] = lim;
         wave_sync();
-        if (any) {
-          for (uint32_t f = 0; f < D; f += WAVE) {
-            const uint32_t g = f + lane;
-            if (g < D) {
+        // pass 2: every matching data neighbour of a surviving prefix enters C(v_i)
+        for (uint32_t f = 0; f < T; f += UNROLL * WAVE) {
+          uint32_t mvs[UNROLL], cis[UNROLL];
+#pragma unroll
+          for (int u = 0; u < UNROLL; ++u) {
+            const uint32_t g = f + u * WAVE + lane;
+            cis[u] = WAVE;
+            if (g < T) {
               const uint32_t ci = seg_search(L.pre, WAVE, g);
-              const uint32_t lim = L.lim[ci];
-              if (lim) {
-                const uint32_t mv = col[L.off[ci] + (g - L.pre[ci])];
-                const uint32_t k = key[mv];
-                ++t_ent;
-                ++t_key;
-                for (uint32_t i = 0; i < lim; ++i)
-                  if (k >= nbr_lo[nb + i] && k <= nbr_hi[nb + i]) {
-                    set_bit(bitmaps + (size_t)nbr_slot[nb + i] * W, mv);
-                    ++t_set;
-                  }
-              }
+              cis[u] = ci;
+              mvs[u] = col[L.base[ci] + (g - L.pre[ci])];
             }
+          }
+#pragma unroll
+          for (int u = 0; u < UNROLL; ++u) {
+            if (cis[u] == WAVE) continue;
+            const uint32_t k = key[mvs[u]];
+            const uint32_t lm = L.lim[cis[u]];
+            ++t_ent;
+            ++t_key;
+            for (uint32_t i = 0; i < lm; ++i)
+              if (k >= nbr_lo[nb + i] && k <= nbr_hi[nb + i]) {
+                set_bit(bitmaps + (size_t)nbr_slot[nb + i] * W, mvs[u]);
+                ++t_set;
+              }
           }
         }
       }

@@ -1,15 +1,458 @@
-// placeholder, replaced by the MFMA implementation
+// Learned join-order planner network on gfx950: MLP 32 -> 128 -> 128 -> 1
+// (ReLU), BF16 weights, FP32 accumulation, trained with Adam.  No reference
+// counterpart (the reference orders joins with a fixed heuristic,
+// match/Match.cu:1599-1661).
+//
+// One wave handles 32 samples.  Both hidden layers and all three weight
+// gradients that are GEMMs run on v_mfma_f32_32x32x16_bf16; the output
+// layer (a 128-wide dot product per sample) and the bias gradients are FP32
+// FMAs.  BF16 rounding points (round to nearest even): inputs, weights,
+// hidden activations h1 (operand of layer 2), and the deltas d2, d1 where
+// they are GEMM operands.  Accumulation order differs from a sequential sum,
+// so results agree with an exact evaluation to FP32 rounding.
+//
+// 32x32x16 MFMA operand layout (64 lanes, l = lane):
+//   A (32 x 16): lane l holds row l%32, k = 8*(l/32) + 0..7
+//   B (16 x 32): lane l holds col l%32, k = 8*(l/32) + 0..7
+//   C (32 x 32): lane l, register r holds row 8*(r/4) + 4*(l/32) + r%4, col l%32
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
 #include "api_internal.h"
-extern "C" {
-struct gpsm_mlp { int dummy; };
-int gpsm_mlp_create(gpsm_ctx*, uint64_t, gpsm_mlp** out) { if (out) *out = nullptr; return GPSM_EINVAL; }
-void gpsm_mlp_free(gpsm_mlp*) {}
-uint64_t gpsm_mlp_num_params(void) { return 0; }
-int gpsm_mlp_get_params(gpsm_mlp*, float*) { return GPSM_EINVAL; }
-int gpsm_mlp_set_params(gpsm_mlp*, const float*) { return GPSM_EINVAL; }
-int gpsm_mlp_forward(gpsm_mlp*, const float*, uint32_t, float*) { return GPSM_EINVAL; }
-int gpsm_mlp_grad(gpsm_mlp*, const float*, const float*, uint32_t, float*, float*) { return GPSM_EINVAL; }
-int gpsm_mlp_train_step(gpsm_mlp*, const float*, const float*, uint32_t, float, float*) { return GPSM_EINVAL; }
-int gpsm_set_planner(gpsm_ctx*, gpsm_mlp*) { return GPSM_EINVAL; }
-int gpsm_plan_features(uint32_t, const gpsm_query*, const uint32_t*, const uint32_t*, const uint32_t*, uint32_t, uint32_t, uint64_t, const uint8_t*, float, float*) { return GPSM_EINVAL; }
+#include "rng.h"
+#include "wave.h"
+
+using namespace gpsm;
+
+namespace {
+
+constexpr int F = GPSM_FEAT, H = GPSM_HIDDEN;
+constexpr int OW1 = 0, OB1 = H * F, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3 = OB2 + H, OB3 = OW3 + H;
+constexpr int NP = OB3 + 1;
+constexpr int XS = F + 8;  // LDS row strides (bf16), padded against bank conflicts
+constexpr int HS = H + 8;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ uint16_t bf_bits(float f) {
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf / nan
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
 }
+__device__ __forceinline__ __bf16 to_bf(float f) { return __builtin_bit_cast(__bf16, bf_bits(f)); }
+__device__ __forceinline__ float from_bf(__bf16 h) {
+  return __builtin_bit_cast(float, (uint32_t)__builtin_bit_cast(uint16_t, h) << 16);
+}
+__device__ __forceinline__ float round_bf(float f) { return from_bf(to_bf(f)); }
+
+__device__ __forceinline__ f32x16 mfma(const bf16x8& a, const bf16x8& b, const f32x16& c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ uint32_t c_row(uint32_t r, uint32_t lane) { return 8 * (r / 4) + 4 * (lane / 32) + (r % 4); }
+
+struct Weights {
+  const __bf16* w1;   // [H][F]
+  const __bf16* w2;   // [H][H]
+  const __bf16* w2t;  // [H][H] transposed
+  const float* master;
+};
+
+struct TileLds {
+  __bf16 xs[32 * XS];
+  __bf16 h1[32 * HS];
+  __bf16 d[32 * HS];
+  float y[32];
+};
+
+// Forward of 32 samples; returns z2 (pre-activation, bias added) in regs.
+__device__ void forward_tile(const Weights& W, const float* __restrict__ x, uint32_t b0, uint32_t B, TileLds& L,
+                             f32x16 z2[4]) {
+  const uint32_t lane = lane_id();
+  // inputs -> bf16 in LDS (zeros past B)
+  for (uint32_t i = lane; i < 32 * F; i += WAVE) {
+    const uint32_t s = i / F, j = i % F;
+    const float v = (b0 + s < B) ? x[(size_t)(b0 + s) * F + j] : 0.0f;
+    L.xs[s * XS + j] = to_bf(v);
+  }
+  wave_sync();
+  const uint32_t rl = lane % 32, kh = 8 * (lane / 32);
+  // layer 1: z1 = x W1^T + b1, h1 = bf16(relu(z1))
+  for (int t = 0; t < H / 32; ++t) {
+    f32x16 acc = {};
+    for (int s = 0; s < F / 16; ++s) {
+      bf16x8 a, b;
+      for (int j = 0; j < 8; ++j) {
+        a[j] = L.xs[rl * XS + 16 * s + kh + j];
+        b[j] = W.w1[(32 * t + rl) * F + 16 * s + kh + j];
+      }
+      acc = mfma(a, b, acc);
+    }
+    const uint32_t colh = 32 * t + rl;
+    const float bias = W.master[OB1 + colh];
+    for (int r = 0; r < 16; ++r) L.h1[c_row(r, lane) * HS + colh] = to_bf(fmaxf(acc[r] + bias, 0.0f));
+  }
+  wave_sync();
+  // layer 2: z2 = h1 W2^T + b2
+  for (int t = 0; t < H / 32; ++t) {
+    f32x16 acc = {};
+    for (int s = 0; s < H / 16; ++s) {
+      bf16x8 a, b;
+      for (int j = 0; j < 8; ++j) {
+        a[j] = L.h1[rl * HS + 16 * s + kh + j];
+        b[j] = W.w2[(32 * t + rl) * H + 16 * s + kh + j];
+      }
+      acc = mfma(a, b, acc);
+    }
+    const float bias = W.master[OB2 + 32 * t + rl];
+    for (int r = 0; r < 16; ++r) acc[r] += bias;
+    z2[t] = acc;
+  }
+  // layer 3: y = relu(z2) . bf16(W3) + b3; per-lane partial sums over its
+  // 32 columns, then a butterfly over the 32 lanes sharing those rows
+  float part[16];
+  for (int r = 0; r < 16; ++r) part[r] = 0.0f;
+  for (int t = 0; t < H / 32; ++t) {
+    const float w3 = round_bf(W.master[OW3 + 32 * t + rl]);
+    for (int r = 0; r < 16; ++r) part[r] += fmaxf(z2[t][r], 0.0f) * w3;
+  }
+  for (int r = 0; r < 16; ++r)
+    for (int d = 1; d < 32; d <<= 1) part[r] += __shfl_xor(part[r], d, WAVE);
+  if (rl == 0)
+    for (int r = 0; r < 16; ++r) L.y[c_row(r, lane)] = part[r] + W.master[OB3];
+  wave_sync();
+}
+
+__global__ void __launch_bounds__(64) mlp_forward_kernel(Weights W, const float* __restrict__ x, uint32_t B,
+                                                         float* __restrict__ y) {
+  __shared__ TileLds L;
+  const uint32_t b0 = blockIdx.x * 32;
+  f32x16 z2[4];
+  forward_tile(W, x, b0, B, L, z2);
+  const uint32_t lane = lane_id();
+  if (lane < 32 && b0 + lane < B) y[b0 + lane] = L.y[lane];
+}
+
+// Forward + backward of 32 samples of the loss 0.5*mean((y-t)^2); gradients
+// are added atomically into grad (FP32, parameter order).
+__global__ void __launch_bounds__(64) mlp_grad_kernel(Weights W, const float* __restrict__ x,
+                                                      const float* __restrict__ tgt, uint32_t B,
+                                                      float* __restrict__ grad, float* __restrict__ loss) {
+  __shared__ TileLds L;
+  __shared__ float dys[32];
+  const uint32_t lane = lane_id(), rl = lane % 32, kh = 8 * (lane / 32);
+  const uint32_t b0 = blockIdx.x * 32;
+  f32x16 z2[4];
+  forward_tile(W, x, b0, B, L, z2);
+  // dL/dy and the loss
+  float lpart = 0.0f;
+  if (lane < 32) {
+    float dy = 0.0f;
+    if (b0 + lane < B) {
+      const float e = L.y[lane] - tgt[b0 + lane];
+      dy = e / (float)B;
+      lpart = 0.5f * e * e / (float)B;
+    }
+    dys[lane] = dy;
+  }
+  wave_sync();
+  lpart = wave_sum_f(lpart);
+  float gb3 = lane < 32 ? dys[lane] : 0.0f;
+  gb3 = wave_sum_f(gb3);
+  if (lane == 0) {
+    atomicAdd(loss, lpart);
+    atomicAdd(&grad[OB3], gb3);
+  }
+  // output layer: gW3 = sum_b dy h2; d2 = [z2 > 0] dy bf16(W3); gb2 = sum_b d2
+  for (int t = 0; t < H / 32; ++t) {
+    const uint32_t colh = 32 * t + rl;
+    const float w3 = round_bf(W.master[OW3 + colh]);
+    float gw3 = 0.0f, gb2 = 0.0f;
+    for (int r = 0; r < 16; ++r) {
+      const uint32_t row = c_row(r, lane);
+      const float dy = dys[row];
+      const float z = z2[t][r];
+      gw3 += dy * fmaxf(z, 0.0f);
+      const float d2 = z > 0.0f ? dy * w3 : 0.0f;
+      gb2 += d2;
+      L.d[row * HS + colh] = to_bf(d2);
+    }
+    gw3 += __shfl_xor(gw3, 32, WAVE);
+    gb2 += __shfl_xor(gb2, 32, WAVE);
+    if (lane < 32) {
+      atomicAdd(&grad[OW3 + colh], gw3);
+      atomicAdd(&grad[OB2 + colh], gb2);
+    }
+  }
+  wave_sync();
+  // gW2[i][j] += sum_b d2[b][i] h1[b][j]   (K = 32 samples)
+  for (int it = 0; it < H / 32; ++it)
+    for (int jt = 0; jt < H / 32; ++jt) {
+      f32x16 acc = {};
+      for (int s = 0; s < 2; ++s) {
+        bf16x8 a, b;
+        for (int j = 0; j < 8; ++j) {
+          const uint32_t bs = 16 * s + kh + j;
+          a[j] = L.d[bs * HS + 32 * it + rl];
+          b[j] = L.h1[bs * HS + 32 * jt + rl];
+        }
+        acc = mfma(a, b, acc);
+      }
+      for (int r = 0; r < 16; ++r) atomicAdd(&grad[OW2 + (32 * it + c_row(r, lane)) * H + 32 * jt + rl], acc[r]);
+    }
+  // dh1[b][j] = sum_i d2[b][i] W2[i][j];  d1 = [h1 > 0] dh1
+  f32x16 dh[4];
+  for (int t = 0; t < H / 32; ++t) {
+    f32x16 acc = {};
+    for (int s = 0; s < H / 16; ++s) {
+      bf16x8 a, b;
+      for (int j = 0; j < 8; ++j) {
+        a[j] = L.d[rl * HS + 16 * s + kh + j];
+        b[j] = W.w2t[(32 * t + rl) * H + 16 * s + kh + j];
+      }
+      acc = mfma(a, b, acc);
+    }
+    dh[t] = acc;
+  }
+  wave_sync();
+  for (int t = 0; t < H / 32; ++t) {
+    const uint32_t colh = 32 * t + rl;
+    float gb1 = 0.0f;
+    for (int r = 0; r < 16; ++r) {
+      const uint32_t row = c_row(r, lane);
+      const float d1 = from_bf(L.h1[row * HS + colh]) > 0.0f ? dh[t][r] : 0.0f;
+      gb1 += d1;
+      L.d[row * HS + colh] = to_bf(d1);
+    }
+    gb1 += __shfl_xor(gb1, 32, WAVE);
+    if (lane < 32) atomicAdd(&grad[OB1 + colh], gb1);
+  }
+  wave_sync();
+  // gW1[i][j] += sum_b d1[b][i] x[b][j]
+  for (int it = 0; it < H / 32; ++it) {
+    f32x16 acc = {};
+    for (int s = 0; s < 2; ++s) {
+      bf16x8 a, b;
+      for (int j = 0; j < 8; ++j) {
+        const uint32_t bs = 16 * s + kh + j;
+        a[j] = L.d[bs * HS + 32 * it + rl];
+        b[j] = L.xs[bs * XS + rl];
+      }
+      acc = mfma(a, b, acc);
+    }
+    for (int r = 0; r < 16; ++r) atomicAdd(&grad[OW1 + (32 * it + c_row(r, lane)) * F + rl], acc[r]);
+  }
+}
+
+// BF16 copies of the weights (and W2 transposed) from the FP32 masters.
+__global__ void refresh_kernel(const float* __restrict__ m, __bf16* __restrict__ w1, __bf16* __restrict__ w2,
+                               __bf16* __restrict__ w2t) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < H * F) w1[i] = to_bf(m[OW1 + i]);
+  if (i < H * H) {
+    const uint32_t r = i / H, c = i % H;
+    const __bf16 v = to_bf(m[OW2 + i]);
+    w2[i] = v;
+    w2t[c * H + r] = v;
+  }
+}
+
+__global__ void adam_kernel(float* __restrict__ p, float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, float lr, float c1, float c2) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= NP) return;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  g[i] = 0.0f;
+}
+
+}  // namespace
+
+struct gpsm_mlp {
+  gpsm_ctx* ctx = nullptr;
+  float* p = nullptr;  // FP32 masters [NP]
+  float* g = nullptr;  // gradients
+  float* m = nullptr;
+  float* v = nullptr;
+  __bf16* w1 = nullptr;
+  __bf16* w2 = nullptr;
+  __bf16* w2t = nullptr;
+  float* loss = nullptr;
+  DevBuf x, t, y;
+  int step = 0;
+};
+
+namespace {
+
+Weights weights_of(const gpsm_mlp* mm) { return Weights{mm->w1, mm->w2, mm->w2t, mm->p}; }
+
+int refresh(gpsm_mlp* mm) {
+  hipLaunchKernelGGL(refresh_kernel, dim3((H * H + 255) / 256), dim3(256), 0, mm->ctx->stream, mm->p, mm->w1, mm->w2,
+                     mm->w2t);
+  return hipGetLastError() == hipSuccess ? GPSM_OK : GPSM_EDEVICE;
+}
+
+int stage(gpsm_mlp* mm, const float* x, const float* t, uint32_t B) {
+  hipStream_t s = mm->ctx->stream;
+  if (mm->x.ensure(sizeof(float) * (size_t)B * F) || mm->y.ensure(sizeof(float) * B) ||
+      (t && mm->t.ensure(sizeof(float) * B)))
+    return GPSM_ENOMEM;
+  if (hipMemcpyAsync(mm->x.ptr, x, sizeof(float) * (size_t)B * F, hipMemcpyHostToDevice, s) != hipSuccess)
+    return GPSM_EDEVICE;
+  if (t && hipMemcpyAsync(mm->t.ptr, t, sizeof(float) * B, hipMemcpyHostToDevice, s) != hipSuccess)
+    return GPSM_EDEVICE;
+  return GPSM_OK;
+}
+
+int run_grad(gpsm_mlp* mm, uint32_t B) {
+  hipStream_t s = mm->ctx->stream;
+  hipMemsetAsync(mm->loss, 0, sizeof(float), s);
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3((B + 31) / 32), dim3(64), 0, s, weights_of(mm),
+                     reinterpret_cast<const float*>(mm->x.ptr), reinterpret_cast<const float*>(mm->t.ptr), B, mm->g,
+                     mm->loss);
+  return hipGetLastError() == hipSuccess ? GPSM_OK : GPSM_EDEVICE;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t gpsm_mlp_num_params(void) { return NP; }
+
+int gpsm_mlp_create(gpsm_ctx* c, uint64_t seed, gpsm_mlp** out) {
+  if (!c || !out) return GPSM_EINVAL;
+  *out = nullptr;
+  hipSetDevice(c->device);
+  gpsm_mlp* mm = new gpsm_mlp();
+  mm->ctx = c;
+  if (hipMalloc(&mm->p, sizeof(float) * NP) || hipMalloc(&mm->g, sizeof(float) * NP) ||
+      hipMalloc(&mm->m, sizeof(float) * NP) || hipMalloc(&mm->v, sizeof(float) * NP) ||
+      hipMalloc(&mm->w1, sizeof(__bf16) * H * F) || hipMalloc(&mm->w2, sizeof(__bf16) * H * H) ||
+      hipMalloc(&mm->w2t, sizeof(__bf16) * H * H) || hipMalloc(&mm->loss, sizeof(float))) {
+    gpsm_mlp_free(mm);
+    return GPSM_ENOMEM;
+  }
+  // He-style uniform initialisation from a SplitMix64 stream
+  std::vector<float> p(NP, 0.0f);
+  SplitMix64 rng(seed, 99);
+  auto fill = [&](int off, int n, int fan_in) {
+    const double a = std::sqrt(6.0 / fan_in);
+    for (int i = 0; i < n; ++i) p[off + i] = (float)((2.0 * rng.uniform() - 1.0) * a);
+  };
+  fill(OW1, H * F, F);
+  fill(OW2, H * H, H);
+  fill(OW3, H, H);
+  int rc = gpsm_mlp_set_params(mm, p.data());
+  if (rc) {
+    gpsm_mlp_free(mm);
+    return rc;
+  }
+  *out = mm;
+  return GPSM_OK;
+}
+
+void gpsm_mlp_free(gpsm_mlp* mm) {
+  if (!mm) return;
+  if (mm->ctx && mm->ctx->planner == mm) mm->ctx->planner = nullptr;
+  hipFree(mm->p);
+  hipFree(mm->g);
+  hipFree(mm->m);
+  hipFree(mm->v);
+  hipFree(mm->w1);
+  hipFree(mm->w2);
+  hipFree(mm->w2t);
+  hipFree(mm->loss);
+  mm->x.release();
+  mm->t.release();
+  mm->y.release();
+  delete mm;
+}
+
+int gpsm_mlp_set_params(gpsm_mlp* mm, const float* in) {
+  if (!mm || !in) return GPSM_EINVAL;
+  hipStream_t s = mm->ctx->stream;
+  if (hipMemcpyAsync(mm->p, in, sizeof(float) * NP, hipMemcpyHostToDevice, s) ||
+      hipMemsetAsync(mm->g, 0, sizeof(float) * NP, s) || hipMemsetAsync(mm->m, 0, sizeof(float) * NP, s) ||
+      hipMemsetAsync(mm->v, 0, sizeof(float) * NP, s))
+    return GPSM_EDEVICE;
+  mm->step = 0;
+  int rc = refresh(mm);
+  if (rc) return rc;
+  return hipStreamSynchronize(s) == hipSuccess ? GPSM_OK : GPSM_EDEVICE;
+}
+
+int gpsm_mlp_get_params(gpsm_mlp* mm, float* out) {
+  if (!mm || !out) return GPSM_EINVAL;
+  if (hipMemcpyAsync(out, mm->p, sizeof(float) * NP, hipMemcpyDeviceToHost, mm->ctx->stream) ||
+      hipStreamSynchronize(mm->ctx->stream))
+    return GPSM_EDEVICE;
+  return GPSM_OK;
+}
+
+int gpsm_mlp_forward(gpsm_mlp* mm, const float* x, uint32_t B, float* y) {
+  if (!mm || (B && (!x || !y))) return GPSM_EINVAL;
+  if (!B) return GPSM_OK;
+  int rc = stage(mm, x, nullptr, B);
+  if (rc) return rc;
+  hipStream_t s = mm->ctx->stream;
+  hipLaunchKernelGGL(mlp_forward_kernel, dim3((B + 31) / 32), dim3(64), 0, s, weights_of(mm),
+                     reinterpret_cast<const float*>(mm->x.ptr), B, reinterpret_cast<float*>(mm->y.ptr));
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(y, mm->y.ptr, sizeof(float) * B, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return GPSM_EDEVICE;
+  return GPSM_OK;
+}
+
+int gpsm_mlp_grad(gpsm_mlp* mm, const float* x, const float* t, uint32_t B, float* loss, float* grad) {
+  if (!mm || !x || !t || !B || !grad) return GPSM_EINVAL;
+  int rc = stage(mm, x, t, B);
+  if (rc) return rc;
+  hipStream_t s = mm->ctx->stream;
+  hipMemsetAsync(mm->g, 0, sizeof(float) * NP, s);
+  if ((rc = run_grad(mm, B))) return rc;
+  float l = 0;
+  if (hipMemcpyAsync(grad, mm->g, sizeof(float) * NP, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(&l, mm->loss, sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return GPSM_EDEVICE;
+  hipMemsetAsync(mm->g, 0, sizeof(float) * NP, s);
+  if (loss) *loss = l;
+  return GPSM_OK;
+}
+
+int gpsm_mlp_train_step(gpsm_mlp* mm, const float* x, const float* t, uint32_t B, float lr, float* loss) {
+  if (!mm || !x || !t || !B) return GPSM_EINVAL;
+  int rc = stage(mm, x, t, B);
+  if (rc) return rc;
+  if ((rc = run_grad(mm, B))) return rc;
+  hipStream_t s = mm->ctx->stream;
+  ++mm->step;
+  const float c1 = 1.0f - std::pow(0.9f, (float)mm->step), c2 = 1.0f - std::pow(0.999f, (float)mm->step);
+  hipLaunchKernelGGL(adam_kernel, dim3((NP + 255) / 256), dim3(256), 0, s, mm->p, mm->g, mm->m, mm->v, lr, c1, c2);
+  if ((rc = refresh(mm))) return rc;
+  float l = 0;
+  if (hipMemcpyAsync(&l, mm->loss, sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return GPSM_EDEVICE;
+  if (loss) *loss = l;
+  return GPSM_OK;
+}
+
+int gpsm_set_planner(gpsm_ctx* c, gpsm_mlp* mm) {
+  if (!c) return GPSM_EINVAL;
+  c->planner = mm;
+  return GPSM_OK;
+}
+
+}  // extern "C"

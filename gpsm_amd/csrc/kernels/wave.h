@@ -107,3 +107,13 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* tmp, u
 }
 
 }  // namespace gpsm
+
+namespace gpsm {
+
+__device__ __forceinline__ float wave_sum_f(float x) {
+#pragma unroll
+  for (int d = WAVE / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, WAVE);
+  return x;
+}
+
+}  // namespace gpsm

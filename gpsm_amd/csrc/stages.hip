@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -206,3 +207,28 @@ int gpsm_compact_bitmap(gpsm_ctx* c, const uint32_t* bits, uint64_t nbits, uint3
 }
 
 }  // extern "C"
+
+extern "C" int gpsm_plan_features(const gpsm_graph* g, const gpsm_query* q, const uint32_t* cand,
+                                  const uint64_t* cet_size, const uint32_t* order, const uint64_t* level_rows,
+                                  float* feat, float* target) {
+  if (!g || !q || !cand || !cet_size || !order || !level_rows || !feat || !target) return GPSM_EINVAL;
+  HostQuery hq;
+  int rc = hq.build(q);
+  if (rc) return rc;
+  std::vector<uint64_t> cs(cet_size, cet_size + hq.ef.size());
+  OrderPlan pl;
+  if (!order_from_vertices(hq, cs, order, pl)) return GPSM_EINVAL;
+  std::vector<uint32_t> cv(cand, cand + hq.n);
+  PlanContext pc{&hq, &g->stats, &cv, &cs};
+  std::vector<uint8_t> placed(hq.n, 0);
+  placed[order[0]] = placed[order[1]] = 1;
+  int rows = 0;
+  for (uint32_t k = 2; k < hq.n; ++k) {
+    step_features(pc, placed, k, pl.par_edge[k], order[k], std::log1p((double)level_rows[k - 1]),
+                  feat + (size_t)rows * GPSM_FEAT);
+    target[rows] = (float)std::log1p((double)level_rows[k]);
+    placed[order[k]] = 1;
+    ++rows;
+  }
+  return rows;
+}

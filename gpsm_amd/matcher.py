@@ -141,7 +141,14 @@ class ResidentGraph:
             _lib.lib().gpsm_result_free(C.byref(r))
         return res
 
-    def match_batch(self, queries, **kw):
+    def match_batch(self, queries, explicit_orders=None, **kw):
+        """explicit_orders: one vertex order per query (with order_mode=ORDER_EXPLICIT)."""
+        if explicit_orders is not None:
+            arr_o = np.zeros((len(queries), _lib.MAX_QV), dtype=np.uint32)
+            for i, od in enumerate(explicit_orders):
+                arr_o[i, :len(od)] = od
+            kw["explicit_order"] = arr_o.reshape(-1)
+            kw.setdefault("order_mode", _lib.ORDER_EXPLICIT)
         o, keep = _opts(**kw)
         arr = (_lib.QueryS * len(queries))()
         for i, q in enumerate(queries):
@@ -150,6 +157,31 @@ class ResidentGraph:
         check(_lib.lib().gpsm_match_batch(self.ctx.h, self.h, len(queries), arr, C.byref(o), res),
               "gpsm_match_batch")
         return [_result(res[i], q.n) for i, q in enumerate(queries)]
+
+    def plan_features(self, q, cand, cet_size, order, level_rows):
+        """Planner training rows of one measured run (gpsm_plan_features)."""
+        qs = q.struct()
+        cand = u32(cand)
+        cs = np.ascontiguousarray(cet_size, dtype=np.uint64)
+        od = u32(order)
+        lr = np.ascontiguousarray(level_rows, dtype=np.uint64)
+        feat = np.zeros((max(q.n - 2, 1), _lib.FEAT), dtype=np.float32)
+        tgt = np.zeros(max(q.n - 2, 1), dtype=np.float32)
+        n = _lib.lib().gpsm_plan_features(self.h, C.byref(qs), ptr(cand), ptr(cs), ptr(od), ptr(lr), ptr(feat),
+                                          ptr(tgt))
+        if n < 0:
+            raise _lib.GpsmError(n, "gpsm_plan_features")
+        return feat[:n], tgt[:n]
+
+    def cet_sizes(self, q):
+        """|CET| of every query edge (gpsm_candidate_edges without the lists)."""
+        qs = q.struct()
+        ef = np.zeros(q.m, dtype=np.uint32)
+        et = np.zeros(q.m, dtype=np.uint32)
+        size = np.zeros(q.m, dtype=np.uint32)
+        check(_lib.lib().gpsm_candidate_edges(self.ctx.h, self.h, C.byref(qs), ptr(ef), ptr(et), ptr(size), None, 0,
+                                              None), "gpsm_candidate_edges")
+        return size.astype(np.uint64)
 
     def filter(self, q):
         W = (self.g.n + 31) // 32

@@ -63,7 +63,9 @@ typedef struct gpsm_query {
 
 typedef struct gpsm_opts {
   int32_t order_mode;             /* GPSM_ORDER_* */
-  const uint32_t* explicit_order; /* EXPLICIT: [n] query ids; every prefix of length >=2 connected */
+  const uint32_t* explicit_order; /* EXPLICIT: query ids, every prefix of length >= 2 connected;
+                                     gpsm_match reads [n], gpsm_match_batch reads query b's order
+                                     at explicit_order + b*GPSM_MAX_QV */
   uint32_t want_tuples;           /* gpsm_match only: materialise the sorted tuple list */
   uint64_t tuple_cap;             /* maximum tuples to materialise (GPSM_EOVERFLOW beyond) */
   uint32_t use_graph;             /* gpsm_match_batch: capture the phases in hipGraphs */
@@ -181,13 +183,17 @@ int gpsm_mlp_train_step(gpsm_mlp* m, const float* x, const float* t, uint32_t B,
 /* Installs (or, with NULL, removes) the network used by GPSM_ORDER_LEARNED. */
 int gpsm_set_planner(gpsm_ctx* ctx, gpsm_mlp* m);
 
-/* Host-side planner features for a query after the filter (no device work):
- * per query edge e the fixed-width feature row the MLP consumes when e is the
- * candidate expand edge of the state given by placed[] (1 = vertex bound). */
-int gpsm_plan_features(uint32_t q_n, const gpsm_query* q, const uint32_t* cand,
-                       const uint32_t* cet_size, const uint32_t* label_count,
-                       uint32_t n_labels_total, uint32_t data_n, uint64_t data_adj,
-                       const uint8_t* placed, float log_rows, float* feat_out);
+/* Training rows for the planner (host only, no device work).  For a vertex
+ * order of q (every prefix connected) and the rows measured at each level
+ * (level_rows[k] = rows with k+1 bound vertices, as gpsm_result reports),
+ * writes for every level k = 2..q->n-1 the GPSM_FEAT features of extending
+ * the prefix order[0..k) by order[k] (log1p of the measured rows of level k-1
+ * among them) into feat[(k-2)*GPSM_FEAT ..] and log1p(level_rows[k]) into
+ * target[k-2].  cand[u] = |C(u)|, cet_size[e] = |CET| of the query edges in
+ * the order gpsm_candidate_edges reports them.  Returns the number of rows. */
+int gpsm_plan_features(const gpsm_graph* g, const gpsm_query* q, const uint32_t* cand,
+                       const uint64_t* cet_size, const uint32_t* order, const uint64_t* level_rows,
+                       float* feat, float* target);
 
 /* --- profiling (no reference counterpart; the reference times whole
  * queries with a millisecond wall clock, util/Util.cpp:21-27) ------------- */
