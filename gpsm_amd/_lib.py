@@ -102,6 +102,9 @@ SIGNATURES = {
                                      C.c_void_p, C.c_void_p, C.c_void_p]),
     "gpsm_tuple_hash": (C.c_uint64, [C.c_void_p, C.c_uint32]),
     "gpsm_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "gpsm_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(QueryS), C.POINTER(Opts),
+                                C.POINTER(C.c_double)]),
+    "gpsm_tuning": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "gpsm_profile_read": (C.c_int, [C.c_void_p, C.POINTER(KStat), C.c_uint32, C.POINTER(C.c_uint32)]),
     # generators and text I/O (include/gpsm_gen.h)
     "gpsm_gen_csr_from_edges": (C.c_int, [C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -118,6 +121,8 @@ SIGNATURES = {
     "gpsm_io_write_graph": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_uint32, C.c_void_p, C.c_uint64,
                                       C.c_void_p, C.c_void_p]),
     "gpsm_io_write_end": (C.c_int, [C.c_char_p]),
+    "gpsm_io_count_graphs": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_uint64)]),
 }
 
 

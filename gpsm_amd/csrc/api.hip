@@ -748,3 +748,70 @@ int gpsm_match_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_q
 }
 
 }  // extern "C"
+
+extern "C" {
+
+int gpsm_tuning(const gpsm_ctx* c, uint32_t* fb, uint32_t* cap, uint32_t* wg) {
+  if (!c) return GPSM_EINVAL;
+  if (fb) *fb = c->tune.filter_block;
+  if (cap) *cap = c->tune.join_cap;
+  if (wg) *wg = c->tune.join_wg_per_cu;
+  return GPSM_OK;
+}
+
+int gpsm_autotune(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* qs, const gpsm_opts* opts,
+                  double* best_ms) {
+  if (!c || !g || !nq || !qs) return GPSM_EINVAL;
+  gpsm_opts o{};
+  if (opts) o = *opts;
+  o.want_tuples = 0;
+  o.block = 0;
+  o.join_cap = 0;
+  std::vector<gpsm_result> ref(nq), out(nq);
+  auto trial = [&](double& ms, bool check) -> int {
+    // best of two runs; the first also warms the buffers up
+    double best = 1e300;
+    for (int rep = 0; rep < 2; ++rep) {
+      auto t0 = std::chrono::steady_clock::now();
+      int rc = run_batch(c, g, nq, qs, &o, check ? out.data() : ref.data(), nullptr);
+      if (rc) return rc;
+      best = std::min(best, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
+    ms = best;
+    if (check)
+      for (uint32_t i = 0; i < nq; ++i)
+        if (out[i].count != ref[i].count || out[i].hash != ref[i].hash) return GPSM_EDEVICE;
+    return GPSM_OK;
+  };
+  double cur;
+  int rc = trial(cur, false);
+  if (rc) return rc;
+  const uint32_t blocks[] = {64, 128, 256}, caps[] = {2048, 4096, 8192}, wgs[] = {2, 4, 8};
+  for (int param = 0; param < 3; ++param) {
+    Tuning best = c->tune;
+    for (int i = 0; i < 3; ++i) {
+      Tuning t = best;
+      if (param == 0) t.filter_block = blocks[i];
+      if (param == 1) t.join_cap = caps[i];
+      if (param == 2) t.join_wg_per_cu = wgs[i];
+      Tuning keep = c->tune;
+      c->tune = t;
+      double ms;
+      rc = trial(ms, true);
+      if (rc) {
+        c->tune = keep;
+        return rc;
+      }
+      if (ms < cur) {
+        cur = ms;
+        best = t;
+      }
+      c->tune = keep;
+    }
+    c->tune = best;
+  }
+  if (best_ms) *best_ms = cur;
+  return GPSM_OK;
+}
+
+}  // extern "C"

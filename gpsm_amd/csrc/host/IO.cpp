@@ -97,3 +97,29 @@ bool IO::output(const unsigned* final_result, unsigned rows, unsigned cols, cons
 void IO::flush() {
   if (ofp) std::fflush(ofp);
 }
+
+#include "gpsm.h"
+#include "gpsm_gen.h"
+
+extern "C" int gpsm_io_count_graphs(const char* path, uint32_t* ngraphs, uint64_t* nverts, uint64_t* nedges) {
+  if (!path || !ngraphs) return GPSM_EINVAL;
+  FILE* fp = std::fopen(path, "r");
+  if (!fp) return GPSM_EINVAL;
+  IO io;
+  uint32_t n = 0;
+  uint64_t nv = 0, ne = 0;
+  for (;;) {
+    Graph* g = io.input(fp);
+    if (!g) break;
+    ++n;
+    nv += g->vertices.size();
+    for (const Vertex& v : g->vertices) ne += v.out.size();
+    delete g;
+  }
+  std::fclose(fp);
+  if (io.error()) return GPSM_EFORMAT;
+  *ngraphs = n;
+  if (nverts) *nverts = nv;
+  if (nedges) *nedges = ne;
+  return GPSM_OK;
+}

@@ -25,6 +25,7 @@ namespace gpsm {
 namespace {
 
 constexpr uint32_t WROWS = 256;  // rows per window = threads per workgroup
+constexpr int EMIT_UNROLL = 4;   // last-level candidates in flight per thread
 
 __device__ __forceinline__ uint32_t rank_of(const uint32_t* __restrict__ bm, const uint32_t* __restrict__ bp,
                                             uint32_t x) {
@@ -175,6 +176,37 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
 
         const uint32_t* lst = a.lists + dn.list_base;
         uint32_t f = 0;
+        if (final && !a.emit) {
+          // last level: nothing is written, so each thread takes EMIT_UNROLL
+          // candidates per step with their list loads in flight together
+          for (; f < T; f += WROWS * EMIT_UNROLL) {
+            uint32_t mvs[EMIT_UNROLL], is[EMIT_UNROLL];
+#pragma unroll
+            for (int u = 0; u < EMIT_UNROLL; ++u) {
+              const uint32_t g = f + u * WROWS + tid;
+              is[u] = WROWS;
+              if (g < T) {
+                const uint32_t i = seg_search(S.pre, nw, g);
+                is[u] = i;
+                mvs[u] = lst[S.start[i] + (g - S.pre[i])];
+              }
+            }
+#pragma unroll
+            for (int u = 0; u < EMIT_UNROLL; ++u) {
+              const uint32_t i = is[u];
+              if (i == WROWS) continue;
+              const uint32_t mv = mvs[u];
+              bool ok = true;
+              for (uint32_t c = 0; c <= k && ok; ++c) ok = val[c * WROWS + i] != mv;
+              for (uint32_t b = bo; b < be && ok; ++b) ok = has_edge(a.row, a.col, mv, val[P.backs[b] * WROWS + i]);
+              if (ok) {
+                ++my_cnt;
+                my_hash += fmix64(S.srow[i] + tuple_term(wn, mv));
+              }
+            }
+          }
+          f = T;
+        }
         for (; f < T; f += WROWS) {
           if (!final && S.fill[nk] + WROWS > cap) break;  // uniform: fill is shared
           const uint32_t g = f + tid;

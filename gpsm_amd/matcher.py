@@ -183,6 +183,19 @@ class ResidentGraph:
                                               None), "gpsm_candidate_edges")
         return size.astype(np.uint64)
 
+    def autotune(self, queries, **kw):
+        """Runs gpsm_autotune on a sample batch; returns (best ms, tuning)."""
+        o, keep = _opts(**kw)
+        arr = (_lib.QueryS * len(queries))()
+        for i, q in enumerate(queries):
+            arr[i] = q.struct()
+        ms = C.c_double(0)
+        check(_lib.lib().gpsm_autotune(self.ctx.h, self.h, len(queries), arr, C.byref(o), C.byref(ms)),
+              "gpsm_autotune")
+        fb, cap, wg = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(_lib.lib().gpsm_tuning(self.ctx.h, C.byref(fb), C.byref(cap), C.byref(wg)), "gpsm_tuning")
+        return ms.value, {"filter_block": fb.value, "join_cap": cap.value, "join_wg_per_cu": wg.value}
+
     def filter(self, q):
         W = (self.g.n + 31) // 32
         bm = np.zeros(q.n * W, dtype=np.uint32)

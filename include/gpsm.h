@@ -125,6 +125,18 @@ int gpsm_match_batch(gpsm_ctx* ctx, const gpsm_graph* g, uint32_t nq,
 
 void gpsm_result_free(gpsm_result* r);
 
+/* Autotuner (no reference counterpart; the reference hard-codes block sizes
+ * 1024/256/32, match/Match.cu:455-1618): times gpsm_match_batch on the given
+ * sample for every candidate filter workgroup size (64/128/256), join level
+ * buffer (2048/4096/8192 rows) and persistent join workgroups per CU
+ * (2/4/8), one parameter at a time, and keeps the fastest in the context.
+ * Results of every trial are checked to agree.  *best_ms receives the time
+ * of the chosen setting.  Later calls with opts->block / join_cap == 0 use it. */
+int gpsm_autotune(gpsm_ctx* ctx, const gpsm_graph* g, uint32_t nq, const gpsm_query* qs,
+                  const gpsm_opts* opts, double* best_ms);
+/* The tuned values: filter workgroup size, join buffer rows, join workgroups per CU. */
+int gpsm_tuning(const gpsm_ctx* ctx, uint32_t* filter_block, uint32_t* join_cap, uint32_t* join_wg_per_cu);
+
 /* --- pipeline stages (parity entry points) ------------------------------ */
 
 /* InitializeCandidateVertices + RefineCandidateVertices

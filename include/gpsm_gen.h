@@ -89,6 +89,11 @@ void gpsm_gen_free_qset(gpsm_qset* s);
 int gpsm_io_write_graph(const char* path, int append, int gid, uint32_t n, const uint32_t* labels,
                         uint64_t m, const uint32_t* src, const uint32_t* dst);
 int gpsm_io_write_end(const char* path);
+/* Reads a whole graph stream with the driver's reader (IO::input,
+ * io/IO.cpp:46-100): *ngraphs receives the number of graphs and *nverts /
+ * *nedges (optional) the totals.  Returns GPSM_EFORMAT on a malformed line or
+ * a missing "t # -1" terminator, GPSM_EINVAL if the file cannot be opened. */
+int gpsm_io_count_graphs(const char* path, uint32_t* ngraphs, uint64_t* nverts, uint64_t* nedges);
 
 #ifdef __cplusplus
 }

@@ -118,3 +118,31 @@ def test_text_writer_format(tmp_path):
     lines = open(p.decode()).read().splitlines()
     assert lines[0] == "t # 0" and lines[1].split()[:2] == ["4", "6"]
     assert lines[2] == "v 0 1" and lines[6] == "e 0 1 0" and lines[-1] == "t # -1"
+
+
+def _count(path):
+    n, v, e = C.c_uint32(0), C.c_uint64(0), C.c_uint64(0)
+    rc = _lib.lib().gpsm_io_count_graphs(str(path).encode(), C.byref(n), C.byref(v), C.byref(e))
+    return rc, n.value, v.value, e.value
+
+
+def test_reader_roundtrip_and_errors(tmp_path):
+    """The text reader reads what the writer writes, and rejects a missing
+    terminator or a malformed line instead of looping (io/IO.cpp:53-97)."""
+    p = tmp_path / "two.g"
+    for gid, g in enumerate([G.complete(5), G.cycle(7)]):
+        src, dst = g.edges()
+        _lib.lib().gpsm_io_write_graph(str(p).encode(), gid, gid, g.n, _lib.ptr(g.labels), len(src),
+                                       _lib.ptr(src), _lib.ptr(dst))
+    _lib.lib().gpsm_io_write_end(str(p).encode())
+    assert _count(p) == (0, 2, 12, 17)
+    text = p.read_text()
+    bad = tmp_path / "unterminated.g"
+    bad.write_text(text.replace("t # -1\n", ""))
+    assert _count(bad)[0] == -8
+    bad2 = tmp_path / "malformed.g"
+    bad2.write_text(text.replace("e 0 1 0", "x 0 1 0", 1))
+    assert _count(bad2)[0] == -8
+    bad3 = tmp_path / "range.g"
+    bad3.write_text(text.replace("e 0 1 0", "e 0 99 0", 1))
+    assert _count(bad3)[0] == -8
