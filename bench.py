@@ -49,7 +49,20 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of 1 GiB (reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--traffic-json", default=None,
+                    help="JSON with per-dispatch HBM KiB of the roofline kernel, measured with rocprofv3")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 4, "copies": 3}))
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (the matcher shards by query across GPUs only "
               "in design); --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -112,8 +125,15 @@ def main():
     avg_s = st["ms"] / 1e3 / st["launches"]
     bytes_per_launch = st["bytes"] / st["launches"]
     achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    traffic = None
+    if args.traffic_json and os.path.exists(args.traffic_json):
+        # HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE runs,
+        # scaled by what the same counters read for a copy of known size
+        t = json.load(open(args.traffic_json))
+        if t.get("kernel") == name:
+            traffic = round(t["bytes_per_dispatch"])
     roofline = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": round(bytes_per_launch),
                 "launches": st["launches"]}
     kernels = {k: {"ms_per_step": round(v["ms"] / args.steps, 3), "launches_per_step": v["launches"] / args.steps,

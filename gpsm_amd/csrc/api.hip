@@ -325,7 +325,8 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   gpsm_opts o{};
   if (opts) o = *opts;
   const uint32_t fblock = o.block ? o.block : c->tune.filter_block;
-  const uint32_t cap = std::max<uint32_t>(o.join_cap ? o.join_cap : c->tune.join_cap, 512);
+  // a level buffer must hold at least two expansion steps (2 x 4 x 256 rows)
+  const uint32_t cap = std::max<uint32_t>(o.join_cap ? o.join_cap : c->tune.join_cap, 2048);
   const uint32_t wg_per_cu = c->tune.join_wg_per_cu;
 
   std::vector<QState> Q(nq);
@@ -510,18 +511,55 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     const size_t E = S.hq.ef.size();
     S.cet_size.resize(E);
     for (size_t e = 0; e < E; ++e) S.cet_size[e] = fdesc[fdesc_of_q[b] + e].total;
+  }
+  const bool learned = o.order_mode == GPSM_ORDER_LEARNED && c->planner;
+  std::vector<uint8_t> learned_ok(nq, 0);
+  if (learned) {
+    // all queries' decisions of one planning round go to the network in one call
+    std::vector<std::vector<uint32_t>> cand(nq);
+    std::vector<PlanContext> pcs(nq);
+    std::vector<LearnedState> ls(nq);
+    std::vector<uint32_t> live;
+    for (uint32_t b = 0; b < nq; ++b) {
+      QState& S = Q[b];
+      if (!S.active) continue;
+      cand[b].resize(S.hq.n);
+      for (uint32_t u = 0; u < S.hq.n; ++u) cand[b][u] = counts[S.vbase + u];
+      pcs[b] = PlanContext{&S.hq, &st, &cand[b], &S.cet_size};
+      learned_start(pcs[b], ls[b]);
+      live.push_back(b);
+    }
+    std::vector<float> feats, y;
+    std::vector<size_t> rows_of(nq);
+    while (!live.empty()) {
+      feats.clear();
+      for (uint32_t b : live) rows_of[b] = learned_emit(ls[b], feats);
+      const uint32_t R = (uint32_t)(feats.size() / GPSM_FEAT);
+      y.assign(R + 1, 0.0f);
+      if (R && gpsm_mlp_forward(c->planner, feats.data(), R, y.data()) != GPSM_OK) break;
+      size_t at = 0;
+      std::vector<uint32_t> next;
+      for (uint32_t b : live) {
+        learned_consume(ls[b], y.data() + at);
+        at += rows_of[b];
+        if (!ls[b].done) next.push_back(b);
+      }
+      live.swap(next);
+    }
+    for (uint32_t b = 0; b < nq; ++b)
+      if (Q[b].active) learned_ok[b] = learned_finish(ls[b], Q[b].plan);
+  }
+  for (uint32_t b = 0; b < nq; ++b) {
+    QState& S = Q[b];
+    if (!S.active) continue;
+    const size_t E = S.hq.ef.size();
     bool ok = true;
     if (o.order_mode == GPSM_ORDER_EXPLICIT) {
       ok = o.explicit_order &&
            order_from_vertices(S.hq, S.cet_size, o.explicit_order + (size_t)b * GPSM_MAX_QV, S.plan);
       if (!ok) S.status = GPSM_EINVAL;
-    } else if (o.order_mode == GPSM_ORDER_LEARNED && c->planner) {
-      std::vector<uint32_t> cand(S.hq.n);
-      for (uint32_t u = 0; u < S.hq.n; ++u) cand[u] = counts[S.vbase + u];
-      PlanContext pc{&S.hq, &st, &cand, &S.cet_size};
-      ok = learned_order(pc, predict_with_ctx, c->planner, S.plan);
-      if (!ok) heuristic_order(S.hq, S.cet_size, st.adj + 1, S.plan);
-      ok = true;
+    } else if (learned) {
+      if (!learned_ok[b]) heuristic_order(S.hq, S.cet_size, st.adj + 1, S.plan);  // fallback
     } else {
       heuristic_order(S.hq, S.cet_size, st.adj + 1, S.plan);
     }
@@ -670,6 +708,10 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     if (prof) {
       std::vector<unsigned long long> fs(5 * (maxsteps + 1), 0);
       HIPCHK(hipMemcpy(fs.data(), fstats, sizeof(unsigned long long) * fs.size(), hipMemcpyDeviceToHost));
+      double join_bytes = 0;
+      for (uint32_t i = 0; i < nplans; ++i) join_bytes += (double)jo[i].bytes;
+      for (auto& pnd : c->prof.pending)
+        if (pnd.kind == K_JOIN) pnd.bytes += join_bytes;
       for (auto& pnd : c->prof.pending) {
         float kms = 0;
         hipEventElapsedTime(&kms, pnd.a, pnd.b);

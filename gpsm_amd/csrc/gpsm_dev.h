@@ -78,6 +78,7 @@ struct JoinOut {
   unsigned long long count;
   unsigned long long hash;
   unsigned long long level_rows[MAXQ];
+  unsigned long long bytes;  // algorithmic bytes the join read and wrote for this query
 };
 
 }  // namespace gpsm

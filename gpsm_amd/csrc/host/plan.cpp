@@ -362,85 +362,121 @@ void first_features(const PlanContext& c, uint32_t edge, uint32_t from, float* f
   step_features(c, placed, 1, edge, to, L1p(c.cand->at(from)), f);
 }
 
-bool learned_order(const PlanContext& c, PredictFn predict, void* user, OrderPlan& out) {
+// The greedy search as a state machine, so a batch of queries can share one
+// network call per decision round: learned_emit appends the feature rows of
+// the pending decision, learned_consume takes their predictions.
+//   round 0: the first edge and its orientation, minimising |CET| (the first
+//            table's rows are known exactly) + min predicted rows of the
+//            level after it;
+//   later:   among unplaced vertices adjacent to the prefix (each through its
+//            smallest-|CET| edge), the one with the smallest predicted rows.
+void learned_start(const PlanContext& c, LearnedState& s) {
+  s.c = &c;
+  s.done = false;
+  s.round = 0;
+  s.order.clear();
+  s.placed.assign(c.q->n, 0);
+}
+
+size_t learned_emit(LearnedState& s, std::vector<float>& feats) {
+  const PlanContext& c = *s.c;
   const HostQuery& q = *c.q;
   const auto& cs = *c.cet_size;
   const size_t E = q.ef.size();
-  // First edge: rows of the first table are known exactly (|CET|); choose the
-  // edge and orientation minimising |CET| + predicted rows of the next level.
-  std::vector<float> feats;
-  std::vector<std::pair<uint32_t, uint32_t>> cands;  // (edge, first vertex)
-  std::vector<std::pair<size_t, size_t>> span;      // per first-choice: rows in feats
-  for (uint32_t e = 0; e < E; ++e)
-    for (int side = 0; side < 2; ++side) {
-      uint32_t a = side ? q.et[e] : q.ef[e], b = side ? q.ef[e] : q.et[e];
-      std::vector<uint8_t> placed(q.n, 0);
-      placed[a] = placed[b] = 1;
-      size_t begin = feats.size() / GPSM_FEAT;
-      double lr = L1p((double)cs[e]);
-      for (uint32_t i = 0; i < E; ++i) {
-        uint32_t x = q.ef[i], y = q.et[i];
-        if (placed[x] == placed[y]) continue;
-        uint32_t child = placed[x] ? y : x;
-        feats.resize(feats.size() + GPSM_FEAT);
-        step_features(c, placed, 2, i, child, lr, feats.data() + feats.size() - GPSM_FEAT);
+  const size_t row0 = feats.size() / GPSM_FEAT;
+  if (s.round == 0) {
+    s.first.clear();
+    s.span.clear();
+    for (uint32_t e = 0; e < E; ++e)
+      for (int side = 0; side < 2; ++side) {
+        const uint32_t a = side ? q.et[e] : q.ef[e], b = side ? q.ef[e] : q.et[e];
+        std::vector<uint8_t> placed(q.n, 0);
+        placed[a] = placed[b] = 1;
+        const size_t begin = feats.size() / GPSM_FEAT - row0;
+        const double lr = std::log1p((double)cs[e]);
+        for (uint32_t i = 0; i < E; ++i) {
+          const uint32_t x = q.ef[i], y = q.et[i];
+          if (placed[x] == placed[y]) continue;
+          feats.resize(feats.size() + GPSM_FEAT);
+          step_features(c, placed, 2, i, placed[x] ? y : x, lr, feats.data() + feats.size() - GPSM_FEAT);
+        }
+        s.first.push_back({e, a});
+        s.span.push_back({begin, feats.size() / GPSM_FEAT - row0});
       }
-      cands.push_back({e, a});
-      span.push_back({begin, feats.size() / GPSM_FEAT});
-    }
-  std::vector<float> y(feats.size() / GPSM_FEAT + 1);
-  if (!feats.empty() && predict(user, feats.data(), (uint32_t)(feats.size() / GPSM_FEAT), y.data()) != 0)
-    return false;
-  double best = 1e300;
-  size_t bi = 0;
-  for (size_t i = 0; i < cands.size(); ++i) {
-    double r1 = (double)cs[cands[i].first];
-    double r2 = 0;
-    if (span[i].second > span[i].first) {
-      r2 = 1e300;
-      for (size_t j = span[i].first; j < span[i].second; ++j) r2 = std::min(r2, std::expm1((double)y[j]));
-    }
-    double cost = r1 + r2;
-    if (cost < best) {
-      best = cost;
-      bi = i;
-    }
-  }
-  std::vector<uint32_t> order;
-  uint32_t e0 = cands[bi].first, a0 = cands[bi].second;
-  order.push_back(a0);
-  order.push_back(q.ef[e0] == a0 ? q.et[e0] : q.ef[e0]);
-  std::vector<uint8_t> placed(q.n, 0);
-  placed[order[0]] = placed[order[1]] = 1;
-  double lr = L1p((double)cs[e0]);
-  while (order.size() < q.n) {
-    // one candidate per unplaced frontier vertex, through its smallest-|CET| edge
-    std::vector<uint32_t> child_of, edge_of;
-    feats.clear();
+  } else {
+    s.child.clear();
     for (uint32_t w = 0; w < q.n; ++w) {
-      if (placed[w]) continue;
+      if (s.placed[w]) continue;
       int be = -1;
       for (uint32_t i = 0; i < E; ++i) {
-        bool hit = (q.ef[i] == w && placed[q.et[i]]) || (q.et[i] == w && placed[q.ef[i]]);
+        const bool hit = (q.ef[i] == w && s.placed[q.et[i]]) || (q.et[i] == w && s.placed[q.ef[i]]);
         if (hit && (be < 0 || cs[i] < cs[be])) be = (int)i;
       }
       if (be < 0) continue;
-      child_of.push_back(w);
-      edge_of.push_back((uint32_t)be);
+      s.child.push_back(w);
       feats.resize(feats.size() + GPSM_FEAT);
-      step_features(c, placed, (uint32_t)order.size(), (uint32_t)be, w, lr, feats.data() + feats.size() - GPSM_FEAT);
+      step_features(c, s.placed, (uint32_t)s.order.size(), (uint32_t)be, w, s.lr,
+                    feats.data() + feats.size() - GPSM_FEAT);
     }
-    if (child_of.empty()) return false;
-    y.resize(child_of.size());
-    if (predict(user, feats.data(), (uint32_t)child_of.size(), y.data()) != 0) return false;
-    size_t j = 0;
-    for (size_t i = 1; i < child_of.size(); ++i)
-      if (y[i] < y[j]) j = i;
-    order.push_back(child_of[j]);
-    placed[child_of[j]] = 1;
-    lr = std::max(0.0f, y[j]);
   }
-  return order_from_vertices(q, cs, order.data(), out);
+  return feats.size() / GPSM_FEAT - row0;
+}
+
+void learned_consume(LearnedState& s, const float* y) {
+  const HostQuery& q = *s.c->q;
+  const auto& cs = *s.c->cet_size;
+  if (s.round == 0) {
+    double best = 1e300;
+    size_t bi = 0;
+    for (size_t i = 0; i < s.first.size(); ++i) {
+      double r2 = 0;
+      if (s.span[i].second > s.span[i].first) {
+        r2 = 1e300;
+        for (size_t j = s.span[i].first; j < s.span[i].second; ++j) r2 = std::min(r2, std::expm1((double)y[j]));
+      }
+      const double cost = (double)cs[s.first[i].first] + r2;
+      if (cost < best) {
+        best = cost;
+        bi = i;
+      }
+    }
+    const uint32_t e0 = s.first[bi].first, a0 = s.first[bi].second;
+    s.order = {a0, q.ef[e0] == a0 ? q.et[e0] : q.ef[e0]};
+    s.placed[s.order[0]] = s.placed[s.order[1]] = 1;
+    s.lr = std::log1p((double)cs[e0]);
+  } else {
+    if (s.child.empty()) {
+      s.done = true;  // disconnected: learned_finish reports failure
+      return;
+    }
+    size_t j = 0;
+    for (size_t i = 1; i < s.child.size(); ++i)
+      if (y[i] < y[j]) j = i;
+    s.order.push_back(s.child[j]);
+    s.placed[s.child[j]] = 1;
+    s.lr = std::max(0.0f, y[j]);
+  }
+  ++s.round;
+  if (s.order.size() >= q.n) s.done = true;
+}
+
+bool learned_finish(const LearnedState& s, OrderPlan& out) {
+  if (s.order.size() != s.c->q->n) return false;
+  return order_from_vertices(*s.c->q, *s.c->cet_size, s.order.data(), out);
+}
+
+bool learned_order(const PlanContext& c, PredictFn predict, void* user, OrderPlan& out) {
+  LearnedState s;
+  learned_start(c, s);
+  std::vector<float> feats, y;
+  while (!s.done) {
+    feats.clear();
+    const size_t R = learned_emit(s, feats);
+    y.assign(R + 1, 0.0f);
+    if (R && predict(user, feats.data(), (uint32_t)R, y.data()) != 0) return false;
+    learned_consume(s, y.data());
+  }
+  return learned_finish(s, out);
 }
 
 }  // namespace gpsm

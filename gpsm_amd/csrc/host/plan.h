@@ -96,4 +96,22 @@ void first_features(const PlanContext& c, uint32_t edge, uint32_t from, float* f
 typedef int (*PredictFn)(void* user, const float* x, uint32_t B, float* y);
 bool learned_order(const PlanContext& c, PredictFn predict, void* user, OrderPlan& out);
 
+// The same search one decision at a time, so that a batch of queries shares
+// one network evaluation per round.
+struct LearnedState {
+  const PlanContext* c = nullptr;
+  bool done = false;
+  int round = 0;
+  std::vector<uint32_t> order;
+  std::vector<uint8_t> placed;
+  double lr = 0;                                          // log1p rows of the current level
+  std::vector<std::pair<uint32_t, uint32_t>> first;       // round 0: (edge, first vertex)
+  std::vector<std::pair<size_t, size_t>> span;            // round 0: feature rows per choice
+  std::vector<uint32_t> child;                            // later rounds: candidate vertices
+};
+void learned_start(const PlanContext& c, LearnedState& s);
+size_t learned_emit(LearnedState& s, std::vector<float>& feats);  // appends rows, returns their number
+void learned_consume(LearnedState& s, const float* y);
+bool learned_finish(const LearnedState& s, OrderPlan& out);
+
 }  // namespace gpsm

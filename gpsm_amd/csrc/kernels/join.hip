@@ -25,7 +25,7 @@ namespace gpsm {
 namespace {
 
 constexpr uint32_t WROWS = 256;  // rows per window = threads per workgroup
-constexpr int EMIT_UNROLL = 4;   // last-level candidates in flight per thread
+constexpr int EXP_UNROLL = 4;    // candidates in flight per thread
 
 __device__ __forceinline__ uint32_t rank_of(const uint32_t* __restrict__ bm, const uint32_t* __restrict__ bp,
                                             uint32_t x) {
@@ -37,7 +37,7 @@ __device__ __forceinline__ uint32_t rank_of(const uint32_t* __restrict__ bm, con
 }
 
 __device__ __forceinline__ bool has_edge(const uint32_t* __restrict__ row, const uint32_t* __restrict__ col,
-                                         uint32_t a, uint32_t b) {
+                                         uint32_t a, uint32_t b, unsigned long long& bytes) {
   uint32_t lo = row[a], hi = row[a + 1];
   const uint32_t lo2 = row[b], hi2 = row[b + 1];
   uint32_t key = b;
@@ -46,9 +46,11 @@ __device__ __forceinline__ bool has_edge(const uint32_t* __restrict__ row, const
     hi = hi2;
     key = a;
   }
+  bytes += 16;
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
     const uint32_t v = col[mid];
+    bytes += 4;
     if (v == key) return true;
     if (v < key) lo = mid + 1;
     else hi = mid;
@@ -87,7 +89,7 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
     const uint32_t* off0 = a.offsets + d0.off_base;
     const uint32_t* cl0 = a.clist + d0.clist_off;
     const uint32_t* ls0 = a.lists + d0.list_base;
-    unsigned long long my_cnt = 0, my_hash = 0;
+    unsigned long long my_cnt = 0, my_hash = 0, my_bytes = 0;
 
     if (tid < MAXQ) {
       S.lrows[tid] = 0;
@@ -159,6 +161,8 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
           unsigned long long s = 0;
           for (uint32_t c = 0; c <= k; ++c) s += tuple_term(P.order[c], val[c * WROWS + tid]);
           S.srow[tid] = s;
+          // row values, rank index (prefix + words) and the offset pair
+          my_bytes += (k == 1 ? 8 : 4 * (k + 1)) + 4 + 4 * (((val[p * WROWS + tid] >> 5) & 7) + 1) + 8;
           const uint32_t j = rank_of(bm_x, bp_x, val[p * WROWS + tid]);
           uint32_t st = offn[j];
           ub = offn[j + 1] - st;
@@ -174,81 +178,71 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
         if (tid == 0 && !final) S.fill[nk] = 0;
         __syncthreads();
 
+        // expand: EXP_UNROLL candidates per thread per step, list loads in
+        // flight together; rows of the next level are appended per wave (one
+        // LDS atomic per wave; row order inside a level does not matter)
         const uint32_t* lst = a.lists + dn.list_base;
+        const uint32_t lane = lane_id();
+        const unsigned long long below = lane ? (~0ULL >> (WAVE - lane)) : 0ULL;
         uint32_t f = 0;
-        if (final && !a.emit) {
-          // last level: nothing is written, so each thread takes EMIT_UNROLL
-          // candidates per step with their list loads in flight together
-          for (; f < T; f += WROWS * EMIT_UNROLL) {
-            uint32_t mvs[EMIT_UNROLL], is[EMIT_UNROLL];
+        for (; f < T; f += WROWS * EXP_UNROLL) {
+          if (!final) {
+            // every thread reads fill before any wave appends in this step
+            const uint32_t fl = S.fill[nk];
+            __syncthreads();
+            if (fl + WROWS * EXP_UNROLL > cap) break;  // uniform
+          }
+          uint32_t mvs[EXP_UNROLL], is[EXP_UNROLL];
 #pragma unroll
-            for (int u = 0; u < EMIT_UNROLL; ++u) {
-              const uint32_t g = f + u * WROWS + tid;
-              is[u] = WROWS;
-              if (g < T) {
-                const uint32_t i = seg_search(S.pre, nw, g);
-                is[u] = i;
-                mvs[u] = lst[S.start[i] + (g - S.pre[i])];
-              }
+          for (int u = 0; u < EXP_UNROLL; ++u) {
+            const uint32_t g = f + u * WROWS + tid;
+            is[u] = WROWS;
+            mvs[u] = 0;
+            if (g < T) {
+              const uint32_t i = seg_search(S.pre, nw, g);
+              is[u] = i;
+              mvs[u] = lst[S.start[i] + (g - S.pre[i])];
             }
+          }
 #pragma unroll
-            for (int u = 0; u < EMIT_UNROLL; ++u) {
-              const uint32_t i = is[u];
-              if (i == WROWS) continue;
-              const uint32_t mv = mvs[u];
-              bool ok = true;
+          for (int u = 0; u < EXP_UNROLL; ++u) {
+            const uint32_t i = is[u], mv = mvs[u];
+            bool ok = i != WROWS;
+            if (ok) {
+              my_bytes += 4;
               for (uint32_t c = 0; c <= k && ok; ++c) ok = val[c * WROWS + i] != mv;
-              for (uint32_t b = bo; b < be && ok; ++b) ok = has_edge(a.row, a.col, mv, val[P.backs[b] * WROWS + i]);
+              for (uint32_t b = bo; b < be && ok; ++b)
+                ok = has_edge(a.row, a.col, mv, val[P.backs[b] * WROWS + i], my_bytes);
+            }
+            if (final) {
               if (ok) {
                 ++my_cnt;
                 my_hash += fmix64(S.srow[i] + tuple_term(wn, mv));
-              }
-            }
-          }
-          f = T;
-        }
-        for (; f < T; f += WROWS) {
-          if (!final && S.fill[nk] + WROWS > cap) break;  // uniform: fill is shared
-          const uint32_t g = f + tid;
-          bool ok = false;
-          uint32_t i = 0, mv = 0;
-          if (g < T) {
-            i = seg_search(S.pre, nw, g);
-            mv = lst[S.start[i] + (g - S.pre[i])];
-            ok = true;
-            for (uint32_t c = 0; c <= k && ok; ++c) ok = val[c * WROWS + i] != mv;
-            for (uint32_t b = bo; b < be && ok; ++b) ok = has_edge(a.row, a.col, mv, val[P.backs[b] * WROWS + i]);
-          }
-          if (final) {
-            if (ok) {
-              ++my_cnt;
-              my_hash += fmix64(S.srow[i] + tuple_term(wn, mv));
-              if (a.emit) {
-                const unsigned long long pos = atomicAdd(a.emit_ctr, 1ULL);
-                if (pos < a.emit_cap) {
-                  uint32_t* t = a.emit + pos * q;
-                  for (uint32_t c = 0; c <= k; ++c) t[P.order[c]] = val[c * WROWS + i];
-                  t[wn] = mv;
+                if (a.emit) {
+                  const unsigned long long pos = atomicAdd(a.emit_ctr, 1ULL);
+                  if (pos < a.emit_cap) {
+                    uint32_t* t = a.emit + pos * q;
+                    for (uint32_t c = 0; c <= k; ++c) t[P.order[c]] = val[c * WROWS + i];
+                    t[wn] = mv;
+                  }
                 }
               }
+            } else {
+              const unsigned long long bal = __ballot(ok);
+              uint32_t wbase = 0;
+              if (lane == 0 && bal) wbase = atomicAdd(&S.fill[nk], (uint32_t)__popcll(bal));
+              wbase = __shfl(wbase, 0, WAVE);
+              if (ok) {
+                const uint64_t r = wbase + __popcll(bal & below);
+                for (uint32_t c = 0; c <= k; ++c) Ln[(uint64_t)c * cap + r] = val[c * WROWS + i];
+                Ln[(uint64_t)nk * cap + r] = mv;
+                my_bytes += 4 * (nk + 1);
+              }
             }
-          } else {
-            uint32_t tot;
-            const uint32_t pos = block_excl_scan(ok ? 1u : 0u, S.tmp, &tot);
-            const uint32_t base = S.fill[nk];
-            if (ok) {
-              const uint64_t r = base + pos;
-              for (uint32_t c = 0; c <= k; ++c) Ln[(uint64_t)c * cap + r] = val[c * WROWS + i];
-              Ln[(uint64_t)nk * cap + r] = mv;
-            }
-            __syncthreads();
-            if (tid == 0) {
-              S.fill[nk] = base + tot;
-              S.lrows[nk] += tot;
-            }
-            __syncthreads();
           }
+          if (!final) __syncthreads();
         }
+        if (!final && tid == 0) S.lrows[nk] += S.fill[nk];
         // advance level k's cursor past what was expanded
         if (tid == 0) {
           if (f >= T) {
@@ -289,6 +283,15 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
       unsigned long long h = 0;
       for (uint32_t w = 0; w < blockDim.x / WAVE; ++w) h += S.red[w];
       atomicAdd(&a.out[task.query].hash, h);
+    }
+    __syncthreads();
+    my_bytes = wave_sum64(my_bytes);
+    if (lane_id() == 0) S.red[tid / WAVE] = my_bytes;
+    __syncthreads();
+    if (tid == 0) {
+      unsigned long long bsum = 0;
+      for (uint32_t w = 0; w < blockDim.x / WAVE; ++w) bsum += S.red[w];
+      atomicAdd(&a.out[task.query].bytes, bsum);
     }
     if (tid >= 1 && tid < q && S.lrows[tid]) atomicAdd(&a.out[task.query].level_rows[tid], S.lrows[tid]);
     __syncthreads();

@@ -1,0 +1,46 @@
+"""The five benchmark workloads, built from seeded generators.
+
+1. ER 10k/40k, 8 uniform labels, one 6-vertex walk query.
+2. Chung-Lu 1M/8M (gamma 2.1), 16 Zipf(1.0) labels, 100 walk queries drawn
+   with 8-16 vertices, half tree-like, half induced.
+3. The config-2 graph with 256 walk queries drawn with 4-32 vertices.
+4. Config 3 matched with the learned join order (same queries).
+5. R-MAT scale 22 (a, b, c, d = .57, .19, .19, .05), 64M undirected edges,
+   32 Zipf(0.8) labels; cycles C4-C8, cliques K4-K8 and walks of up to 50
+   vertices.
+
+Queries of configs 2-5 pass the feasibility rule of gpsm_gen_queries (an
+exact upper bound on the embedding count, `budget`); slots shrink until it
+holds, so realised sizes are reported with every result.
+"""
+from . import _lib
+from . import graphs as G
+
+CONFIG2_BUDGET = 10 ** 8
+
+
+def config1(seed=1):
+    return G.config1(seed)
+
+
+def config2(seed=2, budget=CONFIG2_BUDGET, nq=100):
+    g = G.config2_graph(seed)
+    return g, G.queries(g, nq, 8, 16, _lib.Q_WALK_HALF, seed, budget)
+
+
+def config3(seed=2, budget=CONFIG2_BUDGET, nq=256, g=None):
+    g = g if g is not None else G.config2_graph(seed)
+    return g, G.queries(g, nq, 4, 32, _lib.Q_WALK_HALF, seed + 1000, budget)
+
+
+def config5_graph(seed=5, scale=22, m=64_000_000):
+    g = G.rmat(scale, m, 0.57, 0.19, 0.19, seed)
+    return G.set_labels(g, 32, 0.8, seed)
+
+
+def config5_queries(g, seed=5, budget=CONFIG2_BUDGET, per_kind=8):
+    qs = []
+    qs += G.queries(g, per_kind, 4, 8, _lib.Q_CYCLE, seed, budget)
+    qs += G.queries(g, per_kind, 4, 8, _lib.Q_CLIQUE, seed + 1, budget)
+    qs += G.queries(g, per_kind, 8, 50, _lib.Q_WALK_INDUCED, seed + 2, budget)
+    return qs

@@ -1,0 +1,63 @@
+"""Batched workloads and the learned planner on the GPU.
+
+* A config-3-style mix (walk queries drawn with 4-32 vertices) on a smaller
+  Chung-Lu graph, batched with hipGraph-captured phases, against the
+  independent CPU enumerator query by query.
+* Learned vs heuristic join order: identical counts and hashes, and the
+  trained network's effect on the join work (rows materialised) reported and
+  bounded.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from gpsm_amd import _lib
+from gpsm_amd import graphs as G
+from gpsm_amd import planner
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mix():
+    g = G.chung_lu(100_000, 800_000, 2.1, 77)
+    G.set_labels(g, 16, 1.0, 77)
+    qs = G.queries(g, 64, 4, 32, _lib.Q_WALK_HALF, 78, 2 * 10 ** 6)
+    return g, qs
+
+
+def test_batched_mix_matches_enumerator(ctx, mix):
+    g, qs = mix
+    rg = ctx.upload(g)
+    res = rg.match_batch(qs, use_graph=True)
+    for q, r in zip(qs, res):
+        ref = O.o1_enumerate(g, q)
+        assert (r.count, r.hash) == (ref["count"], ref["hash"]), q.n
+
+
+def test_autotuner_keeps_results(ctx, mix):
+    g, qs = mix
+    rg = ctx.upload(g)
+    before = rg.match_batch(qs)
+    ms, tune = rg.autotune(qs[:32])
+    assert ms > 0 and tune["filter_block"] in (64, 128, 256)
+    after = rg.match_batch(qs)
+    assert [(r.count, r.hash) for r in before] == [(r.count, r.hash) for r in after]
+
+
+@pytest.mark.skipif(not os.path.exists(planner.WEIGHTS), reason="no trained planner weights")
+def test_learned_order_same_results_and_less_join_work(ctx, mix):
+    g, qs = mix
+    rg = ctx.upload(g)
+    heur = rg.match_batch(qs, order_mode=_lib.ORDER_HEURISTIC)
+    planner.install(ctx)
+    learn = rg.match_batch(qs, order_mode=_lib.ORDER_LEARNED)
+    assert [(r.count, r.hash) for r in heur] == [(r.count, r.hash) for r in learn]
+    # join work: rows of every intermediate level (the final level is the
+    # result itself and is equal for both orders)
+    wh = sum(sum(r.level_rows[1:len(r.order) - 1]) for r in heur)
+    wl = sum(sum(r.level_rows[1:len(r.order) - 1]) for r in learn)
+    print("intermediate rows: heuristic %d, learned %d (%.2fx)" % (wh, wl, wh / max(wl, 1)))
+    assert wl <= 1.25 * wh
