@@ -24,8 +24,9 @@ Q_WALK_TREE, Q_WALK_INDUCED, Q_WALK_HALF, Q_CYCLE, Q_CLIQUE = 0, 1, 2, 3, 4
 ERRORS = {
     0: "ok", -1: "invalid argument", -2: "out of memory", -3: "HIP runtime error", -4: "query too large",
     -5: "tuple capacity exceeded", -6: "no HIP device", -7: "query disconnected or without edges",
-    -8: "malformed or unterminated input",
+    -8: "malformed or unterminated input", -9: "join work budget exceeded",
 }
+EBUDGET = -9
 
 
 class GpsmError(RuntimeError):
@@ -52,7 +53,7 @@ class QueryS(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("order_mode", C.c_int32), ("explicit_order", C.c_void_p), ("want_tuples", C.c_uint32),
                 ("tuple_cap", C.c_uint64), ("use_graph", C.c_uint32), ("block", C.c_uint32),
-                ("join_block", C.c_uint32), ("join_cap", C.c_uint32)]
+                ("join_block", C.c_uint32), ("join_cap", C.c_uint32), ("work_budget", C.c_uint64)]
 
 
 class Result(C.Structure):
@@ -98,6 +99,8 @@ SIGNATURES = {
     "gpsm_mlp_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float,
                                       C.POINTER(C.c_float)]),
     "gpsm_set_planner": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "gpsm_mlp_set_input_norm": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gpsm_mlp_get_input_norm": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "gpsm_plan_features": (C.c_int, [C.c_void_p, C.POINTER(QueryS), C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p, C.c_void_p]),
     "gpsm_tuple_hash": (C.c_uint64, [C.c_void_p, C.c_uint32]),

@@ -132,6 +132,7 @@ const char* gpsm_strerror(int code) {
     case GPSM_ENODEV: return "no HIP device";
     case GPSM_EDISCONN: return "query not connected or without edges";
     case GPSM_EFORMAT: return "malformed or unterminated input";
+    case GPSM_EBUDGET: return "join work budget exceeded";
     default: return "unknown error";
   }
 }
@@ -684,6 +685,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       ja.emit_ctr = reinterpret_cast<unsigned long long*>(dptr<uint32_t>(c, B_CTR) + 16);
       ja.emit_cap = emit_cap;
     }
+    ja.budget = o.work_budget;
     pa = pbegin();
     if (!tasks.empty()) launch_join(ja, std::min<uint32_t>(nwg, (uint32_t)tasks.size()), 256, s);
     pend(K_JOIN, pa, 0.0, -1);
@@ -739,6 +741,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       out[b].count = r.count;
       out[b].hash = r.hash;
       for (uint32_t k = 1; k < Q[b].hq.n && k < GPSM_MAX_QV; ++k) out[b].level_rows[k] = r.level_rows[k];
+      if (r.aborted) out[b].status = GPSM_EBUDGET;
     }
     if (emit_cap && nq == 1 && plan_of_q[0] != ~0u) {
       const uint32_t qn = Q[0].hq.n;

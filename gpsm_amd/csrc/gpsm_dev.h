@@ -79,6 +79,8 @@ struct JoinOut {
   unsigned long long hash;
   unsigned long long level_rows[MAXQ];
   unsigned long long bytes;  // algorithmic bytes the join read and wrote for this query
+  unsigned long long work;   // candidate rows examined (checked against the work budget)
+  unsigned long long aborted;
 };
 
 }  // namespace gpsm

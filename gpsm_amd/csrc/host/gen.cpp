@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <vector>
+#include <parallel/algorithm>
 
 #include "gpsm.h"
 #include "gpsm_gen.h"
@@ -383,31 +384,49 @@ int gpsm_gen_chung_lu(uint32_t n, uint64_t m, double gamma, uint64_t seed, gpsm_
 int gpsm_gen_rmat(uint32_t scale, uint64_t m, double a, double b, double c, uint64_t seed,
                   gpsm_csr* out) {
   if (scale < 1 || scale > 31 || a + b + c >= 1.0) return GPSM_EINVAL;
-  uint32_t n = 1u << scale;
-  SplitMix64 rng(seed, 3);
-  EdgeSet es(m);
+  const uint32_t n = 1u << scale;
   const double ab = a + b, abc = a + b + c;
-  uint64_t guard = 0;
-  while (es.order.size() < m) {
-    uint32_t u = 0, v = 0;
-    for (uint32_t l = 0; l < scale; ++l) {
-      double r = rng.uniform();
-      uint32_t bu = 0, bv = 0;
-      if (r < a) {
-      } else if (r < ab) {
-        bv = 1;
-      } else if (r < abc) {
-        bu = 1;
-      } else {
-        bu = bv = 1;
+  // Rounds of candidate edges drawn in fixed chunks, each from its own
+  // stream (seed, round, chunk), then sorted and deduplicated: the edge set
+  // does not depend on the thread count.  If a round overshoots, the m kept
+  // edges are those of smallest fmix64(key ^ seed) (a uniform subset).
+  const uint64_t CH = 1 << 20;
+  std::vector<uint64_t> keys;
+  for (uint32_t round = 0; keys.size() < m; ++round) {
+    if (round > 64) return GPSM_EINVAL;  // too dense to reach m
+    const uint64_t want = (m - keys.size()) + (m - keys.size()) / 5 + 1024;
+    const uint64_t nch = (want + CH - 1) / CH;
+    const size_t base = keys.size();
+    keys.resize(base + nch * CH, ~0ULL);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t ch = 0; ch < (int64_t)nch; ++ch) {
+      SplitMix64 rng(seed, ((uint64_t)(round + 1) << 40) | (uint64_t)ch);
+      uint64_t* dst = keys.data() + base + (uint64_t)ch * CH;
+      for (uint64_t i = 0; i < CH; ++i) {
+        uint32_t u = 0, v = 0;
+        for (uint32_t l = 0; l < scale; ++l) {
+          const double r = rng.uniform();
+          const uint32_t bu = r >= ab, bv = (r >= a && r < ab) || r >= abc;
+          u = (u << 1) | bu;
+          v = (v << 1) | bv;
+        }
+        dst[i] = u == v ? ~0ULL : (((uint64_t)std::min(u, v) << 32) | std::max(u, v));
       }
-      u = (u << 1) | bu;
-      v = (v << 1) | bv;
     }
-    es.insert(u, v);
-    if (++guard > 64 * m + 1000000) return GPSM_EINVAL;  // too dense to reach m
+    __gnu_parallel::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    if (!keys.empty() && keys.back() == ~0ULL) keys.pop_back();
   }
-  return csr_from_keys(n, es.order, out);
+  if (keys.size() > m) {
+    std::vector<std::pair<uint64_t, uint64_t>> pri(keys.size());
+#pragma omp parallel for
+    for (int64_t i = 0; i < (int64_t)keys.size(); ++i) pri[i] = {gpsm::fmix64(keys[i] ^ seed), keys[i]};
+    std::nth_element(pri.begin(), pri.begin() + m, pri.end());
+    keys.resize(m);
+    for (uint64_t i = 0; i < m; ++i) keys[i] = pri[i].second;
+    __gnu_parallel::sort(keys.begin(), keys.end());
+  }
+  return csr_from_keys(n, keys, out);
 }
 
 int gpsm_gen_labels(gpsm_csr* g, uint32_t L, double zipf_s, uint64_t seed) {

@@ -65,7 +65,7 @@ struct JoinShared {
   uint32_t tmp[8];
   uint32_t fill[MAXQ], cur_row[MAXQ], cur_cand[MAXQ];
   unsigned long long lrows[MAXQ];
-  uint32_t task, k;
+  uint32_t task, k, abort;
   unsigned long long red[4];
   // followed by the row window: val[c * WROWS + r], c < qmax
 };
@@ -102,12 +102,17 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
       S.fill[1] = (uint32_t)(task.hi - task.lo);
       S.lrows[1] = task.hi - task.lo;
       S.k = 1;
+      // a query already over its work budget skips its remaining tasks
+      S.abort = a.budget ? (uint32_t)__hip_atomic_load(&a.out[task.query].aborted, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT)
+                         : 0u;
+      if (S.abort) S.fill[1] = 0;
     }
     __syncthreads();
 
     if (q == 2) {
       // the first table is the embedding set
-      for (uint64_t e = task.lo + tid; e < task.hi; e += blockDim.x) {
+      for (uint64_t e = task.lo + tid; e < task.hi && !S.abort; e += blockDim.x) {
         const uint32_t j = seg_search(off0, d0.ncand + 1, (uint32_t)e);
         const uint32_t mu = cl0[j], mv = ls0[e];
         my_hash += fmix64(tuple_term(P.order[0], mu) + tuple_term(P.order[1], mv));
@@ -176,7 +181,16 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
         const uint32_t ex = block_excl_scan(ub, S.tmp, &T);
         S.pre[tid] = tid < nw ? ex : T;
         if (tid == 0 && !final) S.fill[nk] = 0;
+        if (tid == 0 && a.budget) {
+          // work budget: candidate rows examined by all tasks of this query
+          const unsigned long long before = atomicAdd(&a.out[task.query].work, (unsigned long long)T);
+          if (before + T > a.budget) {
+            a.out[task.query].aborted = 1;
+            S.abort = 1;
+          }
+        }
         __syncthreads();
+        if (S.abort) break;  // uniform
 
         // expand: EXP_UNROLL candidates per thread per step, list loads in
         // flight together; rows of the next level are appended per wave (one

@@ -61,6 +61,7 @@ struct Weights {
   const __bf16* w2;   // [H][H]
   const __bf16* w2t;  // [H][H] transposed
   const float* master;
+  const float* norm;  // input standardisation: mu[F], inv_sd[F], applied in FP32 before BF16 rounding
 };
 
 struct TileLds {
@@ -77,7 +78,7 @@ __device__ void forward_tile(const Weights& W, const float* __restrict__ x, uint
   // inputs -> bf16 in LDS (zeros past B)
   for (uint32_t i = lane; i < 32 * F; i += WAVE) {
     const uint32_t s = i / F, j = i % F;
-    const float v = (b0 + s < B) ? x[(size_t)(b0 + s) * F + j] : 0.0f;
+    const float v = (b0 + s < B) ? (x[(size_t)(b0 + s) * F + j] - W.norm[j]) * W.norm[F + j] : 0.0f;
     L.xs[s * XS + j] = to_bf(v);
   }
   wave_sync();
@@ -288,13 +289,14 @@ struct gpsm_mlp {
   __bf16* w2 = nullptr;
   __bf16* w2t = nullptr;
   float* loss = nullptr;
+  float* norm = nullptr;  // mu[F], inv_sd[F]
   DevBuf x, t, y;
   int step = 0;
 };
 
 namespace {
 
-Weights weights_of(const gpsm_mlp* mm) { return Weights{mm->w1, mm->w2, mm->w2t, mm->p}; }
+Weights weights_of(const gpsm_mlp* mm) { return Weights{mm->w1, mm->w2, mm->w2t, mm->p, mm->norm}; }
 
 int refresh(gpsm_mlp* mm) {
   hipLaunchKernelGGL(refresh_kernel, dim3((H * H + 255) / 256), dim3(256), 0, mm->ctx->stream, mm->p, mm->w1, mm->w2,
@@ -338,7 +340,8 @@ int gpsm_mlp_create(gpsm_ctx* c, uint64_t seed, gpsm_mlp** out) {
   if (hipMalloc(&mm->p, sizeof(float) * NP) || hipMalloc(&mm->g, sizeof(float) * NP) ||
       hipMalloc(&mm->m, sizeof(float) * NP) || hipMalloc(&mm->v, sizeof(float) * NP) ||
       hipMalloc(&mm->w1, sizeof(__bf16) * H * F) || hipMalloc(&mm->w2, sizeof(__bf16) * H * H) ||
-      hipMalloc(&mm->w2t, sizeof(__bf16) * H * H) || hipMalloc(&mm->loss, sizeof(float))) {
+      hipMalloc(&mm->w2t, sizeof(__bf16) * H * H) || hipMalloc(&mm->loss, sizeof(float)) ||
+      hipMalloc(&mm->norm, sizeof(float) * 2 * F)) {
     gpsm_mlp_free(mm);
     return GPSM_ENOMEM;
   }
@@ -352,7 +355,8 @@ int gpsm_mlp_create(gpsm_ctx* c, uint64_t seed, gpsm_mlp** out) {
   fill(OW1, H * F, F);
   fill(OW2, H * H, H);
   fill(OW3, H, H);
-  int rc = gpsm_mlp_set_params(mm, p.data());
+  int rc = gpsm_mlp_set_input_norm(mm, nullptr, nullptr);  // identity
+  if (!rc) rc = gpsm_mlp_set_params(mm, p.data());
   if (rc) {
     gpsm_mlp_free(mm);
     return rc;
@@ -372,6 +376,7 @@ void gpsm_mlp_free(gpsm_mlp* mm) {
   hipFree(mm->w2);
   hipFree(mm->w2t);
   hipFree(mm->loss);
+  hipFree(mm->norm);
   mm->x.release();
   mm->t.release();
   mm->y.release();
@@ -446,6 +451,32 @@ int gpsm_mlp_train_step(gpsm_mlp* mm, const float* x, const float* t, uint32_t B
       hipStreamSynchronize(s) != hipSuccess)
     return GPSM_EDEVICE;
   if (loss) *loss = l;
+  return GPSM_OK;
+}
+
+int gpsm_mlp_set_input_norm(gpsm_mlp* mm, const float* mu, const float* inv_sd) {
+  if (!mm) return GPSM_EINVAL;
+  float h[2 * F];
+  for (int j = 0; j < F; ++j) {
+    h[j] = mu ? mu[j] : 0.0f;
+    h[F + j] = inv_sd ? inv_sd[j] : 1.0f;
+  }
+  if (hipMemcpyAsync(mm->norm, h, sizeof(h), hipMemcpyHostToDevice, mm->ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(mm->ctx->stream) != hipSuccess)
+    return GPSM_EDEVICE;
+  return GPSM_OK;
+}
+
+int gpsm_mlp_get_input_norm(gpsm_mlp* mm, float* mu, float* inv_sd) {
+  if (!mm || !mu || !inv_sd) return GPSM_EINVAL;
+  float h[2 * F];
+  if (hipMemcpyAsync(h, mm->norm, sizeof(h), hipMemcpyDeviceToHost, mm->ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(mm->ctx->stream) != hipSuccess)
+    return GPSM_EDEVICE;
+  for (int j = 0; j < F; ++j) {
+    mu[j] = h[j];
+    inv_sd[j] = h[F + j];
+  }
   return GPSM_OK;
 }
 

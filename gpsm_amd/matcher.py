@@ -33,7 +33,7 @@ def _result(r, q):
 
 
 def _opts(order_mode=_lib.ORDER_HEURISTIC, explicit_order=None, want_tuples=False, tuple_cap=0, use_graph=False,
-          block=0, join_cap=0):
+          block=0, join_cap=0, work_budget=0):
     o = _lib.Opts()
     o.order_mode = order_mode
     keep = None
@@ -45,6 +45,7 @@ def _opts(order_mode=_lib.ORDER_HEURISTIC, explicit_order=None, want_tuples=Fals
     o.use_graph = int(use_graph)
     o.block = block
     o.join_cap = join_cap
+    o.work_budget = int(work_budget)
     return o, keep
 
 

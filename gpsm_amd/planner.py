@@ -13,6 +13,7 @@ from . import _lib
 from ._lib import check, ptr
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "planner_weights.npy")
+NORM = os.path.join(os.path.dirname(os.path.abspath(__file__)), "planner_norm.npy")
 
 
 class MLP:
@@ -37,6 +38,17 @@ class MLP:
         assert p.shape == (_lib.N_PARAMS,)
         check(_lib.lib().gpsm_mlp_set_params(self.h, ptr(p)), "gpsm_mlp_set_params")
 
+    def set_norm(self, mu, inv_sd):
+        mu = np.ascontiguousarray(mu, dtype=np.float32)
+        inv_sd = np.ascontiguousarray(inv_sd, dtype=np.float32)
+        check(_lib.lib().gpsm_mlp_set_input_norm(self.h, ptr(mu), ptr(inv_sd)), "gpsm_mlp_set_input_norm")
+
+    def get_norm(self):
+        mu = np.zeros(_lib.FEAT, dtype=np.float32)
+        isd = np.zeros(_lib.FEAT, dtype=np.float32)
+        check(_lib.lib().gpsm_mlp_get_input_norm(self.h, ptr(mu), ptr(isd)), "gpsm_mlp_get_input_norm")
+        return mu, isd
+
     def forward(self, x):
         x = np.ascontiguousarray(x, dtype=np.float32)
         y = np.zeros(x.shape[0], dtype=np.float32)
@@ -60,13 +72,15 @@ class MLP:
         return loss.value
 
 
-def install(ctx, path=WEIGHTS):
+def install(ctx, path=WEIGHTS, norm_path=NORM):
     """Loads trained weights (if present) into a device MLP and makes it the
     context's planner.  Returns the MLP, or None without weights."""
-    if not os.path.exists(path):
+    if not os.path.exists(path) or not os.path.exists(norm_path):
         return None
     m = MLP(ctx)
     m.set(np.load(path, allow_pickle=False))
+    norm = np.load(norm_path, allow_pickle=False)
+    m.set_norm(norm[:_lib.FEAT], norm[_lib.FEAT:])
     check(_lib.lib().gpsm_set_planner(ctx.h, m.h), "gpsm_set_planner")
     ctx.planner = m
     return m

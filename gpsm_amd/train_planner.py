@@ -82,11 +82,13 @@ def train(ctx, X, Y, epochs, lr, log, seed=0):
     perm = rng.permutation(len(X))
     nval = max(len(X) // 10, 1)
     val, tr = perm[:nval], perm[nval:]
-    mu = X[tr].mean(0)
+    mu = X[tr].mean(0).astype(np.float32)
     sd = X[tr].std(0)
     sd[sd < 1e-6] = 1.0
-    Xn = ((X - mu) / sd).astype(np.float32)
+    isd = (1.0 / sd).astype(np.float32)
+    Xn = X.astype(np.float32)  # standardised inside the kernels (FP32, before BF16 rounding)
     m = MLP(ctx, seed=seed)
+    m.set_norm(mu, isd)
     hist = []
     for ep in range(epochs):
         idx = rng.permutation(tr)
@@ -101,13 +103,9 @@ def train(ctx, X, Y, epochs, lr, log, seed=0):
                          "train_mse": float(np.mean((pt - Y[tr[:4096]]) ** 2))})
             log("epoch %d: train mse %.4f  val mse %.4f" % (ep + 1, hist[-1]["train_mse"], hist[-1]["val_mse"]))
     p = m.get().copy()
-    # fold the standardisation into layer 1: W1 (x - mu)/sd + b1
-    W1 = p[OFF_W1:OFF_B1].reshape(128, 32)
-    p[OFF_B1:OFF_B1 + 128] -= (W1 * (mu / sd)[None, :]).sum(1)
-    p[OFF_W1:OFF_B1] = (W1 / sd[None, :]).reshape(-1)
     baseline = float(np.mean((Y[val] - Y[tr].mean()) ** 2))
     m.free()
-    return p, hist, baseline
+    return p, np.concatenate([mu, isd]).astype(np.float32), hist, baseline
 
 
 def main():
@@ -131,14 +129,16 @@ def main():
     X, Y = collect(ctx, args.samples, range(101, 140), args.n, args.m, args.queries_per_graph, int(args.budget),
                    args.orders, log)
     log("collected %d samples" % len(X))
-    p, hist, baseline = train(ctx, X, Y, args.epochs, args.lr, log)
-    # check the folded weights reproduce the trained network on raw features
+    p, norm, hist, baseline = train(ctx, X, Y, args.epochs, args.lr, log)
+    # the saved state reproduces the trained network on raw features
     m = MLP(ctx)
     m.set(p)
+    m.set_norm(norm[:32], norm[32:])
     pred = m.forward(X[:4096])
-    log("folded network mse on raw features: %.4f (constant predictor %.4f)"
+    log("saved network mse on raw features: %.4f (constant predictor %.4f)"
         % (float(np.mean((pred - Y[:4096]) ** 2)), baseline))
     np.save(args.out, p.astype(np.float32), allow_pickle=False)
+    np.save(os.path.join(os.path.dirname(args.out), "planner_norm.npy"), norm, allow_pickle=False)
     meta = {"samples": int(len(X)), "history": hist, "constant_predictor_val_mse": baseline,
             "device": ctx.device_name, "args": vars(args), "seconds": round(time.time() - t0, 1)}
     with open(os.path.splitext(args.out)[0] + ".json", "w") as f:

@@ -19,6 +19,40 @@ from . import graphs as G
 CONFIG2_BUDGET = 10 ** 8
 
 
+def feasible_queries(rg, nq, kmin, kmax, kinds, seed, work_budget, per_slot=6):
+    """Queries that pass an exact device-measured feasibility rule.
+
+    For output slot i, per_slot candidates of kind kinds[i % len(kinds)] are
+    drawn (sizes uniform in [kmin, kmax], no host-side rule), matched in one
+    batch with a join work budget, and the first candidate whose join finishes
+    within the budget is kept.  A slot whose candidates all exceed it keeps its
+    smallest-work candidate's host-rule replacement (gpsm_gen_queries with the
+    same budget as an embedding bound).  Every decision is an exact property
+    of the query, so the set does not depend on the device or thread count.
+    Returns (queries, number of slots that needed the host fallback)."""
+    out, fallbacks = [], 0
+    by_kind = {}
+    for ki, kind in enumerate(kinds):
+        slots = list(range(ki, nq, len(kinds)))
+        cands = G.queries(rg.g, len(slots) * per_slot, kmin, kmax, kind, seed + 7919 * ki, 0)
+        res = rg.match_batch(cands, work_budget=work_budget)
+        for si, slot in enumerate(slots):
+            pick = None
+            for j in range(per_slot):
+                if res[si * per_slot + j].status == 0:
+                    pick = cands[si * per_slot + j]
+                    break
+            by_kind[slot] = pick
+    for slot in range(nq):
+        q = by_kind[slot]
+        if q is None:
+            fallbacks += 1
+            kind = kinds[slot % len(kinds)]
+            q = G.queries(rg.g, 1, kmin, kmax, kind, seed + 104729 * (slot + 1), work_budget)[0]
+        out.append(q)
+    return out, fallbacks
+
+
 def config1(seed=1):
     return G.config1(seed)
 

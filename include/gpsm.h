@@ -41,6 +41,7 @@ extern "C" {
 #define GPSM_ENODEV (-6)      /* no HIP device */
 #define GPSM_EDISCONN (-7)    /* query not connected or without edges */
 #define GPSM_EFORMAT (-8)     /* text reader: malformed or unterminated input */
+#define GPSM_EBUDGET (-9)     /* the join exceeded gpsm_opts.work_budget (result incomplete) */
 
 typedef struct gpsm_ctx gpsm_ctx;
 typedef struct gpsm_graph gpsm_graph;
@@ -72,6 +73,8 @@ typedef struct gpsm_opts {
   uint32_t block;                 /* 0 = autotuned; else workgroup size of the filter/collect kernels */
   uint32_t join_block;            /* 0 = autotuned; else workgroup size of the join kernel */
   uint32_t join_cap;              /* 0 = autotuned; else rows per DFS level buffer of the join */
+  uint64_t work_budget;           /* 0 = none; else a query whose join examines more candidate
+                                     rows than this is abandoned with status GPSM_EBUDGET */
 } gpsm_opts;
 
 typedef struct gpsm_result {
@@ -185,7 +188,12 @@ void gpsm_mlp_free(gpsm_mlp* m);
 uint64_t gpsm_mlp_num_params(void);
 int gpsm_mlp_get_params(gpsm_mlp* m, float* out);
 int gpsm_mlp_set_params(gpsm_mlp* m, const float* in);
-/* Forward: x[B x 32] (host, FP32, rounded to BF16 on the device) -> y[B]. */
+/* Fixed input standardisation applied in FP32 before the BF16 rounding of
+ * the inputs: x' = (x - mu) * inv_sd (NULL restores the identity).  Not
+ * trained; part of the planner's saved state. */
+int gpsm_mlp_set_input_norm(gpsm_mlp* m, const float* mu, const float* inv_sd);
+int gpsm_mlp_get_input_norm(gpsm_mlp* m, float* mu, float* inv_sd);
+/* Forward: x[B x 32] (host, FP32, standardised, rounded to BF16 on the device) -> y[B]. */
 int gpsm_mlp_forward(gpsm_mlp* m, const float* x, uint32_t B, float* y);
 /* Loss 0.5*mean((y-t)^2) and its gradient w.r.t. every parameter (same
  * order as gpsm_mlp_get_params) for one mini-batch. */

@@ -85,3 +85,20 @@ def test_training_fits_a_smooth_target(ctx):
     final = 0.5 * np.mean((y - t) ** 2)
     assert final < 0.05 * first, (first, final)
     m.free()
+
+
+def test_input_standardisation_is_applied_before_rounding(ctx):
+    m = MLP(ctx)
+    p = params(7)
+    m.set(p)
+    rng = np.random.default_rng(7)
+    mu = rng.normal(size=32).astype(np.float32) * 10
+    isd = (1.0 / (0.01 + rng.random(32))).astype(np.float32)
+    m.set_norm(mu, isd)
+    got_mu, got_isd = m.get_norm()
+    assert np.array_equal(got_mu, mu) and np.array_equal(got_isd, isd)
+    x = (mu + rng.normal(size=(64, 32)).astype(np.float32) / isd).astype(np.float32)
+    y = m.forward(x)
+    ref = O.mlp_forward(p, ((x - mu) * isd).astype(np.float32), bf16=True)
+    assert np.allclose(y, ref, rtol=FWD_RTOL, atol=FWD_ATOL)
+    m.free()
