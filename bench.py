@@ -1,8 +1,8 @@
 """Benchmark of the GpSM filter-and-join matcher on one MI355X.
 
-Workload (configs[1] of the benchmark definition): Chung-Lu power-law data
-graph, 1,000,000 vertices and 8,000,000 undirected edges (gamma 2.1), 16 labels
-with Zipf(1.0) frequencies; 100 random-walk queries drawn with 8-16 vertices,
+Workload: Chung-Lu power-law data graph, 1,000,000 vertices and 8,000,000
+undirected edges (gamma 2.1), 16 labels with Zipf(1.0) frequencies; 100
+random-walk queries drawn with 8-16 vertices,
 half tree-like and half with all induced edges, under the feasibility rule of
 gpsm_gen_queries (embedding-count bound 1e8; slots shrink until feasible).
 All inputs are synthetic and seeded.
@@ -74,7 +74,7 @@ def main():
     from gpsm_amd.matcher import Context
 
     t0 = time.time()
-    g = G.config2_graph()
+    g = G.chunglu_1m()
     qs = G.queries(g, args.queries, 8, 16, _lib.Q_WALK_HALF, 2, int(args.budget))
     t_gen = time.time() - t0
     sizes = [q.n for q in qs]

@@ -514,41 +514,30 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     for (size_t e = 0; e < E; ++e) S.cet_size[e] = fdesc[fdesc_of_q[b] + e].total;
   }
   const bool learned = o.order_mode == GPSM_ORDER_LEARNED && c->planner;
-  std::vector<uint8_t> learned_ok(nq, 0);
   if (learned) {
-    // all queries' decisions of one planning round go to the network in one call
+    // every planning round of the whole batch is one network call
     std::vector<std::vector<uint32_t>> cand(nq);
     std::vector<PlanContext> pcs(nq);
-    std::vector<LearnedState> ls(nq);
-    std::vector<uint32_t> live;
+    std::vector<const PlanContext*> live;
+    std::vector<uint32_t> live_b;
     for (uint32_t b = 0; b < nq; ++b) {
       QState& S = Q[b];
       if (!S.active) continue;
       cand[b].resize(S.hq.n);
       for (uint32_t u = 0; u < S.hq.n; ++u) cand[b][u] = counts[S.vbase + u];
       pcs[b] = PlanContext{&S.hq, &st, &cand[b], &S.cet_size};
-      learned_start(pcs[b], ls[b]);
-      live.push_back(b);
+      live.push_back(&pcs[b]);
+      live_b.push_back(b);
     }
-    std::vector<float> feats, y;
-    std::vector<size_t> rows_of(nq);
-    while (!live.empty()) {
-      feats.clear();
-      for (uint32_t b : live) rows_of[b] = learned_emit(ls[b], feats);
-      const uint32_t R = (uint32_t)(feats.size() / GPSM_FEAT);
-      y.assign(R + 1, 0.0f);
-      if (R && gpsm_mlp_forward(c->planner, feats.data(), R, y.data()) != GPSM_OK) break;
-      size_t at = 0;
-      std::vector<uint32_t> next;
-      for (uint32_t b : live) {
-        learned_consume(ls[b], y.data() + at);
-        at += rows_of[b];
-        if (!ls[b].done) next.push_back(b);
-      }
-      live.swap(next);
-    }
-    for (uint32_t b = 0; b < nq; ++b)
-      if (Q[b].active) learned_ok[b] = learned_finish(ls[b], Q[b].plan);
+    std::vector<OrderPlan> chosen_plan;
+    const int prc = learned_plans(
+        live,
+        [](void* u, const float* x, uint32_t B, float* y) {
+          return gpsm_mlp_forward(static_cast<gpsm_mlp*>(u), x, B, y);
+        },
+        c->planner, st.adj + 1, chosen_plan, nullptr);
+    if (prc) return prc;
+    for (size_t i = 0; i < live_b.size(); ++i) Q[live_b[i]].plan = std::move(chosen_plan[i]);
   }
   for (uint32_t b = 0; b < nq; ++b) {
     QState& S = Q[b];
@@ -560,7 +549,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
            order_from_vertices(S.hq, S.cet_size, o.explicit_order + (size_t)b * GPSM_MAX_QV, S.plan);
       if (!ok) S.status = GPSM_EINVAL;
     } else if (learned) {
-      if (!learned_ok[b]) heuristic_order(S.hq, S.cet_size, st.adj + 1, S.plan);  // fallback
+      // planned above
     } else {
       heuristic_order(S.hq, S.cet_size, st.adj + 1, S.plan);
     }

@@ -465,18 +465,173 @@ bool learned_finish(const LearnedState& s, OrderPlan& out) {
   return order_from_vertices(*s.c->q, *s.c->cet_size, s.order.data(), out);
 }
 
-bool learned_order(const PlanContext& c, PredictFn predict, void* user, OrderPlan& out) {
-  LearnedState s;
-  learned_start(c, s);
-  std::vector<float> feats, y;
-  while (!s.done) {
-    feats.clear();
-    const size_t R = learned_emit(s, feats);
-    y.assign(R + 1, 0.0f);
-    if (R && predict(user, feats.data(), (uint32_t)R, y.data()) != 0) return false;
-    learned_consume(s, y.data());
+namespace {
+
+// Greedy continuation of a two-vertex prefix.  Every unplaced vertex adjacent
+// to the prefix is scored by (bound neighbours, smallest |CET| of an edge to
+// the prefix); constrained = true takes the most bound neighbours first.
+void continue_order(const HostQuery& q, const std::vector<uint64_t>& cs, bool constrained, std::vector<uint32_t>& od) {
+  std::vector<uint8_t> placed(q.n, 0);
+  for (uint32_t v : od) placed[v] = 1;
+  const size_t E = q.ef.size();
+  while (od.size() < q.n) {
+    int best = -1;
+    uint32_t bnb = 0;
+    uint64_t bsz = 0;
+    for (uint32_t w = 0; w < q.n; ++w) {
+      if (placed[w]) continue;
+      uint32_t nb = 0;
+      uint64_t sz = ~0ull;
+      for (size_t i = 0; i < E; ++i) {
+        const bool hit = (q.ef[i] == w && placed[q.et[i]]) || (q.et[i] == w && placed[q.ef[i]]);
+        if (!hit) continue;
+        ++nb;
+        sz = std::min(sz, cs[i]);
+      }
+      if (!nb) continue;
+      const bool better = best < 0 || (constrained ? (nb > bnb || (nb == bnb && sz < bsz)) : sz < bsz);
+      if (better) {
+        best = (int)w;
+        bnb = nb;
+        bsz = sz;
+      }
+    }
+    if (best < 0) return;  // disconnected: the caller drops the order
+    od.push_back((uint32_t)best);
+    placed[best] = 1;
   }
-  return learned_finish(s, out);
+}
+
+constexpr uint32_t kFirst = 4;
+
+}  // namespace
+
+void candidate_orders(const HostQuery& q, const std::vector<uint64_t>& cet_size, uint64_t big,
+                      std::vector<std::vector<uint32_t>>& out) {
+  out.clear();
+  OrderPlan h;
+  heuristic_order(q, cet_size, big, h);
+  out.push_back(h.order);
+  const size_t E = q.ef.size();
+  std::vector<uint32_t> edges(E);
+  for (uint32_t i = 0; i < E; ++i) edges[i] = i;
+  std::stable_sort(edges.begin(), edges.end(), [&](uint32_t a, uint32_t b) { return cet_size[a] < cet_size[b]; });
+  for (size_t r = 0; r < edges.size() && r < kFirst; ++r)
+    for (int side = 0; side < 2; ++side)
+      for (int constrained = 0; constrained < 2; ++constrained) {
+        const uint32_t e = edges[r];
+        std::vector<uint32_t> od = {side ? q.et[e] : q.ef[e], side ? q.ef[e] : q.et[e]};
+        continue_order(q, cet_size, constrained != 0, od);
+        if (od.size() != q.n) continue;
+        if (std::find(out.begin(), out.end(), od) == out.end()) out.push_back(std::move(od));
+      }
+}
+
+bool cost_start(const PlanContext& c, const uint32_t* order, PlanCost& s) {
+  s.c = &c;
+  if (!order_from_vertices(*c.q, *c.cet_size, order, s.plan)) return false;
+  s.placed.assign(c.q->n, 0);
+  s.placed[order[0]] = s.placed[order[1]] = 1;
+  s.k = 2;
+  const double first = (double)(*c.cet_size)[s.plan.first_edge];
+  s.lr = std::log1p(first);
+  // level 1 is intermediate whenever the query has a third vertex
+  s.cost = c.q->n > 2 ? first : 0.0;
+  s.done = s.k + 1 >= c.q->n;  // levels 2 .. n-2 are predicted
+  return true;
+}
+
+size_t cost_emit(PlanCost& s, std::vector<float>& feats) {
+  if (s.done) return 0;
+  feats.resize(feats.size() + GPSM_FEAT);
+  step_features(*s.c, s.placed, s.k, s.plan.par_edge[s.k], s.plan.order[s.k], s.lr,
+                feats.data() + feats.size() - GPSM_FEAT);
+  return 1;
+}
+
+void cost_consume(PlanCost& s, const float* y) {
+  if (s.done) return;
+  const double ly = std::max(0.0f, y[0]);
+  s.cost += std::expm1(ly);
+  s.lr = ly;
+  s.placed[s.plan.order[s.k]] = 1;
+  ++s.k;
+  s.done = s.k + 1 >= s.c->q->n;
+}
+
+int learned_plans(const std::vector<const PlanContext*>& qs, PredictFn predict, void* user, uint64_t big,
+                  std::vector<OrderPlan>& out, std::vector<uint32_t>* chosen) {
+  const size_t nq = qs.size();
+  out.assign(nq, OrderPlan());
+  if (chosen) chosen->assign(nq, 0);
+  std::vector<float> feats, y;
+  // 1. greedy learned orders, one decision round at a time for the batch
+  std::vector<LearnedState> ls(nq);
+  std::vector<size_t> live, rows_of(nq);
+  for (size_t b = 0; b < nq; ++b) {
+    learned_start(*qs[b], ls[b]);
+    live.push_back(b);
+  }
+  while (!live.empty()) {
+    feats.clear();
+    for (size_t b : live) rows_of[b] = learned_emit(ls[b], feats);
+    const uint32_t R = (uint32_t)(feats.size() / GPSM_FEAT);
+    y.assign(R + 1, 0.0f);
+    if (R) {
+      const int rc = predict(user, feats.data(), R, y.data());
+      if (rc) return rc;
+    }
+    size_t at = 0;
+    std::vector<size_t> next;
+    for (size_t b : live) {
+      learned_consume(ls[b], y.data() + at);
+      at += rows_of[b];
+      if (!ls[b].done) next.push_back(b);
+    }
+    live.swap(next);
+  }
+  // 2. candidates: heuristic first, the greedy order, the greedy continuations
+  std::vector<std::vector<std::vector<uint32_t>>> cands(nq);
+  std::vector<std::vector<PlanCost>> pc(nq);
+  for (size_t b = 0; b < nq; ++b) {
+    const PlanContext& c = *qs[b];
+    candidate_orders(*c.q, *c.cet_size, big, cands[b]);
+    if (ls[b].order.size() == c.q->n &&
+        std::find(cands[b].begin(), cands[b].end(), ls[b].order) == cands[b].end())
+      cands[b].push_back(ls[b].order);
+    pc[b].resize(cands[b].size());
+    for (size_t i = 0; i < cands[b].size(); ++i)
+      if (!cost_start(c, cands[b][i].data(), pc[b][i])) pc[b][i].done = true, pc[b][i].cost = 1e300;
+  }
+  // 3. chained costs, one level per round for every candidate of the batch
+  for (;;) {
+    feats.clear();
+    for (size_t b = 0; b < nq; ++b)
+      for (auto& s : pc[b]) cost_emit(s, feats);
+    const uint32_t R = (uint32_t)(feats.size() / GPSM_FEAT);
+    if (!R) break;
+    y.assign(R, 0.0f);
+    const int rc = predict(user, feats.data(), R, y.data());
+    if (rc) return rc;
+    size_t at = 0;
+    for (size_t b = 0; b < nq; ++b)
+      for (auto& s : pc[b])
+        if (!s.done) cost_consume(s, y.data() + at++);
+  }
+  // 4. the cheapest candidate, the heuristic unless another is clearly cheaper
+  for (size_t b = 0; b < nq; ++b) {
+    size_t best = 0;
+    for (size_t i = 1; i < pc[b].size(); ++i)
+      if (pc[b][i].cost < pc[b][best].cost) best = i;
+    if (best && !(pc[b][best].cost < kLearnedMargin * pc[b][0].cost)) best = 0;
+    if (pc[b][best].cost >= 1e300) {
+      heuristic_order(*qs[b]->q, *qs[b]->cet_size, big, out[b]);
+      continue;
+    }
+    out[b] = pc[b][best].plan;
+    if (chosen) (*chosen)[b] = (uint32_t)best;
+  }
+  return 0;
 }
 
 }  // namespace gpsm

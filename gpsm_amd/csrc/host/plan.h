@@ -90,14 +90,12 @@ void step_features(const PlanContext& c, const std::vector<uint8_t>& placed, uin
                    uint32_t child, double log_rows, float* f);
 void first_features(const PlanContext& c, uint32_t edge, uint32_t from, float* f);
 
-// Greedy search over predicted log cardinalities: first edge and every next
-// vertex minimise the predicted rows of the next level.  predict(x, B, y)
-// evaluates the network on B feature rows.
+// predict(user, x, B, y) evaluates the network on B feature rows (0 = ok).
 typedef int (*PredictFn)(void* user, const float* x, uint32_t B, float* y);
-bool learned_order(const PlanContext& c, PredictFn predict, void* user, OrderPlan& out);
 
-// The same search one decision at a time, so that a batch of queries shares
-// one network evaluation per round.
+// Greedy search over predicted log cardinalities (first edge and every next
+// vertex minimise the predicted rows of the next level), one decision at a
+// time, so that a batch of queries shares one network evaluation per round.
 struct LearnedState {
   const PlanContext* c = nullptr;
   bool done = false;
@@ -113,5 +111,43 @@ void learned_start(const PlanContext& c, LearnedState& s);
 size_t learned_emit(LearnedState& s, std::vector<float>& feats);  // appends rows, returns their number
 void learned_consume(LearnedState& s, const float* y);
 bool learned_finish(const LearnedState& s, OrderPlan& out);
+
+// Candidate vertex orders for the learned cost model to choose among:
+// the heuristic order first, then, from each of the kFirst smallest-|CET|
+// first edges in both orientations, a min-|CET| continuation and a
+// most-constrained continuation (most bound neighbours, then min |CET|).
+// Duplicates are dropped.
+void candidate_orders(const HostQuery& q, const std::vector<uint64_t>& cet_size, uint64_t big,
+                      std::vector<std::vector<uint32_t>>& out);
+
+// Predicted join cost of one vertex order: the rows of every intermediate
+// level (1 .. n-2), level 1 exactly (|CET| of the first edge), later ones
+// chained through the network (each level's features take the previous
+// level's predicted log rows).  Evaluated one level per round so that all
+// candidates of a batch share one network call per round.
+struct PlanCost {
+  const PlanContext* c = nullptr;
+  OrderPlan plan;
+  std::vector<uint8_t> placed;
+  uint32_t k = 0;
+  double lr = 0;
+  double cost = 0;
+  bool done = false;
+};
+bool cost_start(const PlanContext& c, const uint32_t* order, PlanCost& s);  // false: invalid order
+size_t cost_emit(PlanCost& s, std::vector<float>& feats);                    // 0 or 1 rows
+void cost_consume(PlanCost& s, const float* y);
+
+// A candidate other than the heuristic order is taken only when its
+// predicted cost is below this fraction of the heuristic order's.
+constexpr double kLearnedMargin = 0.9;
+
+// The learned planner for a batch: greedy learned orders, then the cheapest
+// of each query's candidates (candidate_orders plus the greedy order) by
+// chained predicted cost.  out[b] is always a valid plan; chosen[b] (if
+// given) is the index of the candidate taken, 0 = the heuristic order.
+// Returns 0 or the first non-zero code of predict.
+int learned_plans(const std::vector<const PlanContext*>& qs, PredictFn predict, void* user, uint64_t big,
+                  std::vector<OrderPlan>& out, std::vector<uint32_t>* chosen);
 
 }  // namespace gpsm

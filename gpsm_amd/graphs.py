@@ -126,7 +126,7 @@ def queries(g, nq, kmin, kmax, kind, seed, node_budget=0):
 
 # --- the benchmark workloads ---------------------------------------------------
 
-def config1(seed=1):
+def er10k_walk6(seed=1):
     """ER 10,000 vertices / 40,000 edges, 8 uniform labels, one 6-vertex walk query."""
     g = erdos_renyi(10_000, 40_000, seed)
     set_labels(g, 8, 0.0, seed)
@@ -134,7 +134,7 @@ def config1(seed=1):
     return g, q
 
 
-def config2_graph(seed=2, n=1_000_000, m=8_000_000):
+def chunglu_1m(seed=2, n=1_000_000, m=8_000_000):
     """Chung-Lu power law (gamma 2.1), 16 labels Zipf(1.0)."""
     g = chung_lu(n, m, 2.1, seed)
     set_labels(g, 16, 1.0, seed)

@@ -7,6 +7,7 @@ a graph uploaded once.  There is no CPU path: without a HIP device every call
 raises GpsmError.
 """
 import ctypes as C
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -57,6 +58,7 @@ class Context:
         check(_lib.lib().gpsm_init(device, C.byref(h)), "gpsm_init")
         self.h = h
         self.planner = None
+        self.graphs = weakref.WeakSet()  # resident graphs, freed before the context
 
     @property
     def device_name(self):
@@ -64,6 +66,11 @@ class Context:
 
     def close(self):
         if self.h:
+            for g in list(self.graphs):
+                g.free()
+            if self.planner is not None:
+                self.planner.free()
+                self.planner = None
             _lib.lib().gpsm_shutdown(self.h)
             self.h = None
 
@@ -115,11 +122,13 @@ class ResidentGraph:
         check(_lib.lib().gpsm_graph_upload(ctx.h, len(lab), ptr(ro), ptr(col), ptr(lab), C.byref(h)),
               "gpsm_graph_upload")
         self.h = h
+        ctx.graphs.add(self)
 
     def free(self):
-        if self.h:
+        # a graph whose context is already closed was freed with it
+        if self.h and self.ctx.h:
             _lib.lib().gpsm_graph_free(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -174,15 +183,20 @@ class ResidentGraph:
             raise _lib.GpsmError(n, "gpsm_plan_features")
         return feat[:n], tgt[:n]
 
-    def cet_sizes(self, q):
-        """|CET| of every query edge (gpsm_candidate_edges without the lists)."""
+    def cet_edges(self, q):
+        """Query edges (from < to, the planner's edge ids) and |CET| of each
+        (gpsm_candidate_edges without the lists)."""
         qs = q.struct()
-        ef = np.zeros(q.m, dtype=np.uint32)
-        et = np.zeros(q.m, dtype=np.uint32)
-        size = np.zeros(q.m, dtype=np.uint32)
+        ef = np.zeros(max(q.m, 1), dtype=np.uint32)
+        et = np.zeros(max(q.m, 1), dtype=np.uint32)
+        size = np.zeros(max(q.m, 1), dtype=np.uint32)
         check(_lib.lib().gpsm_candidate_edges(self.ctx.h, self.h, C.byref(qs), ptr(ef), ptr(et), ptr(size), None, 0,
                                               None), "gpsm_candidate_edges")
-        return size.astype(np.uint64)
+        return ef[:q.m], et[:q.m], size[:q.m].astype(np.uint64)
+
+    def cet_sizes(self, q):
+        """|CET| of every query edge."""
+        return self.cet_edges(q)[2]
 
     def autotune(self, queries, **kw):
         """Runs gpsm_autotune on a sample batch; returns (best ms, tuning)."""

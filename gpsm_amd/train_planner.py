@@ -1,15 +1,17 @@
 """Trains the learned join-order planner on the GPU.
 
 Samples are cardinalities the matcher itself measures: seeded random-walk
-queries on Chung-Lu graphs with config-2 parameters (gamma 2.1, 16 Zipf(1.0)
-labels, other seeds than the benchmark graph) are matched with their
-heuristic order and with random connected vertex orders; every join level
-of every run gives one (features, log1p(rows)) pair through
-gpsm_plan_features.  The network is trained by the device's own BF16-MFMA
-forward/backward kernels with Adam (gpsm_mlp_train_step), and the feature
-standardisation is folded into the first layer before saving.
+queries (4-24 vertices; half-induced, induced and tree walks) on Chung-Lu
+graphs of 100K, 300K and 1M vertices (gamma 2.1, average degree 16, 16
+Zipf(1.0) labels; other seeds than the benchmark graph) are matched with
+their heuristic order, with the greedy continuations the planner ranks and
+with random connected vertex orders; every join level of every run gives one
+(features, log1p(rows)) pair through gpsm_plan_features.  The network is
+trained by the device's own BF16-MFMA forward/backward kernels with Adam
+(gpsm_mlp_train_step); the FP32 feature standardisation is saved beside the
+weights and applied inside the kernels.
 
-    python -m gpsm_amd.train_planner --samples 20000
+    python -m gpsm_amd.train_planner --samples 60000
 """
 import argparse
 import json
@@ -45,33 +47,78 @@ def random_order(q, rng):
     return order
 
 
-def collect(ctx, target_rows, seeds, n, m, queries_per_graph, budget, orders_per_query, log):
+def continuations(q, ef, et, size, first=4):
+    """The candidate orders the device planner ranks (plan.cpp
+    candidate_orders): from each of the `first` smallest-|CET| edges in both
+    orientations, a min-|CET| and a most-constrained greedy continuation."""
+    out = []
+    for e in sorted(range(len(ef)), key=lambda i: (int(size[i]), i))[:first]:
+        for a, b in ((int(ef[e]), int(et[e])), (int(et[e]), int(ef[e]))):
+            for constrained in (False, True):
+                od, placed = [a, b], {a, b}
+                while len(od) < q.n:
+                    best = None
+                    for w in range(q.n):
+                        if w in placed:
+                            continue
+                        hits = [int(size[i]) for i in range(len(ef))
+                                if (ef[i] == w and et[i] in placed) or (et[i] == w and ef[i] in placed)]
+                        if not hits:
+                            continue
+                        key = (-len(hits), min(hits)) if constrained else (min(hits),)
+                        if best is None or key < best[0]:
+                            best = (key, w)
+                    od.append(best[1])
+                    placed.add(best[1])
+                if od not in out:
+                    out.append(od)
+    return out
+
+
+# graph sizes and query kinds cycled over the training graphs
+SIZES = (100_000, 300_000, 1_000_000)
+KINDS = (_lib.Q_WALK_HALF, _lib.Q_WALK_HALF, _lib.Q_WALK_INDUCED, _lib.Q_WALK_TREE)
+
+
+def collect(ctx, target_rows, seeds, queries_per_graph, budget, orders_per_query, work_budget, log, kmax=24):
     X, Y = [], []
     rng = np.random.default_rng(12345)
-    for seed in seeds:
-        g = G.chung_lu(n, m, 2.1, seed)
+    for gi, seed in enumerate(seeds):
+        n = SIZES[gi % len(SIZES)]
+        g = G.chung_lu(n, 8 * n, 2.1, seed)
         G.set_labels(g, 16, 1.0, seed)
-        qs = G.queries(g, queries_per_graph, 4, 10, _lib.Q_WALK_HALF, seed, budget)
+        kind = KINDS[gi % len(KINDS)]
+        qs = G.queries(g, queries_per_graph, 4, kmax, kind, seed, budget)
         rg = ctx.upload(g)
         base = rg.match_batch(qs)
-        cets = [rg.cet_sizes(q) for q in qs]
-        runs = [(list(r.order), r) for r in base]
-        for rep in range(orders_per_query):
-            ords = [random_order(q, rng) for q in qs]
-            res = rg.match_batch(qs, explicit_orders=ords)
-            for q, r0, r in zip(qs, base, res):
-                assert (r.count, r.hash) == (r0.count, r0.hash)
-            runs += list(zip(ords, res))
-        for i, (od, r) in enumerate(runs):
-            q = qs[i % len(qs)]
-            if r.count == 0 or q.n < 3:
+        tabs = [rg.cet_edges(q) for q in qs]
+        # every (query, order) pair is one entry of a batch
+        pairs = [(i, list(r.order)) for i, r in enumerate(base)]
+        for i, q in enumerate(qs):
+            if q.m == 0:
                 continue
-            f, t = rg.plan_features(q, r.cand, cets[i % len(qs)], od, r.level_rows)
-            X.append(f)
-            Y.append(t)
+            pairs += [(i, od) for od in continuations(q, *tabs[i]) if od != list(base[i].order)]
+            pairs += [(i, random_order(q, rng)) for _ in range(orders_per_query)]
+        used = 0
+        for c0 in range(0, len(pairs), 1024):
+            chunk = pairs[c0:c0 + 1024]
+            res = rg.match_batch([qs[i] for i, _ in chunk], explicit_orders=[od for _, od in chunk],
+                                 work_budget=work_budget)
+            for (i, od), r in zip(chunk, res):
+                q = qs[i]
+                if r.status != 0:
+                    continue  # over the work budget
+                assert (r.count, r.hash) == (base[i].count, base[i].hash)
+                if r.count == 0 or q.n < 3:
+                    continue
+                f, t = rg.plan_features(q, r.cand, tabs[i][2], od, r.level_rows)
+                X.append(f)
+                Y.append(t)
+                used += 1
         rg.free()
         rows = sum(len(y) for y in Y)
-        log("graph seed %d: %d queries, %d rows so far" % (seed, len(qs), rows))
+        log("graph seed %d (n=%d, kind %d): %d queries, %d runs, %d rows so far"
+            % (seed, n, kind, len(qs), used, rows))
         if rows >= target_rows:
             break
     return np.concatenate(X), np.concatenate(Y)
@@ -110,12 +157,11 @@ def train(ctx, X, Y, epochs, lr, log, seed=0):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--samples", type=int, default=20000)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--m", type=int, default=8_000_000)
-    ap.add_argument("--queries-per-graph", type=int, default=400)
-    ap.add_argument("--orders", type=int, default=5)
+    ap.add_argument("--samples", type=int, default=60000)
+    ap.add_argument("--queries-per-graph", type=int, default=300)
+    ap.add_argument("--orders", type=int, default=3)
     ap.add_argument("--budget", type=float, default=3e7)
+    ap.add_argument("--work-budget", type=float, default=4e8)
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default=WEIGHTS)
@@ -126,8 +172,8 @@ def main():
         print("[%6.1fs] %s" % (time.time() - t0, s), flush=True)
 
     ctx = Context(0)
-    X, Y = collect(ctx, args.samples, range(101, 140), args.n, args.m, args.queries_per_graph, int(args.budget),
-                   args.orders, log)
+    X, Y = collect(ctx, args.samples, range(101, 160), args.queries_per_graph, int(args.budget), args.orders,
+                   int(args.work_budget), log)
     log("collected %d samples" % len(X))
     p, norm, hist, baseline = train(ctx, X, Y, args.epochs, args.lr, log)
     # the saved state reproduces the trained network on raw features

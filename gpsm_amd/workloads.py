@@ -1,22 +1,23 @@
-"""The five benchmark workloads, built from seeded generators.
+"""The benchmark workloads, built from seeded generators.
 
-1. ER 10k/40k, 8 uniform labels, one 6-vertex walk query.
-2. Chung-Lu 1M/8M (gamma 2.1), 16 Zipf(1.0) labels, 100 walk queries drawn
-   with 8-16 vertices, half tree-like, half induced.
-3. The config-2 graph with 256 walk queries drawn with 4-32 vertices.
-4. Config 3 matched with the learned join order (same queries).
-5. R-MAT scale 22 (a, b, c, d = .57, .19, .19, .05), 64M undirected edges,
-   32 Zipf(0.8) labels; cycles C4-C8, cliques K4-K8 and walks of up to 50
-   vertices.
+* er10k_walk6: ER 10k vertices / 40k edges, 8 uniform labels, one 6-vertex
+  walk query (the size the CPU enumerator finishes in milliseconds).
+* chunglu1m_walk8_16: Chung-Lu 1M/8M (gamma 2.1), 16 Zipf(1.0) labels, 100
+  walk queries drawn with 8-16 vertices, half tree-like, half induced.
+* chunglu1m_mix4_32: the same graph with 256 walk queries drawn with 4-32
+  vertices (matched with the heuristic or the learned join order).
+* rmat22_64m + rmat_shape_queries: R-MAT scale 22 (a, b, c, d = .57, .19,
+  .19, .05), 64M undirected edges, 32 Zipf(0.8) labels; cycles C4-C8,
+  cliques K4-K8 and induced walks of up to 50 vertices.
 
-Queries of configs 2-5 pass the feasibility rule of gpsm_gen_queries (an
-exact upper bound on the embedding count, `budget`); slots shrink until it
-holds, so realised sizes are reported with every result.
+Queries pass the feasibility rule of gpsm_gen_queries (an exact upper bound
+on the embedding count, `budget`); slots shrink until it holds, so realised
+sizes are reported with every result.
 """
 from . import _lib
 from . import graphs as G
 
-CONFIG2_BUDGET = 10 ** 8
+EMBEDDING_BUDGET = 10 ** 8
 
 
 def feasible_queries(rg, nq, kmin, kmax, kinds, seed, work_budget, per_slot=6):
@@ -53,26 +54,26 @@ def feasible_queries(rg, nq, kmin, kmax, kinds, seed, work_budget, per_slot=6):
     return out, fallbacks
 
 
-def config1(seed=1):
-    return G.config1(seed)
+def er10k_walk6(seed=1):
+    return G.er10k_walk6(seed)
 
 
-def config2(seed=2, budget=CONFIG2_BUDGET, nq=100):
-    g = G.config2_graph(seed)
+def chunglu1m_walk8_16(seed=2, budget=EMBEDDING_BUDGET, nq=100):
+    g = G.chunglu_1m(seed)
     return g, G.queries(g, nq, 8, 16, _lib.Q_WALK_HALF, seed, budget)
 
 
-def config3(seed=2, budget=CONFIG2_BUDGET, nq=256, g=None):
-    g = g if g is not None else G.config2_graph(seed)
+def chunglu1m_mix4_32(seed=2, budget=EMBEDDING_BUDGET, nq=256, g=None):
+    g = g if g is not None else G.chunglu_1m(seed)
     return g, G.queries(g, nq, 4, 32, _lib.Q_WALK_HALF, seed + 1000, budget)
 
 
-def config5_graph(seed=5, scale=22, m=64_000_000):
+def rmat22_64m(seed=5, scale=22, m=64_000_000):
     g = G.rmat(scale, m, 0.57, 0.19, 0.19, seed)
     return G.set_labels(g, 32, 0.8, seed)
 
 
-def config5_queries(g, seed=5, budget=CONFIG2_BUDGET, per_kind=8):
+def rmat_shape_queries(g, seed=5, budget=EMBEDDING_BUDGET, per_kind=8):
     qs = []
     qs += G.queries(g, per_kind, 4, 8, _lib.Q_CYCLE, seed, budget)
     qs += G.queries(g, per_kind, 4, 8, _lib.Q_CLIQUE, seed + 1, budget)

@@ -109,10 +109,10 @@ def test_match_matches_oracle(ctx, case):
         assert r.cand == r2["cand"]
 
 
-def test_config1_full_tuple_list(ctx):
-    """Config 1: ER 10k/40k, 8 labels, 6-vertex walk query: the sorted
+def test_er10k_walk6_full_tuple_list(ctx):
+    """ER 10k/40k, 8 labels, 6-vertex walk query: the sorted
     int32 tuple list is identical to the independent enumerator's."""
-    g, (q,) = G.config1()
+    g, (q,) = G.er10k_walk6()
     r1 = O.o1_enumerate(g, q, tuple_cap=10 ** 6)
     r2 = O.o2_match(g, q, mode=0, tuple_cap=10 ** 6)
     r = ctx.upload(g).match(q, want_tuples=True, tuple_cap=10 ** 6)

@@ -1,7 +1,7 @@
 """Batched workloads and the learned planner on the GPU.
 
-* A config-3-style mix (walk queries drawn with 4-32 vertices) on a smaller
-  Chung-Lu graph, batched with hipGraph-captured phases, against the
+* A mix of walk queries drawn with 4-32 vertices on a 100K-vertex Chung-Lu
+  graph, batched with hipGraph-captured phases, against the
   independent CPU enumerator query by query.
 * Learned vs heuristic join order: identical counts and hashes, and the
   trained network's effect on the join work (rows materialised) reported and
@@ -59,5 +59,9 @@ def test_learned_order_same_results_and_less_join_work(ctx, mix):
     # result itself and is equal for both orders)
     wh = sum(sum(r.level_rows[1:len(r.order) - 1]) for r in heur)
     wl = sum(sum(r.level_rows[1:len(r.order) - 1]) for r in learn)
-    print("intermediate rows: heuristic %d, learned %d (%.2fx)" % (wh, wl, wh / max(wl, 1)))
-    assert wl <= 1.25 * wh
+    changed = sum(h.order != l.order for h, l in zip(heur, learn))
+    print("intermediate rows: heuristic %d, learned %d (%.2fx); %d of %d plans changed"
+          % (wh, wl, wh / max(wl, 1), changed, len(qs)))
+    # the heuristic order is always a candidate and is replaced only when the
+    # network predicts a clearly cheaper plan
+    assert wl <= 1.10 * wh
