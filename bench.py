@@ -2,8 +2,8 @@
 
 Workload: Chung-Lu power-law data graph, 1,000,000 vertices and 8,000,000
 undirected edges (gamma 2.1), 16 labels with Zipf(1.0) frequencies; 100
-random-walk queries drawn with 8-16 vertices,
-half tree-like and half with all induced edges, under the feasibility rule of
+random-walk queries drawn with 8-16 vertices, half tree-like and half with
+all induced edges, under the feasibility rule of
 gpsm_gen_queries (embedding-count bound 1e8; slots shrink until feasible).
 All inputs are synthetic and seeded.
 
@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic__chunglu1M_100q_walk8_16.json")
 
 
 def cpu_model():
@@ -51,8 +52,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 1 GiB (reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--traffic-json", default=None,
-                    help="JSON with per-dispatch HBM KiB of the roofline kernel, measured with rocprofv3")
+    ap.add_argument("--traffic-json", default=TRAFFIC_JSON,
+                    help="per-dispatch HBM bytes of each kernel on this workload, from rocprofv3 FETCH_SIZE and "
+                         "WRITE_SIZE runs scaled by a copy of known size")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -89,13 +91,22 @@ def main():
 
     for _ in range(args.warmup):
         rg.match_batch(qs, order_mode=mode)
-    ctx.profile(True)
+    # algorithmic bytes per launch: one untimed step with the device work
+    # counters (identical inputs give identical counts every step)
+    ctx.profile(2)
+    rg.match_batch(qs, order_mode=mode)
+    counted = ctx.kernel_stats()
+    # timed steps: HIP events around every launch, no counters
+    ctx.profile(1)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         res = rg.match_batch(qs, order_mode=mode)
     t1 = time.perf_counter()
     stats = ctx.kernel_stats()
-    ctx.profile(False)
+    ctx.profile(0)
+    for k, v in stats.items():
+        c = counted.get(k)
+        v["bytes"] = c["bytes"] / c["launches"] * v["launches"] if c and c["launches"] else 0.0
     elapsed = t1 - t0
     ms_per_step = 1e3 * elapsed / args.steps
     emb = sum(r.count for r in res)
@@ -110,6 +121,10 @@ def main():
         dt = (time.perf_counter() - t0) / max(1, args.steps // 2)
         extra["batched_heuristic_qps"] = len(qs) / dt
         assert [(r.count, r.hash) for r in rh] == [(r.count, r.hash) for r in res]
+        # join work (first-table rows + candidates examined) of both orders
+        extra["join_work_learned"] = sum(r.join_work for r in res)
+        extra["join_work_heuristic"] = sum(r.join_work for r in rh)
+        extra["plans_changed"] = sum(a.order != b.order for a, b in zip(res, rh))
     if not args.no_plain:
         rg.match(qs[0])
         t0 = time.perf_counter()
@@ -129,9 +144,9 @@ def main():
     if args.traffic_json and os.path.exists(args.traffic_json):
         # HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE runs,
         # scaled by what the same counters read for a copy of known size
-        t = json.load(open(args.traffic_json))
-        if t.get("kernel") == name:
-            traffic = round(t["bytes_per_dispatch"])
+        t = json.load(open(args.traffic_json)).get("kernels", {}).get(name + "_kernel")
+        if t:
+            traffic = round(t["hbm_bytes_per_dispatch"])
     roofline = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": round(bytes_per_launch),
@@ -144,6 +159,13 @@ def main():
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as O
+        # all cores: O2 across the whole query set (OpenMP, one query per thread)
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        t0 = time.perf_counter()
+        many = O.o2_match_many(g, qs, threads=threads)
+        dt_all = time.perf_counter() - t0
+        assert many == [(r.count, r.hash) for r in res], "CPU oracle disagrees with the device"
+        # one thread: the first queries until the sample length has elapsed
         done, t0 = 0, time.perf_counter()
         for q, r in zip(qs, res):
             ro = O.o2_match(g, q, mode=0)
@@ -152,13 +174,15 @@ def main():
             if time.perf_counter() - t0 > args.cpu_seconds:
                 break
         dt = time.perf_counter() - t0
-        cpu = {"value": round(done / dt, 4), "unit": "queries/s", "cores": 1, "kind": "port",
-               "sample": "O2 host restatement, single thread, first %d of the %d queries (same results checked)"
-                         % (done, len(qs)), "cpu": cpu_model(), "nproc": os.cpu_count()}
+        cpu = {"value": round(len(qs) / dt_all, 4), "unit": "queries/s", "cores": threads, "kind": "port",
+               "sample": "O2 host restatement (g++ -O3) over all %d queries, OpenMP across queries on %d threads, "
+                         "%.1f s; every count and hash equal to the device's" % (len(qs), threads, dt_all),
+               "single_thread": {"value": round(done / dt, 4), "cores": 1,
+                                 "sample": "first %d of the %d queries in %.1f s" % (done, len(qs), dt)},
+               "embeddings_per_sec": round(emb / dt_all), "cpu": cpu_model(), "nproc": os.cpu_count()}
 
     out = {
-        "metric": "queries/sec (exact embedding counts + set hashes), batched %s-order path"
-                  % ("learned" if planner_used else "heuristic"),
+        "metric": "queries/sec + embeddings/sec, learned vs plain join order, same GPU, exact",
         "value": round(qps, 3),
         "unit": "queries/s",
         "n_gpus": 1,

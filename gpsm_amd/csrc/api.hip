@@ -94,7 +94,8 @@ extern "C" {
 
 int gpsm_profile_enable(gpsm_ctx* c, int on) {
   if (!c) return GPSM_EINVAL;
-  c->prof.on = on != 0;
+  if (on < 0 || on > 2) return GPSM_EINVAL;
+  c->prof.mode = on;
   for (auto& a : c->prof.acc) a = Prof::Acc();
   for (auto& p : c->prof.pending) {
     c->prof.pool.push_back(p.a);
@@ -151,7 +152,10 @@ int gpsm_init(int device, gpsm_ctx** out) {
   c->device = device;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
-    std::snprintf(c->name, sizeof(c->name), "%s (%s)", prop.name, prop.gcnArchName);
+    if (prop.name[0])
+      std::snprintf(c->name, sizeof(c->name), "%s (%s)", prop.name, prop.gcnArchName);
+    else
+      std::snprintf(c->name, sizeof(c->name), "%s", prop.gcnArchName);
     c->ncu = prop.multiProcessorCount;
   } else {
     c->ncu = 256;
@@ -228,16 +232,21 @@ int gpsm_graph_upload(gpsm_ctx* c, uint32_t n, const uint32_t* row_offset, const
   d.W = words_of(n);
   d.NB = nblocks_of(n);
   hipSetDevice(c->device);
+  std::vector<uint32_t> fch;
+  filter_chunks(n, row_offset, fch);
+  d.nfchunk = (uint32_t)fch.size() - 1;
   auto fail = [&](int code) {
     hipFree(d.row);
     hipFree(d.col);
     hipFree(d.key);
+    hipFree(d.fchunk);
     delete g;
     return code;
   };
   if (hipMalloc(&d.row, sizeof(uint32_t) * (n + 1)) != hipSuccess ||
       hipMalloc(&d.col, sizeof(uint32_t) * std::max<uint64_t>(A, 1)) != hipSuccess ||
-      hipMalloc(&d.key, sizeof(uint32_t) * n) != hipSuccess) {
+      hipMalloc(&d.key, sizeof(uint32_t) * n) != hipSuccess ||
+      hipMalloc(&d.fchunk, sizeof(uint32_t) * fch.size()) != hipSuccess) {
     (void)hipGetLastError();
     return fail(GPSM_ENOMEM);
   }
@@ -245,11 +254,13 @@ int gpsm_graph_upload(gpsm_ctx* c, uint32_t n, const uint32_t* row_offset, const
           hipSuccess ||
       (A && hipMemcpyAsync(d.col, col, sizeof(uint32_t) * A, hipMemcpyHostToDevice, c->stream) != hipSuccess) ||
       hipMemcpyAsync(d.key, key.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(d.fchunk, fch.data(), sizeof(uint32_t) * fch.size(), hipMemcpyHostToDevice, c->stream) !=
+          hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess) {
     (void)hipGetLastError();
     return fail(GPSM_EDEVICE);
   }
-  g->bytes = sizeof(uint32_t) * ((uint64_t)n + 1 + A + n);
+  g->bytes = sizeof(uint32_t) * ((uint64_t)n + 1 + A + n + fch.size());
   *out = g;
   return GPSM_OK;
 }
@@ -261,6 +272,7 @@ int gpsm_graph_free(gpsm_graph* g) {
   hipFree(g->dg.row);
   hipFree(g->dg.col);
   hipFree(g->dg.key);
+  hipFree(g->dg.fchunk);
   delete g;
   return GPSM_OK;
 }
@@ -350,7 +362,8 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   const uint32_t W = dg.W, NB = dg.NB;
   int rc;
   // per-launch timing (not inside captured graphs)
-  const bool prof = c->prof.on && !o.use_graph;
+  const bool prof = c->prof.mode > 0 && !o.use_graph;
+  const bool counters = prof && c->prof.mode >= 2;
   for (auto& p : c->prof.pending) {
     c->prof.pool.push_back(p.a);
     c->prof.pool.push_back(p.b);
@@ -402,7 +415,8 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       (rc = upload(c, B_NBR_LO, nbr_lo)) || (rc = upload(c, B_NBR_HI, nbr_hi)))
     return rc;
   if ((rc = c->bufs[B_COUNTS].ensure(sizeof(uint32_t) * std::max<uint32_t>(nslots, 1)))) return rc;
-  if ((rc = c->bufs[B_FSTATS].ensure(sizeof(unsigned long long) * 5 * (maxsteps + 1)))) return rc;
+  const size_t fstat_step = (size_t)STAT_COPIES * STAT_STRIDE;
+  if ((rc = c->bufs[B_FSTATS].ensure(sizeof(unsigned long long) * fstat_step * (maxsteps + 1)))) return rc;
   uint32_t* bitmaps = dptr<uint32_t>(c, B_BITMAPS);
   unsigned long long* fstats = dptr<unsigned long long>(c, B_FSTATS);
   {
@@ -411,7 +425,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     hipGraphExec_t exec = nullptr;
     if (o.use_graph && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) captured = true;
     launch_fill_u32(bitmaps, 0u, (uint64_t)nslots * W, s);
-    if (prof) launch_fill_u32(reinterpret_cast<uint32_t*>(fstats), 0u, 10ull * (maxsteps + 1), s);
+    if (counters) launch_fill_u32(reinterpret_cast<uint32_t*>(fstats), 0u, 2ull * fstat_step * (maxsteps + 1), s);
     {
       hipEvent_t a = pbegin();
       launch_init(dg, dptr<InitOp>(c, B_INIT), (uint32_t)init.size(), bitmaps, s);
@@ -422,8 +436,8 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       hipEvent_t a = pbegin();
       launch_filter_step(dg, dptr<FilterOp>(c, B_FOPS) + step_off[sidx], nops_s, dptr<uint32_t>(c, B_NBR_SLOT),
                          dptr<uint32_t>(c, B_NBR_LO), dptr<uint32_t>(c, B_NBR_HI), bitmaps, fblock,
-                         prof ? fstats + 5 * sidx : nullptr, s);
-      pend(K_FILTER, a, 4.0 * W * nops_s, (int)sidx);
+                         counters ? fstats + fstat_step * sidx : nullptr, s);
+      pend(K_FILTER, a, 4.0 * W * nops_s, counters ? (int)sidx : -1);
     }
     launch_row_popcount(dg, bitmaps, nslots, dptr<uint32_t>(c, B_COUNTS), s);
     if (captured) {
@@ -600,8 +614,8 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     plan_of_q[b] = (uint32_t)plans.size();
     plans.push_back(jp);
   }
-  // tasks: slices of the first tables, large enough to amortise a task's
-  // setup, small enough to balance the persistent grid
+  // tasks: interleaved row sets of the first tables, large enough to
+  // amortise a task's setup, small enough to balance the persistent grid
   const uint32_t nwg = std::max<uint32_t>(1, c->ncu * wg_per_cu);
   uint64_t rows_total = 0;
   for (uint32_t b = 0; b < nq; ++b)
@@ -611,9 +625,9 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   for (uint32_t b = 0; b < nq; ++b) {
     if (plan_of_q[b] == ~0u) continue;
     const uint64_t R1 = Q[b].cet_size[Q[b].plan.first_edge];
-    for (uint64_t lo = 0; lo < R1; lo += tsize) tasks.push_back({plan_of_q[b], 0, lo, std::min(R1, lo + tsize)});
+    const uint64_t T = (R1 + tsize - 1) / tsize;
+    for (uint64_t t = 0; t < T; ++t) tasks.push_back({plan_of_q[b], (uint32_t)T, t, (R1 - t + T - 1) / T});
   }
-  // largest queries' tasks first is not needed: tasks are uniform in rows
 
   // ---------------- phase C: candidate-edge tables ----------------
   if ((rc = upload(c, B_JDESC, jdesc)) || (rc = upload(c, B_PLANS, plans)) || (rc = upload(c, B_TASKS, tasks)))
@@ -697,8 +711,9 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     float ms = 0;
     hipEventElapsedTime(&ms, c->ev0, c->ev1);
     if (prof) {
-      std::vector<unsigned long long> fs(5 * (maxsteps + 1), 0);
-      HIPCHK(hipMemcpy(fs.data(), fstats, sizeof(unsigned long long) * fs.size(), hipMemcpyDeviceToHost));
+      std::vector<unsigned long long> fs(fstat_step * (maxsteps + 1), 0);
+      if (counters)
+        HIPCHK(hipMemcpy(fs.data(), fstats, sizeof(unsigned long long) * fs.size(), hipMemcpyDeviceToHost));
       double join_bytes = 0;
       for (uint32_t i = 0; i < nplans; ++i) join_bytes += (double)jo[i].bytes;
       for (auto& pnd : c->prof.pending)
@@ -708,7 +723,9 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
         hipEventElapsedTime(&kms, pnd.a, pnd.b);
         double bytes = pnd.bytes;
         if (pnd.stats_slot >= 0) {
-          const unsigned long long* f = &fs[5 * pnd.stats_slot];
+          unsigned long long f[5] = {0, 0, 0, 0, 0};
+          for (uint32_t cp = 0; cp < STAT_COPIES; ++cp)
+            for (int i = 0; i < 5; ++i) f[i] += fs[fstat_step * pnd.stats_slot + cp * STAT_STRIDE + i];
           // candidates: two row offsets; entries: column index; key gathers;
           // bitmap probes and bit writes count one bit each
           bytes += 8.0 * f[0] + 4.0 * f[1] + 4.0 * f[2] + (f[3] + f[4]) / 8.0;
@@ -730,6 +747,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       out[b].count = r.count;
       out[b].hash = r.hash;
       for (uint32_t k = 1; k < Q[b].hq.n && k < GPSM_MAX_QV; ++k) out[b].level_rows[k] = r.level_rows[k];
+      out[b].join_work = r.work;
       if (r.aborted) out[b].status = GPSM_EBUDGET;
     }
     if (emit_cap && nq == 1 && plan_of_q[0] != ~0u) {

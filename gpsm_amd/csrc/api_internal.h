@@ -77,7 +77,7 @@ namespace gpsm {
 
 // Per-launch timing on the context stream while profiling is on.
 struct Prof {
-  bool on = false;
+  int mode = 0;  // 0 off, 1 HIP-event timing, 2 timing + the filter's byte counters
   struct Pending {
     int kind;
     hipEvent_t a, b;

@@ -67,10 +67,14 @@ struct JoinPlan {
 };
 
 // A slice of the first table of one query: rows [lo, hi).
+// Rows first, first + stride, ..., (count of them) of one query's first
+// table.  A query's tasks interleave its rows, so the rows of hub
+// candidates (the table's first rows on a power-law graph) are spread over
+// all of its tasks instead of making the first task the longest.
 struct JoinTask {
   uint32_t query;
-  uint32_t pad;
-  uint64_t lo, hi;
+  uint32_t stride;
+  uint64_t first, count;
 };
 
 // Per-query accumulators written by the join.

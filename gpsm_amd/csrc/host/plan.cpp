@@ -527,17 +527,38 @@ void candidate_orders(const HostQuery& q, const std::vector<uint64_t>& cet_size,
       }
 }
 
+namespace {
+
+// Candidates examined per row of level k-1 when position k is bound.
+double examine_factor(const PlanCost& s, uint32_t k) {
+  const HostQuery& q = *s.c->q;
+  const uint32_t e = s.plan.par_edge[k], w = s.plan.order[k];
+  const uint32_t parent = q.ef[e] == w ? q.et[e] : q.ef[e];
+  const double fan = (double)(*s.c->cet_size)[e] / (double)std::max<uint32_t>((*s.c->cand)[parent], 1);
+  return fan * (1.0 + (double)s.plan.backs[k].size());
+}
+
+// The final level needs no prediction: add its examined candidates and stop.
+void settle(PlanCost& s) {
+  if (!s.done && s.k + 1 >= s.c->q->n) {
+    s.cost += s.rows * examine_factor(s, s.k);
+    s.done = true;
+  }
+}
+
+}  // namespace
+
 bool cost_start(const PlanContext& c, const uint32_t* order, PlanCost& s) {
   s.c = &c;
   if (!order_from_vertices(*c.q, *c.cet_size, order, s.plan)) return false;
   s.placed.assign(c.q->n, 0);
   s.placed[order[0]] = s.placed[order[1]] = 1;
   s.k = 2;
-  const double first = (double)(*c.cet_size)[s.plan.first_edge];
-  s.lr = std::log1p(first);
-  // level 1 is intermediate whenever the query has a third vertex
-  s.cost = c.q->n > 2 ? first : 0.0;
-  s.done = s.k + 1 >= c.q->n;  // levels 2 .. n-2 are predicted
+  s.rows = (double)(*c.cet_size)[s.plan.first_edge];
+  s.lr = std::log1p(s.rows);
+  s.cost = s.rows;
+  s.done = c.q->n <= 2;
+  settle(s);
   return true;
 }
 
@@ -551,12 +572,13 @@ size_t cost_emit(PlanCost& s, std::vector<float>& feats) {
 
 void cost_consume(PlanCost& s, const float* y) {
   if (s.done) return;
+  s.cost += s.rows * examine_factor(s, s.k);
   const double ly = std::max(0.0f, y[0]);
-  s.cost += std::expm1(ly);
+  s.rows = std::expm1(ly);
   s.lr = ly;
   s.placed[s.plan.order[s.k]] = 1;
   ++s.k;
-  s.done = s.k + 1 >= s.c->q->n;
+  settle(s);
 }
 
 int learned_plans(const std::vector<const PlanContext*>& qs, PredictFn predict, void* user, uint64_t big,

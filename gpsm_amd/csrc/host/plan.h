@@ -120,17 +120,23 @@ bool learned_finish(const LearnedState& s, OrderPlan& out);
 void candidate_orders(const HostQuery& q, const std::vector<uint64_t>& cet_size, uint64_t big,
                       std::vector<std::vector<uint32_t>>& out);
 
-// Predicted join cost of one vertex order: the rows of every intermediate
-// level (1 .. n-2), level 1 exactly (|CET| of the first edge), later ones
-// chained through the network (each level's features take the previous
-// level's predicted log rows).  Evaluated one level per round so that all
-// candidates of a batch share one network call per round.
+// Predicted join cost of one vertex order: the rows of the first table plus
+// the candidates every later level examines,
+//   cost = R_1 + sum_{k=2}^{n-1} R_{k-1} * |CET(e_k)| / |C(parent_k)| * (1 + b_k),
+// with e_k the expand edge of position k, b_k its join-mode (back) edges
+// (each an edge-membership check per candidate) and R_j the rows of level j:
+// R_1 = |CET| of the first edge exactly, R_2 .. R_{n-2} chained through the
+// network (each level's features take the previous level's predicted log
+// rows).  The final level's examined candidates are counted; its rows, the
+// embedding count, are the same for every order.  Evaluated one level per
+// round so that all candidates of a batch share one network call per round.
 struct PlanCost {
   const PlanContext* c = nullptr;
   OrderPlan plan;
   std::vector<uint8_t> placed;
   uint32_t k = 0;
-  double lr = 0;
+  double lr = 0;    // log1p R_{k-1}
+  double rows = 0;  // R_{k-1}
   double cost = 0;
   bool done = false;
 };

@@ -3,11 +3,15 @@
 // mu in C(x) ascending, the data neighbours of mu that are in C(w), in
 // adjacency (ascending) order.
 //
-// One wave per 64 candidates of C(x); the wave walks the concatenated
-// adjacency of its candidates 64 entries per step (balanced across hub and
-// leaf candidates) and tests C(w) membership in the bitmap.  The write pass
-// ranks each hit inside its candidate's run with a ballot, so lists stay in
-// adjacency order without a second scan.
+// One wave per 64 candidates of C(x), taken with a stride: of the N waves
+// of a table, wave j takes candidates j, j + N, j + 2N, ..., so the hubs of
+// a power-law graph (its lowest ids, the first candidates) land on
+// different waves.  The wave walks the concatenated adjacency of its
+// candidates 64 entries per step (balanced across hub and leaf candidates)
+// and tests C(w) membership in the bitmap.  Outputs are per candidate (its
+// count, its list at its offset), so the assignment does not change them.
+// The write pass ranks each hit inside its candidate's run with a ballot,
+// so lists stay in adjacency order without a second scan.
 #include "launch.h"
 #include "wave.h"
 
@@ -21,11 +25,16 @@ struct CetLds {
   uint32_t cnt[WAVE];
 };
 
+// Lane l of chunk j of a table with nch chunks holds candidate j + l * nch.
+__device__ __forceinline__ uint32_t cand_index(uint32_t j, uint32_t nch, uint32_t lane) { return j + lane * nch; }
+
 __device__ __forceinline__ void load_chunk(const uint32_t* __restrict__ row, const uint32_t* __restrict__ clist,
-                                           const CetDesc& d, uint32_t c0, CetLds& L, bool& mine, uint32_t& D) {
+                                           const CetDesc& d, uint32_t j, uint32_t nch, CetLds& L, bool& mine,
+                                           uint32_t& D) {
   const uint32_t lane = lane_id();
-  mine = c0 + lane < d.ncand;
-  const uint32_t id = mine ? clist[d.clist_off + c0 + lane] : 0u;
+  const uint32_t ci = cand_index(j, nch, lane);
+  mine = ci < d.ncand;
+  const uint32_t id = mine ? clist[d.clist_off + ci] : 0u;
   const uint32_t off = mine ? row[id] : 0u;
   const uint32_t deg = mine ? row[id + 1] - off : 0u;
   const uint32_t incl = wave_incl_scan(deg);
@@ -51,10 +60,10 @@ __global__ void __launch_bounds__(256) cet_count_kernel(const uint32_t* __restri
   for (uint32_t it = wave; it < nitems; it += nwaves) {
     const uint32_t di = uniform(seg_search(chunk_base, ndesc + 1, it));
     const CetDesc d = descs[di];
-    const uint32_t c0 = (it - chunk_base[di]) * WAVE;
+    const uint32_t j = it - chunk_base[di], nch = chunk_base[di + 1] - chunk_base[di];
     bool mine;
     uint32_t D;
-    load_chunk(row, clist, d, c0, L, mine, D);
+    load_chunk(row, clist, d, j, nch, L, mine, D);
     const uint32_t* bm_w = bitmaps + (size_t)d.w_slot * W;
     uint32_t tot = 0;
     for (uint32_t f = 0; f < D; f += WAVE) {
@@ -69,7 +78,7 @@ __global__ void __launch_bounds__(256) cet_count_kernel(const uint32_t* __restri
       }
     }
     wave_sync();
-    if (per_cand && mine) offsets[d.off_base + c0 + lane] = L.cnt[lane];
+    if (per_cand && mine) offsets[d.off_base + cand_index(j, nch, lane)] = L.cnt[lane];
     tot = wave_sum(tot);
     if (lane == 0 && tot) atomicAdd(reinterpret_cast<unsigned long long*>(&descs[di].total), (unsigned long long)tot);
     wave_sync();
@@ -110,12 +119,14 @@ __global__ void __launch_bounds__(256) cet_write_kernel(const uint32_t* __restri
   for (uint32_t it = wave; it < nitems; it += nwaves) {
     const uint32_t di = uniform(seg_search(chunk_base, ndesc + 1, it));
     const CetDesc d = descs[di];
-    const uint32_t c0 = (it - chunk_base[di]) * WAVE;
+    const uint32_t j = it - chunk_base[di], nch = chunk_base[di + 1] - chunk_base[di];
     bool mine;
     uint32_t D;
-    load_chunk(row, clist, d, c0, L, mine, D);
+    load_chunk(row, clist, d, j, nch, L, mine, D);
+    // each lane's list starts at its candidate's offset; cnt advances it
+    L.cnt[lane_id()] = mine ? offsets[d.off_base + cand_index(j, nch, lane_id())] : 0u;
+    wave_sync();
     const uint32_t* bm_w = bitmaps + (size_t)d.w_slot * W;
-    const uint32_t* o = offsets + d.off_base + c0;
     uint32_t* out = lists + d.list_base;
     for (uint32_t f = 0; f < D; f += WAVE) {
       const uint32_t g = f + lane;
@@ -132,7 +143,7 @@ __global__ void __launch_bounds__(256) cet_write_kernel(const uint32_t* __restri
         const uint32_t seg_lo = L.pre[ci] > f ? L.pre[ci] - f : 0u;
         const unsigned long long below = (lane ? (~0ULL >> (WAVE - lane)) : 0ULL) & (~0ULL << seg_lo);
         const uint32_t rank = __popcll(hm & below);
-        const uint32_t pos = o[ci] + L.cnt[ci] + rank;
+        const uint32_t pos = L.cnt[ci] + rank;
         if (pos < d.list_cap) out[pos] = mv;
       }
       wave_sync();

@@ -5,13 +5,15 @@
 //
 // Layout: candidate sets are bitmaps (1 bit per data vertex, 32-bit words);
 // the reference used one byte per (query vertex, data vertex).
-// Parallel decomposition: one wave per 64-word bitmap chunk (2048 data
-// vertices) of one op.  The wave compacts the chunk's candidates into LDS
-// (the reference's kernel_collect, fused), then walks the concatenated
-// adjacency lists of 64 candidates at a time, 64 adjacency entries per step
-// (lane t takes entry t; a binary search in the 64 degree prefix sums finds its
-// candidate), so hub and leaf candidates share lanes evenly instead of one
-// 32-lane warp per candidate.
+// Parallel decomposition: one wave per (op, vertex range).  The ranges are
+// fixed per data graph (DevGraph::fchunk): at most 1024 vertices and 4096
+// adjacency entries each, so a power-law graph's hubs (which a Chung-Lu or
+// R-MAT generator gives the lowest ids) do not pile onto one wave.  The wave
+// compacts the range's candidates into LDS (the reference's kernel_collect,
+// fused), then walks the concatenated adjacency lists of 64 candidates at a
+// time, 64 adjacency entries per lane step (lane t takes entry t; a binary
+// search in the 64 degree prefix sums finds its candidate), so hub and leaf
+// candidates share lanes evenly instead of one 32-lane warp per candidate.
 #include "launch.h"
 #include "wave.h"
 
@@ -19,8 +21,8 @@ namespace gpsm {
 
 namespace {
 
-constexpr int CHUNK_WORDS = 32;  // bitmap words per work item (one per lane)
-constexpr int MAX_CAND = CHUNK_WORDS * 32;
+constexpr int MAX_CAND = FCHUNK_VERTS;  // candidates of one vertex range
+static_assert(FCHUNK_VERTS <= 32 * (WAVE - 1), "a range's bitmap words fit one wave");
 
 __global__ void init_kernel(const uint32_t* __restrict__ key, uint32_t n, uint32_t W, const InitOp* __restrict__ ops,
                             uint32_t nops, uint32_t* __restrict__ bitmaps) {
@@ -85,6 +87,7 @@ __device__ __forceinline__ unsigned long long constraint_bits(uint32_t type, uin
 __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __restrict__ row,
                                                           const uint32_t* __restrict__ col,
                                                           const uint32_t* __restrict__ key, uint32_t W,
+                                                          const uint32_t* __restrict__ fchunk, uint32_t chunks,
                                                           const FilterOp* __restrict__ ops, uint32_t nops,
                                                           const uint32_t* __restrict__ nbr_slot,
                                                           const uint32_t* __restrict__ nbr_lo,
@@ -94,10 +97,12 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
   extern __shared__ unsigned char lds_raw[];
   WaveLds& L = reinterpret_cast<WaveLds*>(lds_raw)[threadIdx.x / WAVE];
   const uint32_t lane = lane_id();
-  const uint32_t chunks = (W + CHUNK_WORDS - 1) / CHUNK_WORDS;
   const uint64_t items = (uint64_t)nops * chunks;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
+  // per-lane tallies for the optional byte accounting, over all items
+  unsigned long long t_cand = 0;
+  uint32_t t_ent = 0, t_key = 0, t_probe = 0, t_set = 0;
 
   for (uint64_t it = wave; it < items; it += nwaves) {
     const uint32_t o = uniform((uint32_t)(it / chunks));
@@ -107,15 +112,22 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
     const uint32_t nb = uniform(ops[o].nbr_off);
     uint32_t* bm_u = bitmaps + (size_t)uniform(ops[o].u_slot) * W;
     const unsigned long long full = nv >= 64 ? ~0ULL : ((1ULL << nv) - 1);
-    // per-lane tallies for the optional byte accounting
-    uint32_t t_ent = 0, t_key = 0, t_probe = 0, t_set = 0;
 
-    // collect: candidates of this chunk, ascending
-    const uint32_t wi = c * CHUNK_WORDS + lane;
-    const uint32_t w = (lane < CHUNK_WORDS && wi < W) ? bm_u[wi] : 0u;
+    // collect: candidates of vertex range [v_lo, v_hi), ascending; lane l
+    // reads bitmap word w_lo + l and keeps the range's bits of it
+    const uint32_t v_lo = uniform(fchunk[c]), v_hi = uniform(fchunk[c + 1]);
+    const uint32_t w_lo = v_lo >> 5, nw = ((v_hi - 1) >> 5) - w_lo + 1;
+    const uint32_t wi = w_lo + lane;
+    uint32_t w = 0;
+    if (lane < nw) {
+      w = bm_u[wi];
+      if (lane == 0) w &= ~0u << (v_lo & 31);
+      if (lane == nw - 1 && (v_hi & 31)) w &= (1u << (v_hi & 31)) - 1u;
+    }
     const uint32_t pc = __popc(w);
     const uint32_t incl = wave_incl_scan(pc);
     const uint32_t C = uniform(__shfl(incl, WAVE - 1, WAVE));
+    if (lane == 0) t_cand += C;
     {
       uint32_t pos = incl - pc, x = w;
       while (x) {
@@ -221,15 +233,20 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
       }
       wave_sync();
     }
-    if (stats) {
-      const uint32_t e = wave_sum(t_ent), k = wave_sum(t_key), p = wave_sum(t_probe), b = wave_sum(t_set);
-      if (lane == 0) {
-        atomicAdd(stats + 0, (unsigned long long)C);
-        atomicAdd(stats + 1, (unsigned long long)e);
-        atomicAdd(stats + 2, (unsigned long long)k);
-        atomicAdd(stats + 3, (unsigned long long)p);
-        atomicAdd(stats + 4, (unsigned long long)b);
-      }
+  }
+  if (stats) {
+    // one atomic per counter per workgroup, spread over STAT_COPIES copies
+    __shared__ unsigned long long red[4][5];
+    const unsigned long long v[5] = {t_cand, wave_sum64(t_ent), wave_sum64(t_key), wave_sum64(t_probe),
+                                     wave_sum64(t_set)};
+    const uint32_t w = threadIdx.x / WAVE;
+    if (lane == 0)
+      for (int i = 0; i < 5; ++i) red[w][i] = v[i];
+    __syncthreads();
+    if (threadIdx.x < 5) {
+      unsigned long long tot = 0;
+      for (uint32_t x = 0; x < blockDim.x / WAVE; ++x) tot += red[x][threadIdx.x];
+      if (tot) atomicAdd(stats + (blockIdx.x % STAT_COPIES) * STAT_STRIDE + threadIdx.x, tot);
     }
   }
 }
@@ -250,14 +267,30 @@ void launch_filter_step(const DevGraph& g, const FilterOp* ops, uint32_t nops, c
                         unsigned long long* stats, hipStream_t s) {
   if (!nops) return;
   if (block != 64 && block != 128 && block != 256) block = 256;
-  const uint32_t chunks = (g.W + CHUNK_WORDS - 1) / CHUNK_WORDS;
-  const uint64_t items = (uint64_t)nops * chunks;
+  const uint64_t items = (uint64_t)nops * g.nfchunk;
   const uint32_t wpb = block / WAVE;
   uint64_t grid = (items + wpb - 1) / wpb;
   if (grid > 65536) grid = 65536;
   const size_t lds = sizeof(WaveLds) * wpb;
-  hipLaunchKernelGGL(filter_step_kernel, dim3((uint32_t)grid), dim3(block), lds, s, g.row, g.col, g.key, g.W, ops,
-                     nops, nbr_slot, nbr_lo, nbr_hi, bitmaps, stats);
+  hipLaunchKernelGGL(filter_step_kernel, dim3((uint32_t)grid), dim3(block), lds, s, g.row, g.col, g.key, g.W,
+                     g.fchunk, g.nfchunk, ops, nops, nbr_slot, nbr_lo, nbr_hi, bitmaps, stats);
+}
+
+void filter_chunks(uint32_t n, const uint32_t* row_offset, std::vector<uint32_t>& bounds) {
+  bounds.assign(1, 0);
+  uint32_t v = 0;
+  while (v < n) {
+    uint32_t end = v + 1;
+    uint64_t entries = row_offset[v + 1] - row_offset[v];
+    while (end < n && end - v < FCHUNK_VERTS) {
+      const uint64_t d = row_offset[end + 1] - row_offset[end];
+      if (entries + d > FCHUNK_ENTRIES) break;
+      entries += d;
+      ++end;
+    }
+    bounds.push_back(end);
+    v = end;
+  }
 }
 
 }  // namespace gpsm

@@ -4,8 +4,9 @@
 // The reference materialises every intermediate table breadth-first, finds
 // a row's vertex in the candidate list with a warp-linear scan (quadratic in
 // rows x candidates), and leaves injectivity to the host.  Here:
-//  * a persistent grid pulls tasks (a slice of one query's first table) from
-//    a device counter, so hundreds of uneven queries share one launch;
+//  * a persistent grid pulls tasks (every T-th row of one query's first
+//    table) from a device counter, so hundreds of uneven queries share one
+//    launch and hub rows are spread over a query's T tasks;
 //  * each workgroup walks its slice depth-first through levels k = 1..q-1
 //    (k+1 bound vertices) with one bounded column-major buffer per level, so
 //    memory is O(cap * q^2) per workgroup whatever the result size;
@@ -99,20 +100,22 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
     }
     __syncthreads();
     if (tid == 0) {
-      S.fill[1] = (uint32_t)(task.hi - task.lo);
-      S.lrows[1] = task.hi - task.lo;
+      S.fill[1] = (uint32_t)task.count;
+      S.lrows[1] = task.count;
       S.k = 1;
       // a query already over its work budget skips its remaining tasks
       S.abort = a.budget ? (uint32_t)__hip_atomic_load(&a.out[task.query].aborted, __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_AGENT)
                          : 0u;
       if (S.abort) S.fill[1] = 0;
+      else atomicAdd(&a.out[task.query].work, (unsigned long long)task.count);  // first-table rows
     }
     __syncthreads();
 
     if (q == 2) {
       // the first table is the embedding set
-      for (uint64_t e = task.lo + tid; e < task.hi && !S.abort; e += blockDim.x) {
+      for (uint64_t i = tid; i < task.count && !S.abort; i += blockDim.x) {
+        const uint64_t e = task.first + i * task.stride;
         const uint32_t j = seg_search(off0, d0.ncand + 1, (uint32_t)e);
         const uint32_t mu = cl0[j], mv = ls0[e];
         my_hash += fmix64(tuple_term(P.order[0], mu) + tuple_term(P.order[1], mv));
@@ -127,6 +130,7 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
       }
     } else {
       for (;;) {
+        if (S.abort) break;  // uniform: set by thread 0 before the last barrier
         const uint32_t k = S.k;
         if (S.cur_row[k] >= S.fill[k]) {
           __syncthreads();
@@ -156,7 +160,7 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
         if (tid < nw) {
           const uint32_t r = r0 + tid;
           if (k == 1) {
-            const uint64_t e = task.lo + r;
+            const uint64_t e = task.first + (uint64_t)r * task.stride;
             const uint32_t j = seg_search(off0, d0.ncand + 1, (uint32_t)e);
             val[0 * WROWS + tid] = cl0[j];
             val[1 * WROWS + tid] = ls0[e];
@@ -181,16 +185,7 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
         const uint32_t ex = block_excl_scan(ub, S.tmp, &T);
         S.pre[tid] = tid < nw ? ex : T;
         if (tid == 0 && !final) S.fill[nk] = 0;
-        if (tid == 0 && a.budget) {
-          // work budget: candidate rows examined by all tasks of this query
-          const unsigned long long before = atomicAdd(&a.out[task.query].work, (unsigned long long)T);
-          if (before + T > a.budget) {
-            a.out[task.query].aborted = 1;
-            S.abort = 1;
-          }
-        }
         __syncthreads();
-        if (S.abort) break;  // uniform
 
         // expand: EXP_UNROLL candidates per thread per step, list loads in
         // flight together; rows of the next level are appended per wave (one
@@ -259,6 +254,14 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
         if (!final && tid == 0) S.lrows[nk] += S.fill[nk];
         // advance level k's cursor past what was expanded
         if (tid == 0) {
+          // join work: candidates examined (a window cut short by a full
+          // buffer is reloaded later from the cursor), and the budget on it
+          const unsigned long long done = min(f, T);
+          const unsigned long long before = atomicAdd(&a.out[task.query].work, done);
+          if (a.budget && before + done > a.budget) {
+            a.out[task.query].aborted = 1;
+            S.abort = 1;
+          }
           if (f >= T) {
             S.cur_row[k] = r0 + nw;
             S.cur_cand[k] = 0;

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "gpsm_dev.h"
 
@@ -16,14 +17,29 @@ struct DevGraph {
   uint32_t* row = nullptr;
   uint32_t* col = nullptr;
   uint32_t* key = nullptr;
+  // filter work items: vertex ranges [fchunk[i], fchunk[i+1]) of at most
+  // FCHUNK_VERTS vertices and FCHUNK_ENTRIES adjacency entries (a vertex
+  // with more entries is a range of its own), so that no wave of the filter
+  // walks far more adjacency than the others (power-law hubs)
+  uint32_t* fchunk = nullptr;
+  uint32_t nfchunk = 0;
 };
+
+constexpr uint32_t FCHUNK_VERTS = 1024;
+constexpr uint32_t FCHUNK_ENTRIES = 4096;
+
+// Boundaries of the filter's work items over a CSR (host side).
+void filter_chunks(uint32_t n, const uint32_t* row_offset, std::vector<uint32_t>& bounds);
 
 // filter.hip ------------------------------------------------------------------
 // Start-vertex candidates: C(slot) = {v : lo <= key[v] <= hi} (check_kernel).
 void launch_init(const DevGraph& g, const InitOp* ops, uint32_t nops, uint32_t* bitmaps, hipStream_t s);
 // One superstep: every op (one per query) explores or refines its C(u).
-// stats (optional, 5 u64): candidates, adjacency entries read, key gathers,
-// bitmap probes, bits set.
+// stats (optional, STAT_COPIES x STAT_STRIDE u64, zeroed by the caller):
+// copy c, entries 0..4 = candidates, adjacency entries read, key gathers,
+// bitmap probes, bits set, added by the workgroups with blockIdx % copies
+// == c (the sum over copies is the launch's total).
+constexpr uint32_t STAT_COPIES = 32, STAT_STRIDE = 8;
 void launch_filter_step(const DevGraph& g, const FilterOp* ops, uint32_t nops, const uint32_t* nbr_slot,
                         const uint32_t* nbr_lo, const uint32_t* nbr_hi, uint32_t* bitmaps, uint32_t block,
                         unsigned long long* stats, hipStream_t s);

@@ -25,12 +25,13 @@ class MatchResult:
     cand: list
     device_ms: float
     status: int = 0
+    join_work: int = 0
     tuples: np.ndarray = field(default=None)
 
 
 def _result(r, q):
     return MatchResult(int(r.count), int(r.hash), list(r.order[:q]), [int(x) for x in r.level_rows[:q]],
-                       list(r.cand[:q]), float(r.device_ms), int(r.status))
+                       list(r.cand[:q]), float(r.device_ms), int(r.status), int(r.join_work))
 
 
 def _opts(order_mode=_lib.ORDER_HEURISTIC, explicit_order=None, want_tuples=False, tuple_cap=0, use_graph=False,
@@ -83,8 +84,10 @@ class Context:
     def upload(self, g):
         return ResidentGraph(self, g)
 
-    def profile(self, on=True):
-        check(_lib.lib().gpsm_profile_enable(self.h, int(on)), "gpsm_profile_enable")
+    def profile(self, mode=1):
+        """0 off, 1 per-launch timing, 2 timing + the filter's work counters
+        (algorithmic bytes); setting a mode clears the statistics."""
+        check(_lib.lib().gpsm_profile_enable(self.h, int(mode)), "gpsm_profile_enable")
 
     def kernel_stats(self):
         arr = (_lib.KStat * 16)()

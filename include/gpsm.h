@@ -88,6 +88,8 @@ typedef struct gpsm_result {
   uint32_t* tuples;                 /* library-owned, n_tuples x q, sorted */
   uint64_t n_tuples;
   double device_ms;                 /* device time of this call (batch: whole batch) */
+  uint64_t join_work;               /* candidates the join examined, all levels (the work the
+                                       planner minimises and opts.work_budget bounds) */
 } gpsm_result;
 
 /* --- lifecycle ---------------------------------------------------------- */
@@ -225,10 +227,13 @@ typedef struct gpsm_kstat {
   double bytes;       /* algorithmic bytes of all launches (0 if not accounted) */
 } gpsm_kstat;
 
-/* Enables (1) or disables (0) per-launch HIP-event timing and device byte
- * counters on the context's stream; enabling clears the statistics.
- * Launches captured in a hipGraph are not timed. */
-int gpsm_profile_enable(gpsm_ctx* ctx, int on);
+/* Per-launch statistics on the context's stream: mode 0 off, 1 HIP-event
+ * timing of every launch, 2 timing plus the filter kernel's device counters
+ * of the work it did (candidates, adjacency entries, gathers, probes; a
+ * few atomics per workgroup), from which its algorithmic bytes are
+ * computed.  Setting a mode clears the statistics.  Launches captured in a
+ * hipGraph are not timed. */
+int gpsm_profile_enable(gpsm_ctx* ctx, int mode);
 /* Copies up to max kernel-family statistics; *n receives the number. */
 int gpsm_profile_read(gpsm_ctx* ctx, gpsm_kstat* out, uint32_t max, uint32_t* n);
 

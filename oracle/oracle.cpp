@@ -282,6 +282,7 @@ struct Joiner {
   std::vector<uint32_t> pos2id;
   int mode;
   std::vector<uint64_t> step_rows;
+  uint64_t examined = 0;  // list entries visited by expand steps
   uint64_t count = 0, hash = 0;
   std::vector<uint32_t>* tuples = nullptr;
   uint64_t cap = 0, n_tuples = 0;
@@ -321,6 +322,7 @@ struct Joiner {
       // expand_kernel + link_kernel: append every list entry of row[pos]
       uint32_t j = t.rank[rowv[st.apos]];
       if (j == ~0u) return;
+      examined += t.off[j + 1] - t.off[j];
       for (uint32_t e = t.off[j]; e < t.off[j + 1]; ++e) {
         uint32_t mv = t.list[e];
         if (mode == 0 && std::find(rowv.begin(), rowv.end(), mv) != rowv.end()) continue;
@@ -558,6 +560,7 @@ int orc_o2_match(const orc_graph* g, const orc_query* q, int mode, const uint32_
     }
   out->count = J.count;
   out->hash = J.hash;
+  out->join_work = out->step_rows[0] + J.examined;
   out->n_tuples = J.n_tuples;
   if (out->tuples) {
     uint64_t rows = std::min<uint64_t>(J.n_tuples, out->tuple_cap);
@@ -577,6 +580,26 @@ int orc_o2_match(const orc_graph* g, const orc_query* q, int mode, const uint32_
 // vertex takes the data neighbours of its BFS parent's image that carry its
 // label, are unused, and are adjacent to the images of all its other earlier
 // neighbours.
+int orc_o2_match_many(const orc_graph* g, const orc_query* qs, uint32_t nq, int threads, uint64_t* counts,
+                      uint64_t* hashes) {
+  if (threads <= 0) threads = omp_get_max_threads();
+  int rc = 0;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+  for (int64_t i = 0; i < (int64_t)nq; ++i) {
+    orc_o2_out o;
+    o.tuples = nullptr;
+    o.tuple_cap = 0;
+    const int r = orc_o2_match(g, &qs[i], 0, nullptr, &o);
+    counts[i] = o.count;
+    hashes[i] = o.hash;
+    if (r < 0) {
+#pragma omp critical
+      if (!rc) rc = r;
+    }
+  }
+  return rc;
+}
+
 int orc_o1_enumerate(const orc_graph* g, const orc_query* q, int threads, uint64_t* count,
                      uint64_t* hash, uint32_t* tuples, uint64_t cap, uint64_t* n_tuples) {
   const uint32_t k = q->n;

@@ -71,6 +71,7 @@ typedef struct orc_o2_out {
   uint64_t step_rows[ORC_MAXE];   /* R after each edge step, as the reference's tables */
   uint32_t* tuples;               /* caller buffer [tuple_cap * q] or NULL */
   uint64_t tuple_cap, n_tuples;   /* tuples written sorted; n_tuples may exceed cap (then truncated) */
+  uint64_t join_work;             /* first-table rows + list entries visited by expand steps */
 } orc_o2_out;
 
 /* mode 0: expand rows are kept injective (the build's semantics);
@@ -80,6 +81,12 @@ typedef struct orc_o2_out {
  *        explicit vertex order. */
 int orc_o2_match(const orc_graph* g, const orc_query* q, int mode, const uint32_t* order,
                  orc_o2_out* out);
+
+/* O2 (mode 0, heuristic order) over nq queries, OpenMP across queries
+ * (threads <= 0: all cores): the all-cores CPU baseline.  counts/hashes
+ * [nq]; returns 0 or the first query's error. */
+int orc_o2_match_many(const orc_graph* g, const orc_query* qs, uint32_t nq, int threads, uint64_t* counts,
+                      uint64_t* hashes);
 
 /* Candidate byte sets after InitializeCandidateVertices (init, may be NULL)
  * and after RefineCandidateVertices (refined), each [q->n * g->n].

@@ -47,6 +47,7 @@ class O2Out(C.Structure):
         ("tuples", C.c_void_p),
         ("tuple_cap", C.c_uint64),
         ("n_tuples", C.c_uint64),
+        ("join_work", C.c_uint64),
     ]
 
 
@@ -58,6 +59,8 @@ def lib():
             raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
         L = C.CDLL(path)
         L.orc_o2_match.argtypes = [C.POINTER(Graph), C.POINTER(Query), C.c_int, C.c_void_p, C.POINTER(O2Out)]
+        L.orc_o2_match_many.argtypes = [C.POINTER(Graph), C.POINTER(Query), C.c_uint32, C.c_int, C.c_void_p,
+                                        C.c_void_p]
         L.orc_o2_filter.argtypes = [C.POINTER(Graph), C.POINTER(Query), C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_o2_cet.argtypes = [C.POINTER(Graph), C.POINTER(Query), C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32)]
@@ -130,6 +133,7 @@ def o2_match(g, q, mode=0, order=None, tuple_cap=0):
         "step_rows": list(out.step_rows[: out.n_steps]),
         "step_edge": list(out.step_edge[: out.n_steps]),
         "step_join": list(out.step_join[: out.n_steps]),
+        "join_work": out.join_work,
     }
     if tuple_cap:
         nt = min(out.n_tuples, tuple_cap)
@@ -169,6 +173,23 @@ def o2_cet(g, q):
         s = int(t[2 * m])
         tables.append((t[:m].copy(), t[m:2 * m + 1].copy(), t[2 * m + 1:2 * m + 1 + s].copy()))
     return [(int(ef[i]), int(et[i])) for i in range(ne.value)], tables
+
+
+def o2_match_many(g, qs, threads=0):
+    """O2 over a query list, OpenMP across queries: [(count, hash)]."""
+    gs, k1 = make_graph(g)
+    arr = (Query * len(qs))()
+    keep = []
+    for i, q in enumerate(qs):
+        s, k = make_query(q)
+        arr[i] = s
+        keep.append(k)
+    cnt = np.zeros(len(qs), dtype=np.uint64)
+    hs = np.zeros(len(qs), dtype=np.uint64)
+    rc = lib().orc_o2_match_many(C.byref(gs), arr, len(qs), threads, _ptr(cnt), _ptr(hs))
+    if rc < 0:
+        raise ValueError("orc_o2_match_many failed (%d)" % rc)
+    return [(int(c), int(h)) for c, h in zip(cnt, hs)]
 
 
 def o1_enumerate(g, q, threads=0, tuple_cap=0):

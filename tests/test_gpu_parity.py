@@ -107,6 +107,7 @@ def test_match_matches_oracle(ctx, case):
         assert r.order == r2["order"]
         assert r.level_rows[1:] == r2["level_rows"][1:q.n]
         assert r.cand == r2["cand"]
+        assert r.join_work == r2["join_work"]
 
 
 def test_er10k_walk6_full_tuple_list(ctx):
@@ -169,6 +170,7 @@ def test_batch_equals_single_and_oracle(ctx):
         r1 = O.o1_enumerate(g, q)
         assert (b.count, b.hash) == (s.count, s.hash) == (r1["count"], r1["hash"]) == (gr.count, gr.hash)
         assert b.level_rows == s.level_rows
+        assert b.join_work == s.join_work == gr.join_work
 
 
 def test_small_join_buffers_force_depth_first_spills(ctx):
@@ -179,6 +181,9 @@ def test_small_join_buffers_force_depth_first_spills(ctx):
         r1 = O.o1_enumerate(g, q)
         r = rg.match(q, join_cap=512)
         assert (r.count, r.hash) == (r1["count"], r1["hash"])
+        # windows cut short and reloaded are not counted twice
+        full = rg.match(q)
+        assert r.join_work == full.join_work and r.level_rows == full.level_rows
 
 
 def test_explicit_orders_agree(ctx):
@@ -202,3 +207,24 @@ def test_explicit_orders_agree(ctx):
         assert (r.count, r.hash) == (base.count, base.hash) == (r2["count"], r2["hash"])
         if r2["count"]:
             assert r.level_rows[1:] == r2["level_rows"][1:q.n]
+            assert r.join_work == r2["join_work"]
+
+
+def test_profile_modes_keep_results_and_count_bytes(ctx):
+    """Timing-only and counting profile modes time every launch family and
+    leave results unchanged; only the counting mode adds the filter's
+    measured work to its bytes."""
+    g = G.small_graph(12, n=3000, m=12000, labels=3, kind="cl")
+    qs = G.queries(g, 16, 3, 6, _lib.Q_WALK_HALF, 19, 10 ** 7)
+    rg = ctx.upload(g)
+    base = rg.match_batch(qs)
+    stats = {}
+    for mode in (1, 2):
+        ctx.profile(mode)
+        res = rg.match_batch(qs)
+        stats[mode] = ctx.kernel_stats()
+        assert [(r.count, r.hash) for r in res] == [(r.count, r.hash) for r in base]
+    ctx.profile(0)
+    for k in ("filter_step", "join", "cet_count", "cet_write"):
+        assert stats[1][k]["launches"] == stats[2][k]["launches"] > 0 and stats[1][k]["ms"] > 0
+    assert stats[2]["filter_step"]["bytes"] > stats[1]["filter_step"]["bytes"] > 0

@@ -55,13 +55,15 @@ def test_learned_order_same_results_and_less_join_work(ctx, mix):
     planner.install(ctx)
     learn = rg.match_batch(qs, order_mode=_lib.ORDER_LEARNED)
     assert [(r.count, r.hash) for r in heur] == [(r.count, r.hash) for r in learn]
-    # join work: rows of every intermediate level (the final level is the
-    # result itself and is equal for both orders)
-    wh = sum(sum(r.level_rows[1:len(r.order) - 1]) for r in heur)
-    wl = sum(sum(r.level_rows[1:len(r.order) - 1]) for r in learn)
+    # join work: candidates examined at every level, as the device counts them
+    wh = sum(r.join_work for r in heur)
+    wl = sum(r.join_work for r in learn)
+    rh = sum(sum(r.level_rows[1:len(r.order) - 1]) for r in heur)
+    rl = sum(sum(r.level_rows[1:len(r.order) - 1]) for r in learn)
     changed = sum(h.order != l.order for h, l in zip(heur, learn))
-    print("intermediate rows: heuristic %d, learned %d (%.2fx); %d of %d plans changed"
-          % (wh, wl, wh / max(wl, 1), changed, len(qs)))
+    print("join work: heuristic %d, learned %d (%.2fx); intermediate rows %d vs %d; %d of %d plans changed"
+          % (wh, wl, wh / max(wl, 1), rh, rl, changed, len(qs)))
+    assert all(r.join_work > 0 for r in heur if r.count)
     # the heuristic order is always a candidate and is replaced only when the
     # network predicts a clearly cheaper plan
     assert wl <= 1.10 * wh

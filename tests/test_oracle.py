@@ -209,6 +209,65 @@ def test_order_independence(case):
         assert (r["count"], r["hash"]) == (base["count"], base["hash"])
 
 
+def join_work_by_levels(g, q, cand, order, edges, size):
+    """First-table rows plus, for every later position k, the candidates the
+    expansion visits: |N(r[parent]) & C(w)| summed over the rows of level
+    k-1, where rows are the injective partial maps of order[:k] inside the
+    candidate sets with every query edge among them present.  The parent of
+    w is its smallest-|CET| edge (lowest index on ties) to an earlier vertex."""
+    nbr = [set(int(x) for x in g.col[g.row_offset[v]:g.row_offset[v + 1]]) for v in range(g.n)]
+    pos = {v: k for k, v in enumerate(order)}
+    a, b = order[0], order[1]
+    rows = [(x, y) for x in cand[a] for y in nbr[x] if y in cand[b]]
+    work = len(rows)
+    for k in range(2, q.n):
+        w = order[k]
+        mine = [i for i, (u, v) in enumerate(edges) if (u == w and pos[v] < k) or (v == w and pos[u] < k)]
+        pe = min(mine, key=lambda i: (size[i], i))
+        par = pos[edges[pe][0] if edges[pe][1] == w else edges[pe][1]]
+        backs = [pos[u if v == w else v] for i, (u, v) in enumerate(edges) if i in mine and i != pe]
+        nxt = []
+        for r in rows:
+            ext = [x for x in nbr[r[par]] if x in cand[w]]
+            work += len(ext)
+            nxt += [r + (x,) for x in ext if x not in r and all(x in nbr[r[p]] for p in backs)]
+        rows = nxt
+    return work, len(rows)
+
+
+@pytest.mark.parametrize("case", range(16))
+def test_join_work_by_levels(case):
+    """O2's join work (first-table rows + list entries its expand steps
+    visit) equals a level-by-level enumeration over its own candidate sets,
+    for the heuristic order and for random connected orders."""
+    g, q = small_cases(16, seed=71)[case]
+    base = O.o2_match(g, q, mode=0)
+    if base["status"] != O.OK or base["count"] == 0 or base["count"] > 200000:
+        return
+    st, _, ref, _ = O.o2_filter(g, q)
+    cand = [set(np.nonzero(ref[u])[0].tolist()) for u in range(q.n)]
+    edges, size = base["edges"], base["cet_size"]
+    orders = [base["order"]]
+    rng = np.random.default_rng(case)
+    adj = [set() for _ in range(q.n)]
+    for a, b in zip(q.src, q.dst):
+        adj[a].add(int(b))
+        adj[b].add(int(a))
+    for _ in range(3):
+        start = int(rng.integers(q.n))
+        order, seen = [start], {start}
+        while len(order) < q.n:
+            w = int(rng.choice(sorted({x for v in order for x in adj[v]} - seen)))
+            order.append(w)
+            seen.add(w)
+        orders.append(order)
+    for order in orders:
+        r = O.o2_match(g, q, mode=0, order=order)
+        work, count = join_work_by_levels(g, q, cand, order, edges, size)
+        assert count == r["count"] == base["count"]
+        assert r["join_work"] == work, (order, r["join_work"], work)
+
+
 @pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 1023, 1024, 1025, (1 << 20) + 3])
 def test_scan_oracle(n):
     rng = np.random.default_rng(n)
@@ -297,3 +356,14 @@ def test_mlp_oracle_bf16_rounding():
     y = O.mlp_forward(p, x, bf16=True)
     assert y[0] == 1.0
     assert y[1] == np.float32(1 + 2.0 ** -7)
+
+
+def test_o2_many_equals_single():
+    """The all-cores CPU baseline (O2 across queries) gives each query's
+    single-threaded O2 result."""
+    g = G.small_graph(5, n=3000, m=12000, labels=3, kind="cl")
+    qs = G.queries(g, 24, 3, 7, _lib.Q_WALK_HALF, 17, 10 ** 7)
+    many = O.o2_match_many(g, qs, threads=4)
+    for q, (c, h) in zip(qs, many):
+        r = O.o2_match(g, q, mode=0)
+        assert (c, h) == (r["count"], r["hash"])

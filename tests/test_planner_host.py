@@ -66,26 +66,30 @@ def run(driver, n, edges, labels, cand, cet, mode):
 
 
 def expected_cost(order, edges, cand, mode):
-    """Intermediate rows of levels 1..n-2: level 1 = |CET| of the first edge,
-    later levels through the predictor, each fed the previous prediction."""
+    """R_1 + sum over k = 2..n-1 of R_{k-1} * |CET(e_k)| / |C(parent)| * (1 + back
+    edges of position k): R_1 = |CET| of the first edge, later R_k through the
+    predictor, each fed the previous prediction; the final level adds its
+    examined candidates only."""
     n = len(order)
     pos = {v: k for k, v in enumerate(order)}
     first = next(i for i, (a, b, _) in enumerate(edges) if {a, b} == {order[0], order[1]})
-    if n <= 2:
-        return 0.0
-    cost = float(edges[first][2])
-    lr = math.log1p(edges[first][2])
-    for k in range(2, n - 1):
+    rows = float(edges[first][2])
+    cost, lr = rows, math.log1p(rows)
+    for k in range(2, n):
         w = order[k]
-        best = None
+        best, nb = None, 0
         for i, (a, b, s) in enumerate(edges):
             other = b if a == w else a if b == w else None
             if other is None or pos[other] >= k:
                 continue
+            nb += 1
             if best is None or s < edges[best][2]:
                 best = i
         a, b, s = edges[best]
         parent = b if a == w else a
+        cost += rows * s / max(int(cand[parent]), 1) * nb  # nb = 1 + back edges (no parallel edges here)
+        if k == n - 1:
+            break
         if mode == "zero":
             y = 0.0
         elif mode == "cet":
@@ -93,8 +97,7 @@ def expected_cost(order, edges, cand, mode):
         else:
             y = lr + math.log1p(s) - math.log1p(int(cand[parent]))
         y = max(0.0, y)
-        cost += math.expm1(y)
-        lr = y
+        rows, lr = math.expm1(y), y
     return cost
 
 
@@ -138,6 +141,3 @@ def test_plan_choice(driver, seed, mode):
         assert costs[out["chosen"]] <= costs[0] * (1 + 1e-4)
     else:
         assert out["chosen"] == best, (costs, out["chosen"], best)
-    if mode == "zero":
-        # with no predicted rows beyond level 1 the heuristic (smallest first table) stays
-        assert out["chosen"] == 0
