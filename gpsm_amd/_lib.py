@@ -60,7 +60,7 @@ class Result(C.Structure):
     _fields_ = [("status", C.c_int32), ("q", C.c_uint32), ("count", C.c_uint64), ("hash", C.c_uint64),
                 ("order", C.c_uint32 * MAX_QV), ("level_rows", C.c_uint64 * MAX_QV),
                 ("cand", C.c_uint32 * MAX_QV), ("tuples", C.POINTER(C.c_uint32)), ("n_tuples", C.c_uint64),
-                ("device_ms", C.c_double), ("join_work", C.c_uint64)]
+                ("device_ms", C.c_double), ("join_work", C.c_uint64), ("level_work", C.c_uint64 * MAX_QV)]
 
 
 class KStat(C.Structure):

@@ -528,30 +528,36 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     for (size_t e = 0; e < E; ++e) S.cet_size[e] = fdesc[fdesc_of_q[b] + e].total;
   }
   const bool learned = o.order_mode == GPSM_ORDER_LEARNED && c->planner;
+  // planning statistics of every query; est: predicted join work of its plan
+  std::vector<std::vector<uint32_t>> cand(nq);
+  std::vector<PlanContext> pcs(nq);
+  std::vector<double> est(nq, 0.0);
+  std::vector<const PlanContext*> live;
+  std::vector<uint32_t> live_b;
+  for (uint32_t b = 0; b < nq; ++b) {
+    QState& S = Q[b];
+    if (!S.active) continue;
+    cand[b].resize(S.hq.n);
+    for (uint32_t u = 0; u < S.hq.n; ++u) cand[b][u] = counts[S.vbase + u];
+    pcs[b] = PlanContext{&S.hq, &st, &cand[b], &S.cet_size};
+    live.push_back(&pcs[b]);
+    live_b.push_back(b);
+  }
   if (learned) {
     // every planning round of the whole batch is one network call
-    std::vector<std::vector<uint32_t>> cand(nq);
-    std::vector<PlanContext> pcs(nq);
-    std::vector<const PlanContext*> live;
-    std::vector<uint32_t> live_b;
-    for (uint32_t b = 0; b < nq; ++b) {
-      QState& S = Q[b];
-      if (!S.active) continue;
-      cand[b].resize(S.hq.n);
-      for (uint32_t u = 0; u < S.hq.n; ++u) cand[b][u] = counts[S.vbase + u];
-      pcs[b] = PlanContext{&S.hq, &st, &cand[b], &S.cet_size};
-      live.push_back(&pcs[b]);
-      live_b.push_back(b);
-    }
     std::vector<OrderPlan> chosen_plan;
+    std::vector<double> cost;
     const int prc = learned_plans(
         live,
         [](void* u, const float* x, uint32_t B, float* y) {
           return gpsm_mlp_forward(static_cast<gpsm_mlp*>(u), x, B, y);
         },
-        c->planner, st.adj + 1, chosen_plan, nullptr);
+        c->planner, st.adj + 1, chosen_plan, nullptr, &cost);
     if (prc) return prc;
-    for (size_t i = 0; i < live_b.size(); ++i) Q[live_b[i]].plan = std::move(chosen_plan[i]);
+    for (size_t i = 0; i < live_b.size(); ++i) {
+      Q[live_b[i]].plan = std::move(chosen_plan[i]);
+      est[live_b[i]] = cost[i];
+    }
   }
   for (uint32_t b = 0; b < nq; ++b) {
     QState& S = Q[b];
@@ -572,6 +578,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       continue;
     }
     for (uint32_t p = 0; p < S.hq.n && p < GPSM_MAX_QV; ++p) out[b].order[p] = S.plan.order[p];
+    if (!learned) est[b] = analytic_cost(pcs[b], S.plan);
     // any empty forward table means no embedding
     bool empty = false;
     for (size_t e = 0; e < E; ++e) empty |= S.cet_size[e] == 0;
@@ -614,18 +621,28 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     plan_of_q[b] = (uint32_t)plans.size();
     plans.push_back(jp);
   }
-  // tasks: interleaved row sets of the first tables, large enough to
-  // amortise a task's setup, small enough to balance the persistent grid
+  // tasks: interleaved row sets of the first tables.  A query gets tasks in
+  // proportion to its predicted join work (about 8 tasks per workgroup over
+  // the batch), and at least one per tsize first-table rows, so a query
+  // whose few first rows fan out to most of the batch's work does not leave
+  // a handful of long tasks running after the rest of the grid is idle.
   const uint32_t nwg = std::max<uint32_t>(1, c->ncu * wg_per_cu);
   uint64_t rows_total = 0;
+  double est_total = 0;
   for (uint32_t b = 0; b < nq; ++b)
-    if (plan_of_q[b] != ~0u) rows_total += Q[b].cet_size[Q[b].plan.first_edge];
+    if (plan_of_q[b] != ~0u) {
+      rows_total += Q[b].cet_size[Q[b].plan.first_edge];
+      est_total += est[b];
+    }
   const uint64_t tsize =
       std::min<uint64_t>(std::max<uint64_t>(rows_total / ((uint64_t)nwg * 8) + 1, c->tune.task_min), 1u << 20);
+  const double target = est_total / ((double)nwg * 8.0);
   for (uint32_t b = 0; b < nq; ++b) {
     if (plan_of_q[b] == ~0u) continue;
     const uint64_t R1 = Q[b].cet_size[Q[b].plan.first_edge];
-    const uint64_t T = (R1 + tsize - 1) / tsize;
+    uint64_t T = (R1 + tsize - 1) / tsize;
+    if (target > 0 && est[b] > target) T = std::max<uint64_t>(T, (uint64_t)std::min(est[b] / target + 1.0, 1e9));
+    T = std::min<uint64_t>(std::max<uint64_t>(T, 1), R1);
     for (uint64_t t = 0; t < T; ++t) tasks.push_back({plan_of_q[b], (uint32_t)T, t, (R1 - t + T - 1) / T});
   }
 
@@ -748,6 +765,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       out[b].hash = r.hash;
       for (uint32_t k = 1; k < Q[b].hq.n && k < GPSM_MAX_QV; ++k) out[b].level_rows[k] = r.level_rows[k];
       out[b].join_work = r.work;
+      for (uint32_t k = 1; k < Q[b].hq.n && k < GPSM_MAX_QV; ++k) out[b].level_work[k] = r.level_work[k];
       if (r.aborted) out[b].status = GPSM_EBUDGET;
     }
     if (emit_cap && nq == 1 && plan_of_q[0] != ~0u) {

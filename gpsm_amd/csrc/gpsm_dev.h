@@ -85,6 +85,7 @@ struct JoinOut {
   unsigned long long bytes;  // algorithmic bytes the join read and wrote for this query
   unsigned long long work;   // candidate rows examined (checked against the work budget)
   unsigned long long aborted;
+  unsigned long long level_work[MAXQ];  // [1]: first-table rows; [k]: candidates examined for level k
 };
 
 }  // namespace gpsm

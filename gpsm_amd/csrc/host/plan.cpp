@@ -535,7 +535,7 @@ double examine_factor(const PlanCost& s, uint32_t k) {
   const uint32_t e = s.plan.par_edge[k], w = s.plan.order[k];
   const uint32_t parent = q.ef[e] == w ? q.et[e] : q.ef[e];
   const double fan = (double)(*s.c->cet_size)[e] / (double)std::max<uint32_t>((*s.c->cand)[parent], 1);
-  return fan * (1.0 + (double)s.plan.backs[k].size());
+  return fan * (1.0 + kBackWeight * (double)s.plan.backs[k].size());
 }
 
 // The final level needs no prediction: add its examined candidates and stop.
@@ -581,11 +581,25 @@ void cost_consume(PlanCost& s, const float* y) {
   settle(s);
 }
 
+double analytic_cost(const PlanContext& c, const OrderPlan& plan) {
+  PlanCost s;
+  if (!cost_start(c, plan.order.data(), s)) return 0.0;
+  std::vector<float> f;
+  while (!s.done) {
+    f.clear();
+    cost_emit(s, f);
+    const float y = f[27];
+    cost_consume(s, &y);
+  }
+  return s.cost;
+}
+
 int learned_plans(const std::vector<const PlanContext*>& qs, PredictFn predict, void* user, uint64_t big,
-                  std::vector<OrderPlan>& out, std::vector<uint32_t>* chosen) {
+                  std::vector<OrderPlan>& out, std::vector<uint32_t>* chosen, std::vector<double>* cost) {
   const size_t nq = qs.size();
   out.assign(nq, OrderPlan());
   if (chosen) chosen->assign(nq, 0);
+  if (cost) cost->assign(nq, 0.0);
   std::vector<float> feats, y;
   // 1. greedy learned orders, one decision round at a time for the batch
   std::vector<LearnedState> ls(nq);
@@ -648,10 +662,12 @@ int learned_plans(const std::vector<const PlanContext*>& qs, PredictFn predict, 
     if (best && !(pc[b][best].cost < kLearnedMargin * pc[b][0].cost)) best = 0;
     if (pc[b][best].cost >= 1e300) {
       heuristic_order(*qs[b]->q, *qs[b]->cet_size, big, out[b]);
+      if (cost) (*cost)[b] = analytic_cost(*qs[b], out[b]);
       continue;
     }
     out[b] = pc[b][best].plan;
     if (chosen) (*chosen)[b] = (uint32_t)best;
+    if (cost) (*cost)[b] = pc[b][best].cost;
   }
   return 0;
 }

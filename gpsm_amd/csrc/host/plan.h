@@ -120,16 +120,25 @@ bool learned_finish(const LearnedState& s, OrderPlan& out);
 void candidate_orders(const HostQuery& q, const std::vector<uint64_t>& cet_size, uint64_t big,
                       std::vector<std::vector<uint32_t>>& out);
 
-// Predicted join cost of one vertex order: the rows of the first table plus
-// the candidates every later level examines,
-//   cost = R_1 + sum_{k=2}^{n-1} R_{k-1} * |CET(e_k)| / |C(parent_k)| * (1 + b_k),
+// Predicted join cost of one vertex order, in units of one examined
+// candidate: the rows of the first table plus the candidates every later
+// level examines, each weighted by its edge-membership checks,
+//   cost = R_1 + sum_{k=2}^{n-1} R_{k-1} * |CET(e_k)| / |C(parent_k)| * (1 + kBackWeight * b_k),
 // with e_k the expand edge of position k, b_k its join-mode (back) edges
-// (each an edge-membership check per candidate) and R_j the rows of level j:
+// and R_j the rows of level j:
 // R_1 = |CET| of the first edge exactly, R_2 .. R_{n-2} chained through the
 // network (each level's features take the previous level's predicted log
 // rows).  The final level's examined candidates are counted; its rows, the
 // embedding count, are the same for every order.  Evaluated one level per
 // round so that all candidates of a batch share one network call per round.
+// Join time of one back-edge check (a binary search in the shorter of two
+// adjacency lists) in units of one examined candidate, fitted on an MI355X
+// to the join kernel's time over batches of the 100 walk queries on the
+// 1M-vertex Chung-Lu graph, each query taking one of its candidate orders at
+// random (gpsm_amd/calibrate_planner.py: 24 batches, 24.3-29.9 ms, largest
+// error 3.3 ms; first-table rows and materialised rows got weight 0).
+constexpr double kBackWeight = 6.4;
+
 struct PlanCost {
   const PlanContext* c = nullptr;
   OrderPlan plan;
@@ -154,6 +163,12 @@ constexpr double kLearnedMargin = 0.9;
 // given) is the index of the candidate taken, 0 = the heuristic order.
 // Returns 0 or the first non-zero code of predict.
 int learned_plans(const std::vector<const PlanContext*>& qs, PredictFn predict, void* user, uint64_t big,
-                  std::vector<OrderPlan>& out, std::vector<uint32_t>* chosen);
+                  std::vector<OrderPlan>& out, std::vector<uint32_t>* chosen, std::vector<double>* cost = nullptr);
+
+// The same cost of a complete plan with the analytic chain estimate in place
+// of the network: each level's log rows = previous log rows + log(|CET| of
+// the expand edge / |C(parent)|) + the log densities of its back edges
+// (feature 27 of step_features).  Used to size the join's tasks.
+double analytic_cost(const PlanContext& c, const OrderPlan& plan);
 
 }  // namespace gpsm

@@ -66,6 +66,7 @@ struct JoinShared {
   uint32_t tmp[8];
   uint32_t fill[MAXQ], cur_row[MAXQ], cur_cand[MAXQ];
   unsigned long long lrows[MAXQ];
+  unsigned long long lwork[MAXQ];
   uint32_t task, k, abort;
   unsigned long long red[4];
   // followed by the row window: val[c * WROWS + r], c < qmax
@@ -94,6 +95,7 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
 
     if (tid < MAXQ) {
       S.lrows[tid] = 0;
+      S.lwork[tid] = 0;
       S.fill[tid] = 0;
       S.cur_row[tid] = 0;
       S.cur_cand[tid] = 0;
@@ -102,12 +104,13 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
     if (tid == 0) {
       S.fill[1] = (uint32_t)task.count;
       S.lrows[1] = task.count;
+      S.lwork[1] = task.count;
       S.k = 1;
       // a query already over its work budget skips its remaining tasks
       S.abort = a.budget ? (uint32_t)__hip_atomic_load(&a.out[task.query].aborted, __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_AGENT)
                          : 0u;
-      if (S.abort) S.fill[1] = 0;
+      if (S.abort) S.fill[1] = S.lwork[1] = 0;
       else atomicAdd(&a.out[task.query].work, (unsigned long long)task.count);  // first-table rows
     }
     __syncthreads();
@@ -258,6 +261,7 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
           // buffer is reloaded later from the cursor), and the budget on it
           const unsigned long long done = min(f, T);
           const unsigned long long before = atomicAdd(&a.out[task.query].work, done);
+          S.lwork[nk] += done;
           if (a.budget && before + done > a.budget) {
             a.out[task.query].aborted = 1;
             S.abort = 1;
@@ -311,6 +315,7 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
       atomicAdd(&a.out[task.query].bytes, bsum);
     }
     if (tid >= 1 && tid < q && S.lrows[tid]) atomicAdd(&a.out[task.query].level_rows[tid], S.lrows[tid]);
+    if (tid >= 1 && tid < q && S.lwork[tid]) atomicAdd(&a.out[task.query].level_work[tid], S.lwork[tid]);
     __syncthreads();
   }
 }

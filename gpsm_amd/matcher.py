@@ -26,12 +26,14 @@ class MatchResult:
     device_ms: float
     status: int = 0
     join_work: int = 0
+    level_work: list = field(default=None)
     tuples: np.ndarray = field(default=None)
 
 
 def _result(r, q):
     return MatchResult(int(r.count), int(r.hash), list(r.order[:q]), [int(x) for x in r.level_rows[:q]],
-                       list(r.cand[:q]), float(r.device_ms), int(r.status), int(r.join_work))
+                       list(r.cand[:q]), float(r.device_ms), int(r.status), int(r.join_work),
+                       [int(x) for x in r.level_work[:q]])
 
 
 def _opts(order_mode=_lib.ORDER_HEURISTIC, explicit_order=None, want_tuples=False, tuple_cap=0, use_graph=False,

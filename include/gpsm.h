@@ -88,8 +88,10 @@ typedef struct gpsm_result {
   uint32_t* tuples;                 /* library-owned, n_tuples x q, sorted */
   uint64_t n_tuples;
   double device_ms;                 /* device time of this call (batch: whole batch) */
-  uint64_t join_work;               /* candidates the join examined, all levels (the work the
-                                       planner minimises and opts.work_budget bounds) */
+  uint64_t join_work;               /* first-table rows + candidates the join examined at every
+                                       later level (the work opts.work_budget bounds) */
+  uint64_t level_work[GPSM_MAX_QV]; /* [1]: first-table rows; [k >= 2]: candidates examined
+                                       to produce the rows of step k (sum = join_work) */
 } gpsm_result;
 
 /* --- lifecycle ---------------------------------------------------------- */

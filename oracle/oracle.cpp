@@ -282,7 +282,8 @@ struct Joiner {
   std::vector<uint32_t> pos2id;
   int mode;
   std::vector<uint64_t> step_rows;
-  uint64_t examined = 0;  // list entries visited by expand steps
+  uint64_t examined = 0;           // list entries visited by expand steps
+  std::vector<uint64_t> step_exam;  // ... per step
   uint64_t count = 0, hash = 0;
   std::vector<uint32_t>* tuples = nullptr;
   uint64_t cap = 0, n_tuples = 0;
@@ -323,6 +324,7 @@ struct Joiner {
       uint32_t j = t.rank[rowv[st.apos]];
       if (j == ~0u) return;
       examined += t.off[j + 1] - t.off[j];
+      step_exam[s] += t.off[j + 1] - t.off[j];
       for (uint32_t e = t.off[j]; e < t.off[j + 1]; ++e) {
         uint32_t mv = t.list[e];
         if (mode == 0 && std::find(rowv.begin(), rowv.end(), mv) != rowv.end()) continue;
@@ -529,6 +531,7 @@ int orc_o2_match(const orc_graph* g, const orc_query* q, int mode, const uint32_
     J.steps.assign(steps.begin() + 1, steps.end());
     std::vector<uint64_t> rest(J.steps.size(), 0);
     J.step_rows = rest;
+    J.step_exam = rest;
     uint64_t r0 = 0;
     for (size_t j = 0; j < t0.ids.size(); ++j)
       for (uint32_t e = t0.off[j]; e < t0.off[j + 1]; ++e) {
@@ -561,6 +564,9 @@ int orc_o2_match(const orc_graph* g, const orc_query* q, int mode, const uint32_
   out->count = J.count;
   out->hash = J.hash;
   out->join_work = out->step_rows[0] + J.examined;
+  out->level_work[1] = out->step_rows[0];
+  for (size_t s = 1, w = 2; s < steps.size(); ++s)
+    if (!steps[s].join) out->level_work[w++] = J.step_exam[s - 1];
   out->n_tuples = J.n_tuples;
   if (out->tuples) {
     uint64_t rows = std::min<uint64_t>(J.n_tuples, out->tuple_cap);

@@ -72,6 +72,7 @@ typedef struct orc_o2_out {
   uint32_t* tuples;               /* caller buffer [tuple_cap * q] or NULL */
   uint64_t tuple_cap, n_tuples;   /* tuples written sorted; n_tuples may exceed cap (then truncated) */
   uint64_t join_work;             /* first-table rows + list entries visited by expand steps */
+  uint64_t level_work[ORC_MAXQ];  /* [1] first-table rows, [k] entries visited to reach width k+1 */
 } orc_o2_out;
 
 /* mode 0: expand rows are kept injective (the build's semantics);

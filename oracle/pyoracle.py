@@ -48,6 +48,7 @@ class O2Out(C.Structure):
         ("tuple_cap", C.c_uint64),
         ("n_tuples", C.c_uint64),
         ("join_work", C.c_uint64),
+        ("level_work", C.c_uint64 * MAXQ),
     ]
 
 
@@ -134,6 +135,7 @@ def o2_match(g, q, mode=0, order=None, tuple_cap=0):
         "step_edge": list(out.step_edge[: out.n_steps]),
         "step_join": list(out.step_join[: out.n_steps]),
         "join_work": out.join_work,
+        "level_work": list(out.level_work[: len(q.labels)]),
     }
     if tuple_cap:
         nt = min(out.n_tuples, tuple_cap)

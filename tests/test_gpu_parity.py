@@ -107,7 +107,8 @@ def test_match_matches_oracle(ctx, case):
         assert r.order == r2["order"]
         assert r.level_rows[1:] == r2["level_rows"][1:q.n]
         assert r.cand == r2["cand"]
-        assert r.join_work == r2["join_work"]
+        assert r.join_work == r2["join_work"] == sum(r.level_work)
+        assert r.level_work[1:] == r2["level_work"][1:q.n]
 
 
 def test_er10k_walk6_full_tuple_list(ctx):
@@ -171,6 +172,7 @@ def test_batch_equals_single_and_oracle(ctx):
         assert (b.count, b.hash) == (s.count, s.hash) == (r1["count"], r1["hash"]) == (gr.count, gr.hash)
         assert b.level_rows == s.level_rows
         assert b.join_work == s.join_work == gr.join_work
+        assert b.level_work == s.level_work == gr.level_work
 
 
 def test_small_join_buffers_force_depth_first_spills(ctx):
@@ -184,6 +186,7 @@ def test_small_join_buffers_force_depth_first_spills(ctx):
         # windows cut short and reloaded are not counted twice
         full = rg.match(q)
         assert r.join_work == full.join_work and r.level_rows == full.level_rows
+        assert r.level_work == full.level_work
 
 
 def test_explicit_orders_agree(ctx):
@@ -208,6 +211,7 @@ def test_explicit_orders_agree(ctx):
         if r2["count"]:
             assert r.level_rows[1:] == r2["level_rows"][1:q.n]
             assert r.join_work == r2["join_work"]
+            assert r.level_work[1:] == r2["level_work"][1:q.n]
 
 
 def test_profile_modes_keep_results_and_count_bytes(ctx):

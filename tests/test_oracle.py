@@ -219,7 +219,7 @@ def join_work_by_levels(g, q, cand, order, edges, size):
     pos = {v: k for k, v in enumerate(order)}
     a, b = order[0], order[1]
     rows = [(x, y) for x in cand[a] for y in nbr[x] if y in cand[b]]
-    work = len(rows)
+    levels = [0, len(rows)]
     for k in range(2, q.n):
         w = order[k]
         mine = [i for i, (u, v) in enumerate(edges) if (u == w and pos[v] < k) or (v == w and pos[u] < k)]
@@ -227,12 +227,13 @@ def join_work_by_levels(g, q, cand, order, edges, size):
         par = pos[edges[pe][0] if edges[pe][1] == w else edges[pe][1]]
         backs = [pos[u if v == w else v] for i, (u, v) in enumerate(edges) if i in mine and i != pe]
         nxt = []
+        levels.append(0)
         for r in rows:
             ext = [x for x in nbr[r[par]] if x in cand[w]]
-            work += len(ext)
+            levels[k] += len(ext)
             nxt += [r + (x,) for x in ext if x not in r and all(x in nbr[r[p]] for p in backs)]
         rows = nxt
-    return work, len(rows)
+    return levels, len(rows)
 
 
 @pytest.mark.parametrize("case", range(16))
@@ -263,9 +264,10 @@ def test_join_work_by_levels(case):
         orders.append(order)
     for order in orders:
         r = O.o2_match(g, q, mode=0, order=order)
-        work, count = join_work_by_levels(g, q, cand, order, edges, size)
+        levels, count = join_work_by_levels(g, q, cand, order, edges, size)
         assert count == r["count"] == base["count"]
-        assert r["join_work"] == work, (order, r["join_work"], work)
+        assert r["level_work"][1:] == levels[1:], (order, r["level_work"], levels)
+        assert r["join_work"] == sum(levels)
 
 
 @pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 1023, 1024, 1025, (1 << 20) + 3])

@@ -16,6 +16,7 @@ from gpsm_amd.train_planner import continuations
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 MARGIN = 0.9  # kLearnedMargin in plan.h
+BACK = 6.4  # kBackWeight in plan.h
 
 
 @pytest.fixture(scope="module")
@@ -66,10 +67,10 @@ def run(driver, n, edges, labels, cand, cet, mode):
 
 
 def expected_cost(order, edges, cand, mode):
-    """R_1 + sum over k = 2..n-1 of R_{k-1} * |CET(e_k)| / |C(parent)| * (1 + back
-    edges of position k): R_1 = |CET| of the first edge, later R_k through the
-    predictor, each fed the previous prediction; the final level adds its
-    examined candidates only."""
+    """R_1 + sum over k = 2..n-1 of R_{k-1} * |CET(e_k)| / |C(parent)| * (1 +
+    BACK * back edges of position k): R_1 = |CET| of the first edge, later R_k
+    through the predictor, each fed the previous prediction; the final level
+    adds its examined candidates only."""
     n = len(order)
     pos = {v: k for k, v in enumerate(order)}
     first = next(i for i, (a, b, _) in enumerate(edges) if {a, b} == {order[0], order[1]})
@@ -87,7 +88,7 @@ def expected_cost(order, edges, cand, mode):
                 best = i
         a, b, s = edges[best]
         parent = b if a == w else a
-        cost += rows * s / max(int(cand[parent]), 1) * nb  # nb = 1 + back edges (no parallel edges here)
+        cost += rows * s / max(int(cand[parent]), 1) * (1 + BACK * (nb - 1))  # no parallel edges here
         if k == n - 1:
             break
         if mode == "zero":
