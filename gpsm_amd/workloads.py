@@ -79,3 +79,100 @@ def rmat_shape_queries(g, seed=5, budget=EMBEDDING_BUDGET, per_kind=8):
     qs += G.queries(g, per_kind, 4, 8, _lib.Q_CLIQUE, seed + 1, budget)
     qs += G.queries(g, per_kind, 8, 50, _lib.Q_WALK_INDUCED, seed + 2, budget)
     return qs
+
+
+def build(name, seed=None, budget=EMBEDDING_BUDGET):
+    """(graph, queries) of a named workload."""
+    if name == "er10k_walk6":
+        return er10k_walk6(*(() if seed is None else (seed,)))
+    if name == "chunglu1m_walk8_16":
+        return chunglu1m_walk8_16(2 if seed is None else seed, budget)
+    if name == "chunglu1m_mix4_32":
+        return chunglu1m_mix4_32(2 if seed is None else seed, budget)
+    if name == "rmat22_64m":
+        g = rmat22_64m(5 if seed is None else seed)
+        return g, rmat_shape_queries(g, 5 if seed is None else seed, budget)
+    raise ValueError("unknown workload %r" % name)
+
+
+def run(name, steps=5, autotune=False, plain=True, seed=None, budget=EMBEDDING_BUDGET, log=print):
+    """Times one workload on the device: the batched launch with the
+    heuristic and with the learned join order (when trained weights exist),
+    and the plain per-query path; all three must agree on every count and
+    hash.  Returns a dict of rates (queries/s, embeddings/s) and kernel times."""
+    import time
+
+    from . import planner
+    from .matcher import Context
+
+    t0 = time.time()
+    g, qs = build(name, seed, budget)
+    t_gen = time.time() - t0
+    log("%s: n=%d, %d undirected edges, %d queries (sizes %s), generated in %.1f s"
+        % (name, g.n, g.adj // 2, len(qs), sorted({q.n for q in qs}), t_gen))
+    out = {"workload": name, "n": g.n, "undirected_edges": g.adj // 2, "queries": len(qs),
+           "query_sizes": {str(k): [q.n for q in qs].count(k) for k in sorted({q.n for q in qs})}}
+    with Context(0) as ctx:
+        rg = ctx.upload(g)
+        out["device"] = ctx.device_name
+        if autotune:
+            ms, tune = rg.autotune(qs[:min(len(qs), 64)])
+            out["tuning"] = tune
+            log("autotuned: %s (%.2f ms)" % (tune, ms))
+        modes = [("heuristic", _lib.ORDER_HEURISTIC)]
+        if planner.install(ctx):
+            modes.append(("learned", _lib.ORDER_LEARNED))
+        ref = None
+        for label, mode in modes:
+            rg.match_batch(qs, order_mode=mode)
+            ctx.profile(1)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                res = rg.match_batch(qs, order_mode=mode)
+            dt = (time.perf_counter() - t0) / steps
+            ks = ctx.kernel_stats()
+            ctx.profile(0)
+            got = [(r.count, r.hash, r.status) for r in res]
+            assert ref is None or got == ref, "%s order changed a result" % label
+            ref = got
+            emb = sum(r.count for r in res)
+            out["batched_" + label] = {
+                "queries_per_s": len(qs) / dt, "embeddings_per_s": emb / dt, "ms_per_batch": 1e3 * dt,
+                "join_work": sum(r.join_work for r in res),
+                "kernels_ms": {k: v["ms"] / steps for k, v in ks.items()}}
+            log("batched %-9s %9.1f queries/s  %.3g embeddings/s  %.2f ms/batch"
+                % (label, len(qs) / dt, emb / dt, 1e3 * dt))
+        out["embeddings"] = sum(c for c, _, _ in ref)
+        out["statuses"] = sorted({s for _, _, s in ref})
+        if plain:
+            rg.match(qs[0])
+            t0 = time.perf_counter()
+            rp = [rg.match(q) for q in qs]
+            dt = time.perf_counter() - t0
+            assert [(r.count, r.hash, r.status) for r in rp] == ref, "plain path changed a result"
+            out["plain_heuristic"] = {"queries_per_s": len(qs) / dt, "embeddings_per_s": out["embeddings"] / dt}
+            log("plain     heuristic %9.1f queries/s" % (len(qs) / dt))
+    return out
+
+
+def main():
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="Times a benchmark workload on the device")
+    ap.add_argument("workload", choices=["er10k_walk6", "chunglu1m_walk8_16", "chunglu1m_mix4_32", "rmat22_64m"])
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--budget", type=float, default=EMBEDDING_BUDGET)
+    ap.add_argument("--json", default=None)
+    args = ap.parse_args()
+    out = run(args.workload, args.steps, args.autotune, not args.no_plain, budget=int(args.budget),
+              log=lambda s: print(s, flush=True))
+    if args.json:
+        with open(args.json, "w") as fh:
+            json.dump(out, fh, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -16,6 +16,9 @@ import pyoracle as O
 from gpsm_amd import _lib
 from gpsm_amd import graphs as G
 from gpsm_amd import planner
+from gpsm_amd.calibrate_planner import backs_per_level
+
+BACK = 6.4  # kBackWeight in gpsm_amd/csrc/host/plan.h
 
 pytestmark = pytest.mark.gpu
 
@@ -55,14 +58,21 @@ def test_learned_order_same_results_and_less_join_work(ctx, mix):
     planner.install(ctx)
     learn = rg.match_batch(qs, order_mode=_lib.ORDER_LEARNED)
     assert [(r.count, r.hash) for r in heur] == [(r.count, r.hash) for r in learn]
-    # join work: candidates examined at every level, as the device counts them
-    wh = sum(r.join_work for r in heur)
-    wl = sum(r.join_work for r in learn)
-    rh = sum(sum(r.level_rows[1:len(r.order) - 1]) for r in heur)
-    rl = sum(sum(r.level_rows[1:len(r.order) - 1]) for r in learn)
+    # join work as the device counts it per level, each examined candidate
+    # weighted by the edge-membership checks of its level (the planner's
+    # cost unit, with the weight measured for a back-edge check)
+    def weighted(q, r):
+        tabs = rg.cet_edges(q)
+        backs = backs_per_level(q, r.order, *tabs)
+        return sum(r.level_work[k] * (1 + BACK * backs[k]) for k in range(1, q.n))
+
+    wh = sum(weighted(q, r) for q, r in zip(qs, heur))
+    wl = sum(weighted(q, r) for q, r in zip(qs, learn))
+    jh = sum(r.join_work for r in heur)
+    jl = sum(r.join_work for r in learn)
     changed = sum(h.order != l.order for h, l in zip(heur, learn))
-    print("join work: heuristic %d, learned %d (%.2fx); intermediate rows %d vs %d; %d of %d plans changed"
-          % (wh, wl, wh / max(wl, 1), rh, rl, changed, len(qs)))
+    print("weighted join work: heuristic %d, learned %d (%.2fx); candidates %d vs %d; %d of %d plans changed"
+          % (wh, wl, wh / max(wl, 1), jh, jl, changed, len(qs)))
     assert all(r.join_work > 0 for r in heur if r.count)
     # the heuristic order is always a candidate and is replaced only when the
     # network predicts a clearly cheaper plan
