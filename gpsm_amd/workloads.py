@@ -6,13 +6,15 @@
   walk queries drawn with 8-16 vertices, half tree-like, half induced.
 * chunglu1m_mix4_32: the same graph with 256 walk queries drawn with 4-32
   vertices (matched with the heuristic or the learned join order).
-* rmat22_64m + rmat_shape_queries: R-MAT scale 22 (a, b, c, d = .57, .19,
-  .19, .05), 64M undirected edges, 32 Zipf(0.8) labels; cycles C4-C8,
-  cliques K4-K8 and induced walks of up to 50 vertices.
+* rmat22_64m + rmat_shape_queries_device: R-MAT scale 22 (a, b, c, d =
+  .57, .19, .19, .05), 64M undirected edges, 32 Zipf(0.8) labels; cycles
+  C4-C8, cliques K4-K8 and induced walks of 8-50 vertices.
 
-Queries pass the feasibility rule of gpsm_gen_queries (an exact upper bound
-on the embedding count, `budget`); slots shrink until it holds, so realised
-sizes are reported with every result.
+The Chung-Lu queries pass the feasibility rule of gpsm_gen_queries (an exact
+upper bound on the embedding count, `budget`; slots shrink until it holds).
+The R-MAT queries pass the device-exact rule of feasible_queries (the join
+finishes within `budget` examined candidates).  Realised sizes are reported
+with every result.
 """
 from . import _lib
 from . import graphs as G
@@ -73,7 +75,24 @@ def rmat22_64m(seed=5, scale=22, m=64_000_000):
     return G.set_labels(g, 32, 0.8, seed)
 
 
+def rmat_shape_queries_device(rg, seed=5, work_budget=EMBEDDING_BUDGET, per_kind=8, per_slot=6):
+    """Cycles C4-C8, cliques K4-K8 and induced walks of 8-50 vertices drawn at
+    their sizes and kept by the device-exact rule (feasible_queries).  On the
+    R-MAT graph the spanning-tree bound of the host rule passes almost no
+    cycle or clique of 4 or more vertices (paths through 10^5-degree hubs),
+    while their joins are small.  Returns (queries, slots that fell back to
+    the host rule)."""
+    qs, fb = [], 0
+    for kind, kmin, kmax, s in ((_lib.Q_CYCLE, 4, 8, seed), (_lib.Q_CLIQUE, 4, 8, seed + 1),
+                                (_lib.Q_WALK_INDUCED, 8, 50, seed + 2)):
+        q, f = feasible_queries(rg, per_kind, kmin, kmax, [kind], s, work_budget, per_slot)
+        qs += q
+        fb += f
+    return qs, fb
+
+
 def rmat_shape_queries(g, seed=5, budget=EMBEDDING_BUDGET, per_kind=8):
+    """The same shapes under the host rule (sizes shrink until the bound holds)."""
     qs = []
     qs += G.queries(g, per_kind, 4, 8, _lib.Q_CYCLE, seed, budget)
     qs += G.queries(g, per_kind, 4, 8, _lib.Q_CLIQUE, seed + 1, budget)
@@ -91,7 +110,7 @@ def build(name, seed=None, budget=EMBEDDING_BUDGET):
         return chunglu1m_mix4_32(2 if seed is None else seed, budget)
     if name == "rmat22_64m":
         g = rmat22_64m(5 if seed is None else seed)
-        return g, rmat_shape_queries(g, 5 if seed is None else seed, budget)
+        return g, None  # queries need the device (rmat_shape_queries_device)
     raise ValueError("unknown workload %r" % name)
 
 
@@ -107,13 +126,17 @@ def run(name, steps=5, autotune=False, plain=True, seed=None, budget=EMBEDDING_B
 
     t0 = time.time()
     g, qs = build(name, seed, budget)
-    t_gen = time.time() - t0
-    log("%s: n=%d, %d undirected edges, %d queries (sizes %s), generated in %.1f s"
-        % (name, g.n, g.adj // 2, len(qs), sorted({q.n for q in qs}), t_gen))
-    out = {"workload": name, "n": g.n, "undirected_edges": g.adj // 2, "queries": len(qs),
-           "query_sizes": {str(k): [q.n for q in qs].count(k) for k in sorted({q.n for q in qs})}}
     with Context(0) as ctx:
         rg = ctx.upload(g)
+        fallbacks = None
+        if qs is None:
+            qs, fallbacks = rmat_shape_queries_device(rg, 5 if seed is None else seed, budget)
+        t_gen = time.time() - t0
+        log("%s: n=%d, %d undirected edges, %d queries (sizes %s), generated in %.1f s"
+            % (name, g.n, g.adj // 2, len(qs), [q.n for q in qs], t_gen))
+        out = {"workload": name, "n": g.n, "undirected_edges": g.adj // 2, "queries": len(qs),
+               "query_sizes": {str(k): [q.n for q in qs].count(k) for k in sorted({q.n for q in qs})},
+               "query_edges": [q.m for q in qs], "host_rule_fallbacks": fallbacks, "generation_s": t_gen}
         out["device"] = ctx.device_name
         if autotune:
             ms, tune = rg.autotune(qs[:min(len(qs), 64)])
