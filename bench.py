@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default tuning (the autotuner otherwise runs before the timed steps)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 1 GiB (reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--traffic-json", default=TRAFFIC_JSON,
@@ -89,6 +91,11 @@ def main():
         mode = _lib.ORDER_LEARNED
         planner_used = True
 
+    tuning = None
+    if not args.no_autotune:
+        # filter block, join buffer rows and join workgroups per CU, chosen on
+        # this query set before the timed region (results checked equal)
+        _, tuning = rg.autotune(qs, order_mode=mode)
     for _ in range(args.warmup):
         rg.match_batch(qs, order_mode=mode)
     # algorithmic bytes per launch: one untimed step with the device work
@@ -197,6 +204,7 @@ def main():
         "config": {"workload": "chunglu-1M-8M-zipf16/%d-walk-queries" % len(qs), "n": g.n, "undirected_edges": g.adj // 2,
                    "queries": len(qs), "query_sizes": {str(k): sizes.count(k) for k in sorted(set(sizes))},
                    "feasibility_budget": args.budget, "order": "learned" if planner_used else "heuristic",
+                   "tuning": tuning,
                    "device": ctx.device_name},
         "embeddings_per_step": emb,
         "embeddings_per_sec": round(emb * args.steps / elapsed),

@@ -79,6 +79,8 @@ SIGNATURES = {
                                     C.POINTER(C.c_void_p)]),
     "gpsm_graph_free": (C.c_int, [C.c_void_p]),
     "gpsm_graph_device_bytes": (C.c_uint64, [C.c_void_p]),
+    "gpsm_set_hub_policy": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64]),
+    "gpsm_graph_hubs": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "gpsm_match": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(QueryS), C.POINTER(Opts), C.POINTER(Result)]),
     "gpsm_match_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(QueryS), C.POINTER(Opts),
                                    C.POINTER(Result)]),
@@ -107,7 +109,9 @@ SIGNATURES = {
     "gpsm_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "gpsm_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(QueryS), C.POINTER(Opts),
                                 C.POINTER(C.c_double)]),
-    "gpsm_tuning": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "gpsm_tuning": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                              C.POINTER(C.c_uint32)]),
+    "gpsm_set_tuning": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "gpsm_profile_read": (C.c_int, [C.c_void_p, C.POINTER(KStat), C.c_uint32, C.POINTER(C.c_uint32)]),
     # generators and text I/O (include/gpsm_gen.h)
     "gpsm_gen_csr_from_edges": (C.c_int, [C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,

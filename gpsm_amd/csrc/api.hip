@@ -88,6 +88,69 @@ hipEvent_t Prof::get() {
 
 }  // namespace gpsm
 
+namespace {
+
+// Hub adjacency bitmaps of a freshly uploaded graph (see DevGraph::hub_of).
+// Without room for them the graph has no hubs and every edge test searches.
+int build_hubs(gpsm_ctx* c, const uint32_t* row_offset, gpsm_graph* g) {
+  DevGraph& d = g->dg;
+  const uint32_t n = d.n;
+  auto deg = [&](uint32_t v) { return row_offset[v + 1] - row_offset[v]; };
+  auto count_over = [&](uint64_t D) {
+    uint64_t k = 0;
+    for (uint32_t v = 0; v < n; ++v) k += deg(v) > D;
+    return k;
+  };
+  uint64_t D = std::max<uint32_t>(c->hub_min_degree, 1);
+  uint64_t nh = c->hub_max_bytes ? count_over(D) : 0;
+  while (nh && nh * d.W * 4ull > c->hub_max_bytes) nh = count_over(D *= 2);
+  std::vector<uint32_t> hub_of(n, ~0u), hubs, block_base(1, 0);
+  if (nh) {
+    for (uint32_t v = 0; v < n; ++v)
+      if (deg(v) > D) {
+        hub_of[v] = (uint32_t)hubs.size();
+        hubs.push_back(v);
+        block_base.push_back(block_base.back() + (deg(v) + 63) / 64);  // 64-entry blocks, one wave each
+      }
+  }
+  if (hipMalloc(&d.hub_of, sizeof(uint32_t) * n) != hipSuccess) {
+    (void)hipGetLastError();
+    return GPSM_ENOMEM;
+  }
+  uint32_t* tmp = nullptr;
+  if (!hubs.empty()) {
+    if (hipMalloc(&d.hub_bits, sizeof(uint32_t) * (size_t)hubs.size() * d.W) != hipSuccess ||
+        hipMalloc(&tmp, sizeof(uint32_t) * (hubs.size() + block_base.size())) != hipSuccess) {
+      // no room: no hubs
+      (void)hipGetLastError();
+      hipFree(d.hub_bits);
+      d.hub_bits = nullptr;
+      hubs.clear();
+      std::fill(hub_of.begin(), hub_of.end(), ~0u);
+    }
+  }
+  d.nhub = (uint32_t)hubs.size();
+  d.hub_degree = d.nhub ? (uint32_t)D : 0;
+  hipStream_t s = c->stream;
+  if (hipMemcpyAsync(d.hub_of, hub_of.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s) != hipSuccess)
+    return GPSM_EDEVICE;
+  if (d.nhub) {
+    launch_fill_u32(d.hub_bits, 0u, (uint64_t)d.nhub * d.W, s);
+    if (hipMemcpyAsync(tmp, hubs.data(), sizeof(uint32_t) * hubs.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(tmp + hubs.size(), block_base.data(), sizeof(uint32_t) * block_base.size(),
+                       hipMemcpyHostToDevice, s) != hipSuccess)
+      return GPSM_EDEVICE;
+    launch_hub_bits(d, tmp, tmp + hubs.size(), block_base.back(), s);
+  }
+  const bool ok = hipStreamSynchronize(s) == hipSuccess && hipGetLastError() == hipSuccess;
+  hipFree(tmp);
+  if (!ok) return GPSM_EDEVICE;
+  g->bytes += sizeof(uint32_t) * ((uint64_t)n + (uint64_t)d.nhub * d.W);
+  return GPSM_OK;
+}
+
+}  // namespace
+
 // ---------------------------------------------------------------------------
 
 extern "C" {
@@ -240,6 +303,8 @@ int gpsm_graph_upload(gpsm_ctx* c, uint32_t n, const uint32_t* row_offset, const
     hipFree(d.col);
     hipFree(d.key);
     hipFree(d.fchunk);
+    hipFree(d.hub_of);
+    hipFree(d.hub_bits);
     delete g;
     return code;
   };
@@ -261,6 +326,7 @@ int gpsm_graph_upload(gpsm_ctx* c, uint32_t n, const uint32_t* row_offset, const
     return fail(GPSM_EDEVICE);
   }
   g->bytes = sizeof(uint32_t) * ((uint64_t)n + 1 + A + n + fch.size());
+  if (int rc = build_hubs(c, row_offset, g)) return fail(rc);
   *out = g;
   return GPSM_OK;
 }
@@ -273,11 +339,27 @@ int gpsm_graph_free(gpsm_graph* g) {
   hipFree(g->dg.col);
   hipFree(g->dg.key);
   hipFree(g->dg.fchunk);
+  hipFree(g->dg.hub_of);
+  hipFree(g->dg.hub_bits);
   delete g;
   return GPSM_OK;
 }
 
 uint64_t gpsm_graph_device_bytes(const gpsm_graph* g) { return g ? g->bytes : 0; }
+
+int gpsm_set_hub_policy(gpsm_ctx* c, uint32_t min_degree, uint64_t max_bytes) {
+  if (!c || !min_degree) return GPSM_EINVAL;
+  c->hub_min_degree = min_degree;
+  c->hub_max_bytes = max_bytes;
+  return GPSM_OK;
+}
+
+int gpsm_graph_hubs(const gpsm_graph* g, uint32_t* nhub, uint32_t* hub_degree) {
+  if (!g) return GPSM_EINVAL;
+  if (nhub) *nhub = g->dg.nhub;
+  if (hub_degree) *hub_degree = g->dg.hub_degree;
+  return GPSM_OK;
+}
 
 void gpsm_result_free(gpsm_result* r) {
   if (!r) return;
@@ -627,6 +709,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   // whose few first rows fan out to most of the batch's work does not leave
   // a handful of long tasks running after the rest of the grid is idle.
   const uint32_t nwg = std::max<uint32_t>(1, c->ncu * wg_per_cu);
+  const uint64_t tpw = std::max<uint32_t>(1, c->tune.tasks_per_wg);
   uint64_t rows_total = 0;
   double est_total = 0;
   for (uint32_t b = 0; b < nq; ++b)
@@ -635,8 +718,8 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       est_total += est[b];
     }
   const uint64_t tsize =
-      std::min<uint64_t>(std::max<uint64_t>(rows_total / ((uint64_t)nwg * 8) + 1, c->tune.task_min), 1u << 20);
-  const double target = est_total / ((double)nwg * 8.0);
+      std::min<uint64_t>(std::max<uint64_t>(rows_total / ((uint64_t)nwg * tpw) + 1, c->tune.task_min), 1u << 20);
+  const double target = est_total / ((double)nwg * tpw);
   for (uint32_t b = 0; b < nq; ++b) {
     if (plan_of_q[b] == ~0u) continue;
     const uint64_t R1 = Q[b].cet_size[Q[b].plan.first_edge];
@@ -683,6 +766,8 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     JoinArgs ja{};
     ja.row = dg.row;
     ja.col = dg.col;
+    ja.hub_of = dg.hub_of;
+    ja.hub_bits = dg.hub_bits;
     ja.bitmaps = bitmaps;
     ja.bprefix = dptr<uint32_t>(c, B_BPREFIX);
     ja.clist = dptr<uint32_t>(c, B_CLIST);
@@ -766,7 +851,12 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       for (uint32_t k = 1; k < Q[b].hq.n && k < GPSM_MAX_QV; ++k) out[b].level_rows[k] = r.level_rows[k];
       out[b].join_work = r.work;
       for (uint32_t k = 1; k < Q[b].hq.n && k < GPSM_MAX_QV; ++k) out[b].level_work[k] = r.level_work[k];
-      if (r.aborted) out[b].status = GPSM_EBUDGET;
+      if (r.aborted) {
+        // abandoned part-way: no partial count or hash
+        out[b].status = GPSM_EBUDGET;
+        out[b].count = 0;
+        out[b].hash = 0;
+      }
     }
     if (emit_cap && nq == 1 && plan_of_q[0] != ~0u) {
       const uint32_t qn = Q[0].hq.n;
@@ -821,11 +911,23 @@ int gpsm_match_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_q
 
 extern "C" {
 
-int gpsm_tuning(const gpsm_ctx* c, uint32_t* fb, uint32_t* cap, uint32_t* wg) {
+int gpsm_tuning(const gpsm_ctx* c, uint32_t* fb, uint32_t* cap, uint32_t* wg, uint32_t* tpw) {
   if (!c) return GPSM_EINVAL;
   if (fb) *fb = c->tune.filter_block;
   if (cap) *cap = c->tune.join_cap;
   if (wg) *wg = c->tune.join_wg_per_cu;
+  if (tpw) *tpw = c->tune.tasks_per_wg;
+  return GPSM_OK;
+}
+
+int gpsm_set_tuning(gpsm_ctx* c, uint32_t fb, uint32_t cap, uint32_t wg, uint32_t tpw) {
+  if (!c || (fb != 64 && fb != 128 && fb != 256) || cap < 2048 || cap > (1u << 20) || wg < 1 || wg > 16 ||
+      tpw < 1 || tpw > 256)
+    return GPSM_EINVAL;
+  c->tune.filter_block = fb;
+  c->tune.join_cap = cap;
+  c->tune.join_wg_per_cu = wg;
+  c->tune.tasks_per_wg = tpw;
   return GPSM_OK;
 }
 
@@ -856,14 +958,15 @@ int gpsm_autotune(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_quer
   double cur;
   int rc = trial(cur, false);
   if (rc) return rc;
-  const uint32_t blocks[] = {64, 128, 256}, caps[] = {2048, 4096, 8192}, wgs[] = {2, 4, 8};
-  for (int param = 0; param < 3; ++param) {
+  const uint32_t values[4][4] = {{64, 128, 256, 0}, {2048, 4096, 8192, 0}, {2, 4, 8, 0}, {4, 8, 16, 32}};
+  for (int param = 0; param < 4; ++param) {
     Tuning best = c->tune;
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < 4 && values[param][i]; ++i) {
       Tuning t = best;
-      if (param == 0) t.filter_block = blocks[i];
-      if (param == 1) t.join_cap = caps[i];
-      if (param == 2) t.join_wg_per_cu = wgs[i];
+      if (param == 0) t.filter_block = values[param][i];
+      if (param == 1) t.join_cap = values[param][i];
+      if (param == 2) t.join_wg_per_cu = values[param][i];
+      if (param == 3) t.tasks_per_wg = values[param][i];
       Tuning keep = c->tune;
       c->tune = t;
       double ms;

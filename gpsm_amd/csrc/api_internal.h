@@ -53,6 +53,7 @@ struct Tuning {
   uint32_t join_cap = 4096;      // rows per join level buffer
   uint32_t join_wg_per_cu = 4;   // persistent join workgroups per CU
   uint32_t task_min = 256;       // minimum first-table rows per join task
+  uint32_t tasks_per_wg = 8;     // join tasks per persistent workgroup over a batch
 };
 
 struct BatchHooks {
@@ -110,6 +111,10 @@ struct gpsm_ctx {
   gpsm::Tuning tune;
   gpsm_mlp* planner = nullptr;
   std::vector<uint32_t> h_counts;
+  // hub adjacency bitmaps of later uploads: vertices of degree > the hub
+  // degree, starting at hub_min_degree and doubled until they fit hub_max_bytes
+  uint32_t hub_min_degree = 512;
+  uint64_t hub_max_bytes = 8ull << 30;
 };
 
 struct gpsm_graph {

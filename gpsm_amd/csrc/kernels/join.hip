@@ -37,8 +37,17 @@ __device__ __forceinline__ uint32_t rank_of(const uint32_t* __restrict__ bm, con
   return r + __popc(bm[wx] & ((1u << (x & 31)) - 1u));
 }
 
+// Edge {a, b}: one bit of a hub's adjacency bitmap when either end is a hub,
+// else a binary search in the shorter of the two (short) adjacency lists.
 __device__ __forceinline__ bool has_edge(const uint32_t* __restrict__ row, const uint32_t* __restrict__ col,
-                                         uint32_t a, uint32_t b, unsigned long long& bytes) {
+                                         const uint32_t* __restrict__ hub_of, const uint32_t* __restrict__ hub_bits,
+                                         uint32_t W, uint32_t a, uint32_t b, unsigned long long& bytes) {
+  const uint32_t ha = hub_of[a], hb = hub_of[b];
+  bytes += 8;
+  if (ha != ~0u || hb != ~0u) {
+    bytes += 4;
+    return ha != ~0u ? bit_of(hub_bits + (size_t)ha * W, b) != 0 : bit_of(hub_bits + (size_t)hb * W, a) != 0;
+  }
   uint32_t lo = row[a], hi = row[a + 1];
   const uint32_t lo2 = row[b], hi2 = row[b + 1];
   uint32_t key = b;
@@ -110,8 +119,17 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
       S.abort = a.budget ? (uint32_t)__hip_atomic_load(&a.out[task.query].aborted, __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_AGENT)
                          : 0u;
-      if (S.abort) S.fill[1] = S.lwork[1] = 0;
-      else atomicAdd(&a.out[task.query].work, (unsigned long long)task.count);  // first-table rows
+      if (S.abort) {
+        S.fill[1] = S.lwork[1] = 0;
+      } else {
+        // first-table rows are join work too (the whole join of a 2-vertex query)
+        const unsigned long long before = atomicAdd(&a.out[task.query].work, (unsigned long long)task.count);
+        if (a.budget && before + task.count > a.budget) {
+          a.out[task.query].aborted = 1;
+          S.abort = 1;
+          S.fill[1] = 0;
+        }
+      }
     }
     __syncthreads();
 
@@ -224,7 +242,7 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
               my_bytes += 4;
               for (uint32_t c = 0; c <= k && ok; ++c) ok = val[c * WROWS + i] != mv;
               for (uint32_t b = bo; b < be && ok; ++b)
-                ok = has_edge(a.row, a.col, mv, val[P.backs[b] * WROWS + i], my_bytes);
+                ok = has_edge(a.row, a.col, a.hub_of, a.hub_bits, a.W, mv, val[P.backs[b] * WROWS + i], my_bytes);
             }
             if (final) {
               if (ok) {

@@ -23,6 +23,13 @@ struct DevGraph {
   // walks far more adjacency than the others (power-law hubs)
   uint32_t* fchunk = nullptr;
   uint32_t nfchunk = 0;
+  // hubs (degree above hub_degree): hub_of[v] = index of v's adjacency
+  // bitmap (W words each, in hub_bits) or ~0u, so an edge test touching a
+  // hub is one bit; other edge tests search the shorter adjacency list
+  uint32_t* hub_of = nullptr;
+  uint32_t* hub_bits = nullptr;
+  uint32_t nhub = 0;
+  uint32_t hub_degree = 0;
 };
 
 constexpr uint32_t FCHUNK_VERTS = 1024;
@@ -30,6 +37,12 @@ constexpr uint32_t FCHUNK_ENTRIES = 4096;
 
 // Boundaries of the filter's work items over a CSR (host side).
 void filter_chunks(uint32_t n, const uint32_t* row_offset, std::vector<uint32_t>& bounds);
+
+// Sets the adjacency bitmaps of the hubs: hub_bits (nhub * W words) must be
+// zero; hubs[h] is the vertex of bitmap h; block_base[h] = 64-entry blocks
+// of the adjacency of hubs before h (nhub + 1 entries, last = nblocks).
+void launch_hub_bits(const DevGraph& g, const uint32_t* hubs, const uint32_t* block_base, uint32_t nblocks,
+                     hipStream_t s);
 
 // filter.hip ------------------------------------------------------------------
 // Start-vertex candidates: C(slot) = {v : lo <= key[v] <= hi} (check_kernel).
@@ -77,6 +90,8 @@ void launch_cet_write(const DevGraph& g, const uint32_t* bitmaps, const uint32_t
 struct JoinArgs {
   const uint32_t* row;
   const uint32_t* col;
+  const uint32_t* hub_of;    // DevGraph::hub_of / hub_bits
+  const uint32_t* hub_bits;
   const uint32_t* bitmaps;
   const uint32_t* bprefix;
   const uint32_t* clist;

@@ -98,6 +98,21 @@ class Context:
         return {arr[i].name.decode(): {"launches": int(arr[i].launches), "ms": float(arr[i].total_ms),
                                        "bytes": float(arr[i].bytes)} for i in range(n.value)}
 
+    def set_hub_policy(self, min_degree=512, max_bytes=8 << 30):
+        """Hub adjacency bitmaps for graphs uploaded afterwards (vertices of
+        degree > D, D doubled from min_degree until they fit max_bytes)."""
+        check(_lib.lib().gpsm_set_hub_policy(self.h, min_degree, max_bytes), "gpsm_set_hub_policy")
+
+    def tuning(self):
+        """The context's launch tuning (defaults until gpsm_autotune runs)."""
+        v = [C.c_uint32() for _ in range(4)]
+        check(_lib.lib().gpsm_tuning(self.h, *[C.byref(x) for x in v]), "gpsm_tuning")
+        return dict(zip(["filter_block", "join_cap", "join_wg_per_cu", "tasks_per_wg"], [x.value for x in v]))
+
+    def set_tuning(self, filter_block, join_cap, join_wg_per_cu, tasks_per_wg):
+        check(_lib.lib().gpsm_set_tuning(self.h, filter_block, join_cap, join_wg_per_cu, tasks_per_wg),
+              "gpsm_set_tuning")
+
     # --- parity entry points -------------------------------------------------
     def scan(self, a):
         a = u32(a)
@@ -188,6 +203,16 @@ class ResidentGraph:
             raise _lib.GpsmError(n, "gpsm_plan_features")
         return feat[:n], tgt[:n]
 
+    def hubs(self):
+        """(number of hubs, degree above which a vertex is one)."""
+        nh, d = C.c_uint32(), C.c_uint32()
+        check(_lib.lib().gpsm_graph_hubs(self.h, C.byref(nh), C.byref(d)), "gpsm_graph_hubs")
+        return nh.value, d.value
+
+    @property
+    def device_bytes(self):
+        return int(_lib.lib().gpsm_graph_device_bytes(self.h))
+
     def cet_edges(self, q):
         """Query edges (from < to, the planner's edge ids) and |CET| of each
         (gpsm_candidate_edges without the lists)."""
@@ -212,9 +237,7 @@ class ResidentGraph:
         ms = C.c_double(0)
         check(_lib.lib().gpsm_autotune(self.ctx.h, self.h, len(queries), arr, C.byref(o), C.byref(ms)),
               "gpsm_autotune")
-        fb, cap, wg = C.c_uint32(), C.c_uint32(), C.c_uint32()
-        check(_lib.lib().gpsm_tuning(self.ctx.h, C.byref(fb), C.byref(cap), C.byref(wg)), "gpsm_tuning")
-        return ms.value, {"filter_block": fb.value, "join_cap": cap.value, "join_wg_per_cu": wg.value}
+        return ms.value, self.ctx.tuning()
 
     def filter(self, q):
         W = (self.g.n + 31) // 32

@@ -22,38 +22,37 @@ from . import graphs as G
 EMBEDDING_BUDGET = 10 ** 8
 
 
-def feasible_queries(rg, nq, kmin, kmax, kinds, seed, work_budget, per_slot=6):
+def feasible_queries(rg, nq, kmin, kmax, kinds, seed, work_budget, per_slot=6, rounds=4, log=None):
     """Queries that pass an exact device-measured feasibility rule.
 
     For output slot i, per_slot candidates of kind kinds[i % len(kinds)] are
-    drawn (sizes uniform in [kmin, kmax], no host-side rule), matched in one
-    batch with a join work budget, and the first candidate whose join finishes
-    within the budget is kept.  A slot whose candidates all exceed it keeps its
-    smallest-work candidate's host-rule replacement (gpsm_gen_queries with the
-    same budget as an embedding bound).  Every decision is an exact property
-    of the query, so the set does not depend on the device or thread count.
-    Returns (queries, number of slots that needed the host fallback)."""
-    out, fallbacks = [], 0
-    by_kind = {}
-    for ki, kind in enumerate(kinds):
-        slots = list(range(ki, nq, len(kinds)))
-        cands = G.queries(rg.g, len(slots) * per_slot, kmin, kmax, kind, seed + 7919 * ki, 0)
-        res = rg.match_batch(cands, work_budget=work_budget)
-        for si, slot in enumerate(slots):
-            pick = None
-            for j in range(per_slot):
-                if res[si * per_slot + j].status == 0:
-                    pick = cands[si * per_slot + j]
-                    break
-            by_kind[slot] = pick
-    for slot in range(nq):
-        q = by_kind[slot]
-        if q is None:
-            fallbacks += 1
-            kind = kinds[slot % len(kinds)]
-            q = G.queries(rg.g, 1, kmin, kmax, kind, seed + 104729 * (slot + 1), work_budget)[0]
-        out.append(q)
-    return out, fallbacks
+    drawn (sizes uniform in the round's range, no host-side rule), matched in
+    one batch with a join work budget, and the first candidate whose join
+    finishes within the budget is kept.  Slots left empty are drawn again in
+    the next round with the upper size bound halved towards kmin; after
+    `rounds` rounds a slot that is still empty is dropped.  Every decision is
+    an exact property of the query, so the set does not depend on the device
+    or thread count.  Returns (queries, number of dropped slots)."""
+    picks = [None] * nq
+    hi = kmax
+    for r in range(rounds):
+        for ki, kind in enumerate(kinds):
+            slots = [s for s in range(ki, nq, len(kinds)) if picks[s] is None]
+            if not slots:
+                continue
+            cands = G.queries(rg.g, len(slots) * per_slot, kmin, hi, kind, seed + 7919 * ki + 104729 * r, 0)
+            res = rg.match_batch(cands, work_budget=work_budget)
+            for si, slot in enumerate(slots):
+                for j in range(per_slot):
+                    if res[si * per_slot + j].status == 0:
+                        picks[slot] = cands[si * per_slot + j]
+                        break
+            if log:
+                log("  feasibility round %d, kind %d, sizes %d-%d: %d of %d slots filled"
+                    % (r, kind, kmin, hi, sum(picks[s] is not None for s in slots), len(slots)))
+        hi = max(kmin, (kmin + hi) // 2)
+    out = [q for q in picks if q is not None]
+    return out, nq - len(out)
 
 
 def er10k_walk6(seed=1):
@@ -75,20 +74,20 @@ def rmat22_64m(seed=5, scale=22, m=64_000_000):
     return G.set_labels(g, 32, 0.8, seed)
 
 
-def rmat_shape_queries_device(rg, seed=5, work_budget=EMBEDDING_BUDGET, per_kind=8, per_slot=6):
+def rmat_shape_queries_device(rg, seed=5, work_budget=EMBEDDING_BUDGET, per_kind=8, per_slot=6, log=None):
     """Cycles C4-C8, cliques K4-K8 and induced walks of 8-50 vertices drawn at
     their sizes and kept by the device-exact rule (feasible_queries).  On the
     R-MAT graph the spanning-tree bound of the host rule passes almost no
     cycle or clique of 4 or more vertices (paths through 10^5-degree hubs),
-    while their joins are small.  Returns (queries, slots that fell back to
-    the host rule)."""
-    qs, fb = [], 0
+    and evaluating it for one 8-50-vertex walk takes minutes of host time.
+    Returns (queries, dropped slots)."""
+    qs, dropped = [], 0
     for kind, kmin, kmax, s in ((_lib.Q_CYCLE, 4, 8, seed), (_lib.Q_CLIQUE, 4, 8, seed + 1),
                                 (_lib.Q_WALK_INDUCED, 8, 50, seed + 2)):
-        q, f = feasible_queries(rg, per_kind, kmin, kmax, [kind], s, work_budget, per_slot)
+        q, d = feasible_queries(rg, per_kind, kmin, kmax, [kind], s, work_budget, per_slot, log=log)
         qs += q
-        fb += f
-    return qs, fb
+        dropped += d
+    return qs, dropped
 
 
 def rmat_shape_queries(g, seed=5, budget=EMBEDDING_BUDGET, per_kind=8):
@@ -126,17 +125,19 @@ def run(name, steps=5, autotune=False, plain=True, seed=None, budget=EMBEDDING_B
 
     t0 = time.time()
     g, qs = build(name, seed, budget)
+    log("%s: graph of %d vertices and %d undirected edges built in %.1f s" % (name, g.n, g.adj // 2, time.time() - t0))
     with Context(0) as ctx:
         rg = ctx.upload(g)
-        fallbacks = None
+        log("graph resident on %s" % ctx.device_name)
+        dropped = None
         if qs is None:
-            qs, fallbacks = rmat_shape_queries_device(rg, 5 if seed is None else seed, budget)
+            qs, dropped = rmat_shape_queries_device(rg, 5 if seed is None else seed, budget, log=log)
         t_gen = time.time() - t0
         log("%s: n=%d, %d undirected edges, %d queries (sizes %s), generated in %.1f s"
             % (name, g.n, g.adj // 2, len(qs), [q.n for q in qs], t_gen))
         out = {"workload": name, "n": g.n, "undirected_edges": g.adj // 2, "queries": len(qs),
                "query_sizes": {str(k): [q.n for q in qs].count(k) for k in sorted({q.n for q in qs})},
-               "query_edges": [q.m for q in qs], "host_rule_fallbacks": fallbacks, "generation_s": t_gen}
+               "query_edges": [q.m for q in qs], "dropped_slots": dropped, "generation_s": t_gen}
         out["device"] = ctx.device_name
         if autotune:
             ms, tune = rg.autotune(qs[:min(len(qs), 64)])

@@ -114,6 +114,16 @@ int gpsm_graph_upload(gpsm_ctx* ctx, uint32_t n, const uint32_t* row_offset,
 int gpsm_graph_free(gpsm_graph* g);
 /* Bytes of device memory the resident graph occupies. */
 uint64_t gpsm_graph_device_bytes(const gpsm_graph* g);
+/* Hub adjacency bitmaps for graphs uploaded after this call (no counterpart
+ * in the reference, whose join scans candidate lists linearly): every vertex
+ * of degree > D gets an n-bit adjacency bitmap in device memory, so a
+ * join-mode edge test touching it is one bit instead of a binary search.
+ * D starts at min_degree (default 512) and doubles until the bitmaps fit
+ * max_bytes (default 8 GiB; 0 = no hubs).  Results do not depend on it. */
+int gpsm_set_hub_policy(gpsm_ctx* ctx, uint32_t min_degree, uint64_t max_bytes);
+/* The hubs of an uploaded graph: their number and the degree D above which
+ * a vertex is one (0 when there are none). */
+int gpsm_graph_hubs(const gpsm_graph* g, uint32_t* nhub, uint32_t* hub_degree);
 
 /* --- matching ----------------------------------------------------------- */
 
@@ -135,14 +145,23 @@ void gpsm_result_free(gpsm_result* r);
 /* Autotuner (no reference counterpart; the reference hard-codes block sizes
  * 1024/256/32, match/Match.cu:455-1618): times gpsm_match_batch on the given
  * sample for every candidate filter workgroup size (64/128/256), join level
- * buffer (2048/4096/8192 rows) and persistent join workgroups per CU
- * (2/4/8), one parameter at a time, and keeps the fastest in the context.
+ * buffer (2048/4096/8192 rows), persistent join workgroups per CU (2/4/8)
+ * and join tasks per workgroup (4/8/16/32), one parameter at a time, and
+ * keeps the fastest in the context.
  * Results of every trial are checked to agree.  *best_ms receives the time
  * of the chosen setting.  Later calls with opts->block / join_cap == 0 use it. */
 int gpsm_autotune(gpsm_ctx* ctx, const gpsm_graph* g, uint32_t nq, const gpsm_query* qs,
                   const gpsm_opts* opts, double* best_ms);
-/* The tuned values: filter workgroup size, join buffer rows, join workgroups per CU. */
-int gpsm_tuning(const gpsm_ctx* ctx, uint32_t* filter_block, uint32_t* join_cap, uint32_t* join_wg_per_cu);
+/* The tuned values: filter workgroup size, join buffer rows, join workgroups
+ * per CU, join tasks per workgroup (any pointer may be NULL). */
+int gpsm_tuning(const gpsm_ctx* ctx, uint32_t* filter_block, uint32_t* join_cap, uint32_t* join_wg_per_cu,
+                uint32_t* tasks_per_wg);
+/* Sets the same values, e.g. a tuning saved from an earlier gpsm_autotune
+ * (filter_block 64/128/256, join_cap 2048..2^20, join_wg_per_cu 1..16,
+ * tasks_per_wg 1..256; GPSM_EINVAL otherwise).  No counterpart in the
+ * reference, whose launch shapes are constants. */
+int gpsm_set_tuning(gpsm_ctx* ctx, uint32_t filter_block, uint32_t join_cap, uint32_t join_wg_per_cu,
+                    uint32_t tasks_per_wg);
 
 /* --- pipeline stages (parity entry points) ------------------------------ */
 

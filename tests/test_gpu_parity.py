@@ -232,3 +232,63 @@ def test_profile_modes_keep_results_and_count_bytes(ctx):
     for k in ("filter_step", "join", "cet_count", "cet_write"):
         assert stats[1][k]["launches"] == stats[2][k]["launches"] > 0 and stats[1][k]["ms"] > 0
     assert stats[2]["filter_step"]["bytes"] > stats[1]["filter_step"]["bytes"] > 0
+
+
+def test_work_budget_is_exact(ctx):
+    """A budget equal to a query's join work lets it finish with the same
+    result; one less abandons it (GPSM_EBUDGET, no partial count), alone
+    and next to other queries in a batch."""
+    g = G.small_graph(14, n=3000, m=15000, labels=2, kind="cl")
+    qs = [q for q in G.queries(g, 12, 4, 6, _lib.Q_WALK_HALF, 23, 10 ** 7) if q.n > 2]
+    rg = ctx.upload(g)
+    base = rg.match_batch(qs)
+    i = max(range(len(qs)), key=lambda j: base[j].join_work)
+    q, b = qs[i], base[i]
+    assert b.join_work > 1000
+    ok = rg.match(q, work_budget=b.join_work)
+    assert (ok.status, ok.count, ok.hash, ok.join_work) == (0, b.count, b.hash, b.join_work)
+    with pytest.raises(_lib.GpsmError) as err:  # the single-query call returns the status
+        rg.match(q, work_budget=b.join_work - 1)
+    assert err.value.code == _lib.EBUDGET
+    budgets = [r.join_work for r in base]
+    res = rg.match_batch(qs, work_budget=b.join_work - 1)
+    for r, r0, w in zip(res, base, budgets):
+        if w <= b.join_work - 1:
+            assert (r.status, r.count, r.hash) == (0, r0.count, r0.hash)
+        else:
+            assert (r.status, r.count, r.hash) == (_lib.EBUDGET, 0, 0)
+    assert res[i].status == _lib.EBUDGET
+
+
+def test_hub_bitmaps_give_the_same_results(ctx):
+    """The join's edge tests through hub adjacency bitmaps (here every vertex
+    of degree > 8 is a hub) give the same counts, hashes and work as binary
+    searches only, and as the enumerator, on induced queries (many back
+    edges) over a power-law graph."""
+    g = G.small_graph(15, n=3000, m=24000, labels=2, kind="cl")
+    qs = G.queries(g, 16, 4, 7, _lib.Q_WALK_INDUCED, 29, 10 ** 7)
+    try:
+        ctx.set_hub_policy(8, 1 << 30)
+        rh = ctx.upload(g)
+        ctx.set_hub_policy(512, 0)
+        rn = ctx.upload(g)
+    finally:
+        ctx.set_hub_policy()  # defaults
+    nh, d = rh.hubs()
+    assert nh > 100 and d == 8 and rn.hubs() == (0, 0)
+    assert rh.device_bytes > rn.device_bytes
+    a = rh.match_batch(qs)
+    b = rn.match_batch(qs)
+    for q, x, y in zip(qs, a, b):
+        r1 = O.o1_enumerate(g, q)
+        assert (x.count, x.hash) == (y.count, y.hash) == (r1["count"], r1["hash"])
+        assert x.level_work == y.level_work and x.level_rows == y.level_rows
+    assert sum(x.count for x in a) > 0
+    # the doubling rule: a cap below the bitmaps of degree > 8 raises the degree
+    try:
+        ctx.set_hub_policy(8, nh * ((g.n + 31) // 32) * 4 - 1)
+        rc = ctx.upload(g)
+    finally:
+        ctx.set_hub_policy()
+    nh2, d2 = rc.hubs()
+    assert d2 >= 16 and nh2 < nh and nh2 * ((g.n + 31) // 32) * 4 <= nh * ((g.n + 31) // 32) * 4 - 1

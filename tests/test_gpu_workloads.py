@@ -45,9 +45,19 @@ def test_autotuner_keeps_results(ctx, mix):
     rg = ctx.upload(g)
     before = rg.match_batch(qs)
     ms, tune = rg.autotune(qs[:32])
-    assert ms > 0 and tune["filter_block"] in (64, 128, 256)
+    assert ms > 0 and tune["filter_block"] in (64, 128, 256) and tune["tasks_per_wg"] in (4, 8, 16, 32)
     after = rg.match_batch(qs)
     assert [(r.count, r.hash) for r in before] == [(r.count, r.hash) for r in after]
+    # a saved tuning is set back exactly; out-of-range values are refused
+    ctx.set_tuning(128, 2048, 2, 32)
+    assert ctx.tuning() == {"filter_block": 128, "join_cap": 2048, "join_wg_per_cu": 2, "tasks_per_wg": 32}
+    with pytest.raises(_lib.GpsmError):
+        ctx.set_tuning(100, 4096, 4, 8)
+    with pytest.raises(_lib.GpsmError):
+        ctx.set_tuning(256, 1024, 4, 8)
+    small = rg.match_batch(qs)
+    assert [(r.count, r.hash) for r in small] == [(r.count, r.hash) for r in before]
+    ctx.set_tuning(256, 4096, 4, 8)  # the defaults, for the tests after this one
 
 
 @pytest.mark.skipif(not os.path.exists(planner.WEIGHTS), reason="no trained planner weights")
