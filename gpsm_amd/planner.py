@@ -82,5 +82,7 @@ def install(ctx, path=WEIGHTS, norm_path=NORM):
     norm = np.load(norm_path, allow_pickle=False)
     m.set_norm(norm[:_lib.FEAT], norm[_lib.FEAT:])
     check(_lib.lib().gpsm_set_planner(ctx.h, m.h), "gpsm_set_planner")
+    if ctx.planner is not None:  # replaced: release the previous network
+        ctx.planner.free()
     ctx.planner = m
     return m
