@@ -78,6 +78,10 @@ SIGNATURES = {
     "gpsm_graph_upload": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.POINTER(C.c_void_p)]),
     "gpsm_graph_free": (C.c_int, [C.c_void_p]),
+    "gpsm_graph_upload_edges": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.POINTER(C.c_void_p)]),
+    "gpsm_graph_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
+    "gpsm_graph_csr": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "gpsm_graph_device_bytes": (C.c_uint64, [C.c_void_p]),
     "gpsm_set_hub_policy": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64]),
     "gpsm_graph_hubs": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
@@ -130,6 +134,9 @@ SIGNATURES = {
     "gpsm_io_write_end": (C.c_int, [C.c_char_p]),
     "gpsm_io_count_graphs": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
                                        C.POINTER(C.c_uint64)]),
+    "gpsm_io_write_csr": (C.c_int, [C.c_char_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gpsm_io_map_csr": (C.c_int, [C.c_char_p, C.POINTER(CSR)]),
+    "gpsm_io_unmap_csr": (None, [C.POINTER(CSR)]),
 }
 
 

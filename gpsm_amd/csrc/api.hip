@@ -250,6 +250,76 @@ void gpsm_shutdown(gpsm_ctx* c) {
 
 const char* gpsm_device_name(const gpsm_ctx* c) { return c ? c->name : ""; }
 
+}  // extern "C"
+
+namespace {
+
+void free_graph_arrays(DevGraph& d) {
+  hipFree(d.row);
+  hipFree(d.col);
+  hipFree(d.key);
+  hipFree(d.fchunk);
+  hipFree(d.hub_of);
+  hipFree(d.hub_bits);
+  d = DevGraph();
+}
+
+// The rest of a resident graph once row/col are on the device: label
+// statistics, the label/degree keys, the filter's vertex ranges and the hub
+// bitmaps.  row_offset and labels are host copies.  On failure g is freed.
+int finish_graph(gpsm_ctx* c, gpsm_graph* g, const uint32_t* row_offset, const uint32_t* labels) {
+  DevGraph& d = g->dg;
+  const uint32_t n = d.n;
+  auto fail = [&](int code) {
+    free_graph_arrays(d);
+    delete g;
+    return code;
+  };
+  GraphStats& s = g->stats;
+  s.n = n;
+  s.adj = d.adj;
+  std::vector<uint32_t> lv(labels, labels + n);
+  std::sort(lv.begin(), lv.end());
+  if (lv[0] == 0) return fail(GPSM_EINVAL);
+  for (uint32_t i = 0; i < n; ++i) {
+    if (i == 0 || lv[i] != lv[i - 1]) {
+      s.label_value.push_back(lv[i]);
+      s.label_count.push_back(0);
+    }
+    s.label_count.back()++;
+  }
+  if (s.label_value.size() > MAX_LABELS) return fail(GPSM_EINVAL);
+  std::vector<uint32_t> key(n);
+  for (uint32_t v = 0; v < n; ++v) {
+    uint32_t deg = row_offset[v + 1] - row_offset[v];
+    key[v] = (s.dense(labels[v]) << KEY_DEG_BITS) | std::min(deg, KEY_DEG_MASK);
+  }
+  d.W = words_of(n);
+  d.NB = nblocks_of(n);
+  std::vector<uint32_t> fch;
+  filter_chunks(n, row_offset, fch);
+  d.nfchunk = (uint32_t)fch.size() - 1;
+  if (hipMalloc(&d.key, sizeof(uint32_t) * n) != hipSuccess ||
+      hipMalloc(&d.fchunk, sizeof(uint32_t) * fch.size()) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(GPSM_ENOMEM);
+  }
+  if (hipMemcpyAsync(d.key, key.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(d.fchunk, fch.data(), sizeof(uint32_t) * fch.size(), hipMemcpyHostToDevice, c->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(GPSM_EDEVICE);
+  }
+  g->bytes = sizeof(uint32_t) * ((uint64_t)n + 1 + d.adj + n + fch.size());
+  if (int rc = build_hubs(c, row_offset, g)) return fail(rc);
+  return GPSM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int gpsm_graph_upload(gpsm_ctx* c, uint32_t n, const uint32_t* row_offset, const uint32_t* col,
                       const uint32_t* labels, gpsm_graph** out) {
   if (!c || !out || !row_offset || !labels || n == 0) return GPSM_EINVAL;
@@ -266,68 +336,82 @@ int gpsm_graph_upload(gpsm_ctx* c, uint32_t n, const uint32_t* row_offset, const
     }
     if (labels[v] == 0) return GPSM_EINVAL;
   }
+  hipSetDevice(c->device);
   gpsm_graph* g = new gpsm_graph();
   g->ctx = c;
-  GraphStats& s = g->stats;
-  s.n = n;
-  s.adj = A;
-  std::vector<uint32_t> lv(labels, labels + n);
-  std::sort(lv.begin(), lv.end());
-  for (uint32_t i = 0; i < n; ++i) {
-    if (i == 0 || lv[i] != lv[i - 1]) {
-      s.label_value.push_back(lv[i]);
-      s.label_count.push_back(0);
-    }
-    s.label_count.back()++;
-  }
-  if (s.label_value.size() > MAX_LABELS) {
-    delete g;
-    return GPSM_EINVAL;
-  }
-  std::vector<uint32_t> key(n);
-  for (uint32_t v = 0; v < n; ++v) {
-    uint32_t deg = row_offset[v + 1] - row_offset[v];
-    key[v] = (s.dense(labels[v]) << KEY_DEG_BITS) | std::min(deg, KEY_DEG_MASK);
-  }
   DevGraph& d = g->dg;
   d.n = n;
   d.adj = A;
-  d.W = words_of(n);
-  d.NB = nblocks_of(n);
-  hipSetDevice(c->device);
-  std::vector<uint32_t> fch;
-  filter_chunks(n, row_offset, fch);
-  d.nfchunk = (uint32_t)fch.size() - 1;
-  auto fail = [&](int code) {
-    hipFree(d.row);
-    hipFree(d.col);
-    hipFree(d.key);
-    hipFree(d.fchunk);
-    hipFree(d.hub_of);
-    hipFree(d.hub_bits);
-    delete g;
-    return code;
-  };
   if (hipMalloc(&d.row, sizeof(uint32_t) * (n + 1)) != hipSuccess ||
-      hipMalloc(&d.col, sizeof(uint32_t) * std::max<uint64_t>(A, 1)) != hipSuccess ||
-      hipMalloc(&d.key, sizeof(uint32_t) * n) != hipSuccess ||
-      hipMalloc(&d.fchunk, sizeof(uint32_t) * fch.size()) != hipSuccess) {
+      hipMalloc(&d.col, sizeof(uint32_t) * std::max<uint64_t>(A, 1)) != hipSuccess) {
     (void)hipGetLastError();
-    return fail(GPSM_ENOMEM);
+    free_graph_arrays(d);
+    delete g;
+    return GPSM_ENOMEM;
   }
   if (hipMemcpyAsync(d.row, row_offset, sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice, c->stream) !=
           hipSuccess ||
-      (A && hipMemcpyAsync(d.col, col, sizeof(uint32_t) * A, hipMemcpyHostToDevice, c->stream) != hipSuccess) ||
-      hipMemcpyAsync(d.key, key.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-      hipMemcpyAsync(d.fchunk, fch.data(), sizeof(uint32_t) * fch.size(), hipMemcpyHostToDevice, c->stream) !=
+      (A && hipMemcpyAsync(d.col, col, sizeof(uint32_t) * A, hipMemcpyHostToDevice, c->stream) != hipSuccess)) {
+    (void)hipGetLastError();
+    free_graph_arrays(d);
+    delete g;
+    return GPSM_EDEVICE;
+  }
+  if (int rc = finish_graph(c, g, row_offset, labels)) return rc;
+  *out = g;
+  return GPSM_OK;
+}
+
+int gpsm_graph_upload_edges(gpsm_ctx* c, uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                            const uint32_t* labels, gpsm_graph** out) {
+  if (!c || !out || !labels || n == 0 || (m && (!src || !dst)) || 2 * m >= (1ull << 32)) return GPSM_EINVAL;
+  *out = nullptr;
+  hipSetDevice(c->device);
+  gpsm_graph* g = new gpsm_graph();
+  g->ctx = c;
+  DevGraph& d = g->dg;
+  d.n = n;
+  uint64_t adj = 0;
+  int rc = build_csr_from_edges(n, m, src, dst, c->stream, &d.row, &d.col, &adj);
+  if (rc) {
+    free_graph_arrays(d);
+    delete g;
+    return rc;
+  }
+  d.adj = adj;
+  std::vector<uint32_t> ro(n + 1);
+  if (hipMemcpyAsync(ro.data(), d.row, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToHost, c->stream) !=
           hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess) {
     (void)hipGetLastError();
-    return fail(GPSM_EDEVICE);
+    free_graph_arrays(d);
+    delete g;
+    return GPSM_EDEVICE;
   }
-  g->bytes = sizeof(uint32_t) * ((uint64_t)n + 1 + A + n + fch.size());
-  if (int rc = build_hubs(c, row_offset, g)) return fail(rc);
+  if ((rc = finish_graph(c, g, ro.data(), labels))) return rc;
   *out = g;
+  return GPSM_OK;
+}
+
+int gpsm_graph_info(const gpsm_graph* g, uint32_t* n, uint64_t* adj) {
+  if (!g) return GPSM_EINVAL;
+  if (n) *n = g->dg.n;
+  if (adj) *adj = g->dg.adj;
+  return GPSM_OK;
+}
+
+int gpsm_graph_csr(const gpsm_graph* g, uint32_t* row_offset, uint32_t* col) {
+  if (!g || !row_offset || (g->dg.adj && !col)) return GPSM_EINVAL;
+  hipSetDevice(g->ctx->device);
+  hipStream_t s = g->ctx->stream;
+  if (hipMemcpyAsync(row_offset, g->dg.row, sizeof(uint32_t) * ((size_t)g->dg.n + 1), hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      (g->dg.adj &&
+       hipMemcpyAsync(col, g->dg.col, sizeof(uint32_t) * g->dg.adj, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    (void)hipGetLastError();
+    return GPSM_EDEVICE;
+  }
   return GPSM_OK;
 }
 
@@ -335,12 +419,7 @@ int gpsm_graph_free(gpsm_graph* g) {
   if (!g) return GPSM_EINVAL;
   hipSetDevice(g->ctx->device);
   hipStreamSynchronize(g->ctx->stream);
-  hipFree(g->dg.row);
-  hipFree(g->dg.col);
-  hipFree(g->dg.key);
-  hipFree(g->dg.fchunk);
-  hipFree(g->dg.hub_of);
-  hipFree(g->dg.hub_bits);
+  free_graph_arrays(g->dg);
   delete g;
   return GPSM_OK;
 }

@@ -28,6 +28,18 @@ void Graph::transformToCSR() {
     undirected_column_index.insert(undirected_column_index.end(), rows.begin(), rows.end());
     undirected_row_offset[v + 1] = (unsigned)undirected_column_index.size();
   }
+  build_inverse_labels();
+}
+
+void Graph::setCSR(unsigned n, const unsigned* row_offset, const unsigned* col, const unsigned* labels) {
+  vertex_num = n;
+  vertex_value.assign(labels, labels + n);
+  undirected_row_offset.assign(row_offset, row_offset + n + 1);
+  undirected_column_index.assign(col, col + row_offset[n]);
+  build_inverse_labels();
+}
+
+void Graph::build_inverse_labels() {
   // inverse label list: labels ascending with their vertex counts
   std::vector<unsigned> lv(vertex_value);
   std::sort(lv.begin(), lv.end());

@@ -36,6 +36,14 @@ class Graph {
   std::vector<unsigned> inverse_num;              // vertices per label
 
   void transformToCSR();
+  // The CSR fields (and the inverse label list) from an existing undirected
+  // CSR, e.g. a mapped binary cache; the in/out lists stay empty.
+  void setCSR(unsigned n, const unsigned* row_offset, const unsigned* col, const unsigned* labels);
+
+ private:
+  void build_inverse_labels();
+
+ public:
   // directed, edge-labelled membership (graph/Graph.cpp:107-118)
   bool isEdgeContained(int from, int to, int label) const;
   unsigned eSize() const { return undirected_row_offset.empty() ? 0 : undirected_row_offset[vertex_num]; }

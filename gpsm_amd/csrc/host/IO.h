@@ -23,6 +23,8 @@ class IO {
   // next graph of a stream; nullptr at the terminator, sets error() on failure
   Graph* input(FILE* fp);
   bool error() const { return err; }
+  // a data graph read by other means (a binary CSR cache) counts as the next
+  void next_data() { ++data_id; }
   bool output(int qid);
   // one line per embedding: "(query id, data id)" pairs in query-id order
   bool output(const unsigned* final_result, unsigned rows, unsigned cols, const int* id_map, Graph* query,

@@ -51,6 +51,13 @@ void Match::releaseGPU() {
   g_ctx = nullptr;
 }
 
+void Match::forget(const Graph* data) {
+  auto it = g_resident.find(data);
+  if (it == g_resident.end()) return;
+  gpsm_graph_free(it->second);
+  g_resident.erase(it);
+}
+
 Match::Match(Graph* q, Graph* d) : query(q), data(d) {}
 
 Match::~Match() { delete[] id2pos; }

@@ -25,6 +25,9 @@ class Match {
   // Selects the device; returns false (never exits) when none is usable.
   static bool initGPU(int dev);
   static void releaseGPU();
+  // Frees the device copy of a data graph (call before deleting the Graph:
+  // resident copies are keyed by its address).
+  static void forget(const Graph* data);
   int last_status() const { return status; }
 
  private:

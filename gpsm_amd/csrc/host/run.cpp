@@ -15,11 +15,40 @@
 #include "Graph.h"
 #include "IO.h"
 #include "Match.h"
+#include "gpsm_gen.h"
 
 static long now_ms() {
   timeval tv;
   gettimeofday(&tv, nullptr);
   return tv.tv_sec * 1000 + tv.tv_usec / 1000;
+}
+
+// The first 8 bytes of a binary CSR cache (gpsm_io_write_csr).
+static bool is_cache(const std::string& path) {
+  char m[8] = {0};
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) return false;
+  const bool ok = std::fread(m, 1, 8, f) == 8 && std::memcmp(m, "GPSMCSR\x01", 8) == 0;
+  std::fclose(f);
+  return ok;
+}
+
+// GpSM --write-cache <data.g> <out.csr>: the first data graph of a text
+// file as a binary CSR cache (no device needed).
+static int write_cache(const std::string& data, const std::string& out) {
+  IO io(data, data, "/dev/null");
+  if (!io.ok()) return 1;
+  Graph* g = nullptr;
+  if (!io.input(g)) {
+    std::fprintf(stderr, "data file: malformed or missing terminator\n");
+    return 1;
+  }
+  const int rc = gpsm_io_write_csr(out.c_str(), g->vertex_num, g->undirected_row_offset.data(),
+                                   g->undirected_column_index.data(), g->vertex_value.data());
+  std::fprintf(stderr, "%s: %u vertices, %zu adjacency entries -> %s (%s)\n", data.c_str(), g->vertex_num,
+               g->undirected_column_index.size(), out.c_str(), gpsm_strerror(rc));
+  delete g;
+  return rc == GPSM_OK ? 0 : 1;
 }
 
 int main(int argc, const char* argv[]) {
@@ -29,14 +58,20 @@ int main(int argc, const char* argv[]) {
     if (std::strcmp(argv[i], "--count") == 0) count_only = true;
     else args.push_back(argv[i]);
   }
+  if (args.size() == 3 && args[0] == "--write-cache") return write_cache(args[1], args[2]);
   if (args.size() < 2 || args.size() > 4) {
-    std::fprintf(stderr, "usage: GpSM [--count] <data> <query> [output=ans.txt] [dev=0]\n");
+    std::fprintf(stderr,
+                 "usage: GpSM [--count] <data> <query> [output=ans.txt] [dev=0]\n"
+                 "       GpSM --write-cache <data.g> <out.csr>\n"
+                 "<data> is a text graph stream or a binary CSR cache (one data graph).\n");
     return 2;
   }
   const std::string output = args.size() >= 3 ? args[2] : "ans.txt";
   const int dev = args.size() == 4 ? std::atoi(args[3].c_str()) : 0;
+  const bool cache = is_cache(args[0]);
   if (!Match::initGPU(dev)) return 1;
-  IO io(args[1], args[0], output);
+  // a cache is read below; the text reader then only serves the queries
+  IO io(args[1], cache ? args[1] : args[0], output);
   if (!io.ok()) return 1;
   std::vector<Graph*> queries;
   if (!io.input(queries)) {
@@ -44,9 +79,8 @@ int main(int argc, const char* argv[]) {
     return 1;
   }
   std::fprintf(stderr, "input ok!\n");
-  Graph* data = nullptr;
   int rc = 0;
-  while (io.input(data)) {
+  auto run_data = [&](Graph* data) {
     long start = now_ms();
     for (size_t i = 0; i < queries.size(); ++i) {
       Match m(queries[i], data);
@@ -68,11 +102,28 @@ int main(int argc, const char* argv[]) {
       delete[] result;
     }
     std::fprintf(stderr, "match used: %ld ms\n", now_ms() - start);
+    Match::forget(data);  // its device copy is keyed by the address about to be freed
     delete data;
-  }
-  if (io.error()) {
-    std::fprintf(stderr, "data file: malformed or missing terminator\n");
-    rc = 1;
+  };
+  if (cache) {
+    gpsm_csr c;
+    const int mrc = gpsm_io_map_csr(args[0].c_str(), &c);
+    if (mrc != GPSM_OK) {
+      std::fprintf(stderr, "data cache: %s\n", gpsm_strerror(mrc));
+      return 1;
+    }
+    Graph* data = new Graph;
+    data->setCSR(c.n, c.row_offset, c.col, c.labels);
+    gpsm_io_unmap_csr(&c);
+    io.next_data();
+    run_data(data);
+  } else {
+    Graph* data = nullptr;
+    while (io.input(data)) run_data(data);
+    if (io.error()) {
+      std::fprintf(stderr, "data file: malformed or missing terminator\n");
+      rc = 1;
+    }
   }
   for (Graph* q : queries) delete q;
   io.flush();

@@ -38,6 +38,13 @@ constexpr uint32_t FCHUNK_ENTRIES = 4096;
 // Boundaries of the filter's work items over a CSR (host side).
 void filter_chunks(uint32_t n, const uint32_t* row_offset, std::vector<uint32_t>& bounds);
 
+// build.hip: the undirected CSR of an edge list (host arrays src/dst of m
+// edges, either orientation; duplicates and self-loops dropped), built on
+// the device.  *row (n+1) and *col (*adj) are new device allocations.
+// GPSM_EINVAL if an endpoint is >= n.
+int build_csr_from_edges(uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst, hipStream_t s,
+                         uint32_t** row, uint32_t** col, uint64_t* adj);
+
 // Sets the adjacency bitmaps of the hubs: hub_bits (nhub * W words) must be
 // zero; hubs[h] is the vertex of bitmap h; block_base[h] = 64-entry blocks
 // of the adjacency of hubs before h (nhub + 1 entries, last = nblocks).

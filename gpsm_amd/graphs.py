@@ -57,6 +57,44 @@ class Query:
         return _lib.QueryS(self.n, self.m, ptr(self.labels), ptr(self.src), ptr(self.dst))
 
 
+class MappedGraph(DataGraph):
+    """A DataGraph whose arrays are a read-only mapping of a binary CSR
+    cache file (gpsm_io_map_csr); close() unmaps it."""
+
+    def __init__(self, s):
+        n, adj = int(s.n), int(s.adj)
+        super().__init__(np.ctypeslib.as_array(s.row_offset, shape=(n + 1,)),
+                         np.ctypeslib.as_array(s.col, shape=(max(adj, 1),))[:adj],
+                         np.ctypeslib.as_array(s.labels, shape=(n,)))
+        self._s = s
+
+    def close(self):
+        if self._s is not None:
+            self.row_offset = self.col = self.labels = None
+            _lib.lib().gpsm_io_unmap_csr(C.byref(self._s))
+            self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def save_csr(g, path):
+    """Writes g as a binary CSR cache file (gpsm_io_write_csr)."""
+    ro, col, lab = u32(g.row_offset), u32(g.col), u32(g.labels)
+    check(_lib.lib().gpsm_io_write_csr(str(path).encode(), len(lab), ptr(ro), ptr(col), ptr(lab)),
+          "gpsm_io_write_csr")
+
+
+def load_csr(path):
+    """Maps a binary CSR cache file (gpsm_io_map_csr): a MappedGraph."""
+    s = _lib.CSR()
+    check(_lib.lib().gpsm_io_map_csr(str(path).encode(), C.byref(s)), "gpsm_io_map_csr")
+    return MappedGraph(s)
+
+
 def _take_csr(s):
     n = s.n
     adj = s.adj

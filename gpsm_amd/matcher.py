@@ -86,6 +86,23 @@ class Context:
     def upload(self, g):
         return ResidentGraph(self, g)
 
+    def upload_edges(self, n, src, dst, labels):
+        """A resident graph built on the device from an edge list
+        (gpsm_graph_upload_edges); its CSR is read back into .g."""
+        from .graphs import DataGraph
+        src, dst, lab = u32(src), u32(dst), u32(labels)
+        h = C.c_void_p()
+        check(_lib.lib().gpsm_graph_upload_edges(self.h, n, len(src), ptr(src), ptr(dst), ptr(lab), C.byref(h)),
+              "gpsm_graph_upload_edges")
+        nn, adj = C.c_uint32(), C.c_uint64()
+        _lib.lib().gpsm_graph_info(h, C.byref(nn), C.byref(adj))
+        ro = np.zeros(nn.value + 1, dtype=np.uint32)
+        col = np.zeros(max(adj.value, 1), dtype=np.uint32)
+        rc = _lib.lib().gpsm_graph_csr(h, ptr(ro), ptr(col))
+        rg = ResidentGraph(self, DataGraph(ro, col[:adj.value], lab.copy()), handle=h)
+        check(rc, "gpsm_graph_csr")
+        return rg
+
     def profile(self, mode=1):
         """0 off, 1 per-launch timing, 2 timing + the filter's work counters
         (algorithmic bytes); setting a mode clears the statistics."""
@@ -134,14 +151,15 @@ class ResidentGraph:
     """A data graph uploaded once to HBM (replaces Match::copyGraphToGPU,
     which re-uploaded it for every query, match/Match.cu:128-143)."""
 
-    def __init__(self, ctx, g):
+    def __init__(self, ctx, g, handle=None):
         self.ctx = ctx
         self.g = g
-        h = C.c_void_p()
-        ro, col, lab = u32(g.row_offset), u32(g.col), u32(g.labels)
-        check(_lib.lib().gpsm_graph_upload(ctx.h, len(lab), ptr(ro), ptr(col), ptr(lab), C.byref(h)),
-              "gpsm_graph_upload")
-        self.h = h
+        if handle is None:
+            handle = C.c_void_p()
+            ro, col, lab = u32(g.row_offset), u32(g.col), u32(g.labels)
+            check(_lib.lib().gpsm_graph_upload(ctx.h, len(lab), ptr(ro), ptr(col), ptr(lab), C.byref(handle)),
+                  "gpsm_graph_upload")
+        self.h = handle
         ctx.graphs.add(self)
 
     def free(self):

@@ -111,6 +111,19 @@ const char* gpsm_device_name(const gpsm_ctx* ctx);
  * self-loops and no duplicates, labels[n] >= 1. */
 int gpsm_graph_upload(gpsm_ctx* ctx, uint32_t n, const uint32_t* row_offset,
                       const uint32_t* col, const uint32_t* labels, gpsm_graph** out);
+/* The same resident graph built on the device from an edge list of m edges
+ * (src[i], dst[i]) in either orientation, duplicates and self-loops dropped
+ * (no counterpart in the reference, which builds the CSR on the host,
+ * graph/Graph.cpp:30-104; its note at graph/Graph.cpp:26-28 asks for a GPU
+ * build): one radix sort of 64-bit (row, column) keys and one unique pass.
+ * GPSM_EINVAL if an endpoint is >= n or 2m >= 2^32. */
+int gpsm_graph_upload_edges(gpsm_ctx* ctx, uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                            const uint32_t* labels, gpsm_graph** out);
+/* Vertices and adjacency entries (2 x undirected edges) of a resident graph. */
+int gpsm_graph_info(const gpsm_graph* g, uint32_t* n, uint64_t* adj);
+/* Copies a resident graph's CSR to host buffers row_offset[n+1], col[adj]
+ * (e.g. to write a binary cache of a graph built on the device). */
+int gpsm_graph_csr(const gpsm_graph* g, uint32_t* row_offset, uint32_t* col);
 int gpsm_graph_free(gpsm_graph* g);
 /* Bytes of device memory the resident graph occupies. */
 uint64_t gpsm_graph_device_bytes(const gpsm_graph* g);

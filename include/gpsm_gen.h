@@ -95,6 +95,18 @@ int gpsm_io_write_end(const char* path);
  * a missing "t # -1" terminator, GPSM_EINVAL if the file cannot be opened. */
 int gpsm_io_count_graphs(const char* path, uint32_t* ngraphs, uint64_t* nverts, uint64_t* nedges);
 
+/* Binary CSR cache of one data graph (no counterpart in the reference, whose
+ * text reader io/IO.cpp:46-100 parses token by token): a 64-byte header
+ * ("GPSMCSR" + version 1, n, adj) followed by row_offset[n+1], col[adj] and
+ * labels[n] as little-endian u32.  map: read-only mmap of such a file; out's
+ * arrays point into the mapping until gpsm_io_unmap_csr.  The header, the
+ * file length and the offsets (monotone, ending at adj) are checked
+ * (GPSM_EFORMAT otherwise); rows are validated by gpsm_graph_upload. */
+int gpsm_io_write_csr(const char* path, uint32_t n, const uint32_t* row_offset, const uint32_t* col,
+                      const uint32_t* labels);
+int gpsm_io_map_csr(const char* path, gpsm_csr* out);
+void gpsm_io_unmap_csr(gpsm_csr* g);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,6 +4,7 @@ touches a GPU."""
 import ctypes as C
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -146,3 +147,53 @@ def test_reader_roundtrip_and_errors(tmp_path):
     bad3 = tmp_path / "range.g"
     bad3.write_text(text.replace("e 0 1 0", "e 0 99 0", 1))
     assert _count(bad3)[0] == -8
+
+
+def test_binary_csr_cache_roundtrip_and_errors(tmp_path):
+    """A graph written to the binary cache maps back with identical arrays;
+    damaged files are refused with GPSM_EFORMAT."""
+    g = G.small_graph(3, n=5000, m=20000, labels=7, kind="cl")
+    p = tmp_path / "g.csr"
+    G.save_csr(g, p)
+    m = G.load_csr(p)
+    assert m.n == g.n and m.adj == g.adj
+    assert np.array_equal(m.row_offset, g.row_offset) and np.array_equal(m.col, g.col)
+    assert np.array_equal(m.labels, g.labels)
+    m.close()
+    raw = p.read_bytes()
+    assert raw[:8] == b"GPSMCSR\x01" and len(raw) == 64 + 4 * (2 * g.n + 1 + g.adj)
+    bad = tmp_path / "bad.csr"
+    cases = [raw[:-4],                                   # truncated
+             b"GPSMCSR\x02" + raw[8:],                   # unknown version
+             raw[:64] + (1).to_bytes(4, "little") + raw[68:]]  # row_offset[0] != 0
+    ro_end = 64 + 4 * g.n  # row_offset[n]
+    cases.append(raw[:ro_end] + (g.adj - 1).to_bytes(4, "little") + raw[ro_end + 4:])  # offsets end early
+    for data in cases:
+        bad.write_bytes(data)
+        with pytest.raises(_lib.GpsmError) as e:
+            G.load_csr(bad)
+        assert e.value.code == -8
+    with pytest.raises(_lib.GpsmError):
+        G.load_csr(tmp_path / "missing.csr")
+
+
+def test_cli_writes_the_binary_cache(tmp_path):
+    """GpSM --write-cache turns the first data graph of a text file into a
+    binary cache whose arrays equal the CSR the reader builds."""
+    g = G.small_graph(4, n=2000, m=9000, labels=5, kind="er")
+    p = tmp_path / "data.g"
+    src, dst = g.edges()
+    _lib.lib().gpsm_io_write_graph(str(p).encode(), 0, 0, g.n, _lib.ptr(g.labels), len(src), _lib.ptr(src),
+                                   _lib.ptr(dst))
+    _lib.lib().gpsm_io_write_end(str(p).encode())
+    out = tmp_path / "data.csr"
+    exe = os.path.join(os.path.dirname(_lib.LIB_PATH), "GpSM")
+    r = subprocess.run([exe, "--write-cache", str(p), str(out)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    m = G.load_csr(out)
+    assert np.array_equal(m.row_offset, g.row_offset) and np.array_equal(m.col, g.col)
+    assert np.array_equal(m.labels, g.labels)
+    m.close()
+    bad = subprocess.run([exe, "--write-cache", str(tmp_path / "missing.g"), str(out)], capture_output=True,
+                         timeout=60)
+    assert bad.returncode != 0

@@ -292,3 +292,62 @@ def test_hub_bitmaps_give_the_same_results(ctx):
         ctx.set_hub_policy()
     nh2, d2 = rc.hubs()
     assert d2 >= 16 and nh2 < nh and nh2 * ((g.n + 31) // 32) * 4 <= nh * ((g.n + 31) // 32) * 4 - 1
+
+
+def test_device_csr_build_equals_host_build(ctx):
+    """gpsm_graph_upload_edges: an edge list with both orientations,
+    duplicates and self-loops gives the host builder's CSR, and the same
+    match results; an endpoint out of range is refused."""
+    rng = np.random.default_rng(3)
+    n, m = 4000, 30000
+    a = rng.integers(0, n, m).astype(np.uint32)
+    b = (a + rng.zipf(1.6, m) % n).astype(np.uint32) % n  # skewed, with self-loops (offset 0 mod n)
+    a = np.concatenate([a, b[:5000]]).astype(np.uint32)   # reversed duplicates
+    b = np.concatenate([b, a[:5000]]).astype(np.uint32)
+    labels = (rng.integers(0, 3, n) + 1).astype(np.uint32)
+    host = G.from_edges(n, a, b, labels)
+    assert (a == b).any()
+    dev = ctx.upload_edges(n, a, b, labels)
+    assert np.array_equal(dev.g.row_offset, host.row_offset) and np.array_equal(dev.g.col, host.col)
+    rh = ctx.upload(host)
+    qs = G.queries(host, 12, 3, 6, _lib.Q_WALK_HALF, 31, 10 ** 7)
+    x = rh.match_batch(qs)
+    y = dev.match_batch(qs)
+    assert [(r.count, r.hash, r.join_work) for r in x] == [(r.count, r.hash, r.join_work) for r in y]
+    assert sum(r.count for r in x) > 0
+    with pytest.raises(_lib.GpsmError):
+        ctx.upload_edges(n, np.array([0, n], dtype=np.uint32), np.array([1, 2], dtype=np.uint32), labels)
+    empty = ctx.upload_edges(5, np.zeros(0, dtype=np.uint32), np.zeros(0, dtype=np.uint32),
+                             np.ones(5, dtype=np.uint32))
+    assert empty.g.adj == 0 and np.array_equal(empty.g.row_offset, np.zeros(6, dtype=np.uint32))
+
+
+def test_cli_counts_from_text_and_from_cache_agree(tmp_path):
+    """GpSM --count on a text data graph and on its binary cache prints the
+    same counts and hashes, equal to the enumerator's."""
+    import os
+    import subprocess
+    g = G.small_graph(16, n=2500, m=10000, labels=3, kind="cl")
+    qs = G.queries(g, 4, 3, 5, _lib.Q_WALK_HALF, 37, 10 ** 6)
+    data, query, cache = tmp_path / "d.g", tmp_path / "q.g", tmp_path / "d.csr"
+    src, dst = g.edges()
+    _lib.lib().gpsm_io_write_graph(str(data).encode(), 0, 0, g.n, _lib.ptr(g.labels), len(src), _lib.ptr(src),
+                                   _lib.ptr(dst))
+    _lib.lib().gpsm_io_write_end(str(data).encode())
+    for i, q in enumerate(qs):
+        q.struct()
+        _lib.lib().gpsm_io_write_graph(str(query).encode(), i, i, q.n, _lib.ptr(q.labels), q.m, _lib.ptr(q.src),
+                                       _lib.ptr(q.dst))
+    _lib.lib().gpsm_io_write_end(str(query).encode())
+    exe = os.path.join(os.path.dirname(_lib.LIB_PATH), "GpSM")
+    assert subprocess.run([exe, "--write-cache", str(data), str(cache)], timeout=60).returncode == 0
+    outs = []
+    for d in (data, cache):
+        r = subprocess.run([exe, "--count", str(d), str(query), str(tmp_path / "ans.txt")], capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs.append([l for l in r.stdout.splitlines() if l.startswith("query ")])
+    assert outs[0] == outs[1] and len(outs[0]) == len(qs)
+    for line, q in zip(outs[0], qs):
+        ref = O.o1_enumerate(g, q)
+        assert line.endswith("%d embeddings, hash %016x" % (ref["count"], ref["hash"])), line
