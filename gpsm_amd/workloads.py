@@ -179,6 +179,60 @@ def run(name, steps=5, autotune=False, plain=True, seed=None, budget=EMBEDDING_B
     return out
 
 
+def loaders(name, tmpdir, text=True, log=print):
+    """Times the ways a data graph reaches the device: parsing the text
+    format (the reference's reader, io/IO.cpp:46-100) and uploading the CSR,
+    mapping a binary CSR cache and uploading it, and building the CSR on the
+    device from the edge list (gpsm_graph_upload_edges)."""
+    import ctypes as C
+    import os
+    import time
+
+    from .matcher import Context
+
+    g, _ = build(name)
+    src, dst = g.edges()
+    out = {"workload": name, "n": g.n, "undirected_edges": int(len(src))}
+    log("%s: %d vertices, %d edges" % (name, g.n, len(src)))
+    with Context(0) as ctx:
+        ctx.set_hub_policy(512, 0)  # graph arrays only: the same work on every path
+        cache = os.path.join(tmpdir, name + ".csr")
+        t0 = time.perf_counter()
+        G.save_csr(g, cache)
+        out["cache_write_s"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        m = G.load_csr(cache)
+        rg = ctx.upload(m)
+        out["cache_map_upload_s"] = time.perf_counter() - t0
+        rg.free()
+        m.close()
+        log("binary cache: write %.2f s, map + upload %.2f s" % (out["cache_write_s"], out["cache_map_upload_s"]))
+        t0 = time.perf_counter()
+        rd = ctx.upload_edges(g.n, src, dst, g.labels)
+        out["device_build_s"] = time.perf_counter() - t0
+        assert rd.g.adj == g.adj
+        rd.free()
+        log("device build from the edge list (incl. reading the CSR back): %.2f s" % out["device_build_s"])
+        if text:
+            path = os.path.join(tmpdir, name + ".g")
+            t0 = time.perf_counter()
+            check = _lib.lib().gpsm_io_write_graph(path.encode(), 0, 0, g.n, _lib.ptr(g.labels), len(src),
+                                                   _lib.ptr(src), _lib.ptr(dst))
+            _lib.lib().gpsm_io_write_end(path.encode())
+            out["text_write_s"] = time.perf_counter() - t0
+            out["text_bytes"] = os.path.getsize(path)
+            log("text: written in %.1f s (%d bytes)" % (out["text_write_s"], out["text_bytes"]))
+            t0 = time.perf_counter()
+            ng, nv, ne = C.c_uint32(), C.c_uint64(), C.c_uint64()
+            rc = _lib.lib().gpsm_io_count_graphs(path.encode(), C.byref(ng), C.byref(nv), C.byref(ne))
+            out["text_parse_s"] = time.perf_counter() - t0
+            assert rc == 0 and check == 0 and ng.value == 1
+            log("text: parsed by the driver's reader in %.1f s" % out["text_parse_s"])
+            os.remove(path)
+        os.remove(cache)
+    return out
+
+
 def main():
     import argparse
     import json
@@ -190,9 +244,15 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--budget", type=float, default=EMBEDDING_BUDGET)
     ap.add_argument("--json", default=None)
+    ap.add_argument("--loaders", metavar="TMPDIR", default=None,
+                    help="time text parsing, the binary cache and the device CSR build instead")
+    ap.add_argument("--no-text", action="store_true", help="with --loaders: skip the text format")
     args = ap.parse_args()
-    out = run(args.workload, args.steps, args.autotune, not args.no_plain, budget=int(args.budget),
-              log=lambda s: print(s, flush=True))
+    if args.loaders:
+        out = loaders(args.workload, args.loaders, not args.no_text, log=lambda s: print(s, flush=True))
+    else:
+        out = run(args.workload, args.steps, args.autotune, not args.no_plain, budget=int(args.budget),
+                  log=lambda s: print(s, flush=True))
     if args.json:
         with open(args.json, "w") as fh:
             json.dump(out, fh, indent=1)
