@@ -680,7 +680,9 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   std::vector<JoinPlan> plans;
   std::vector<JoinTask> tasks;
   std::vector<uint32_t> plan_of_q(nq, ~0u);
-  uint64_t off_words = 0, list_words = 0;
+  uint64_t off_words = 0, list_words = 0, term_words = 0;
+  std::vector<uint32_t> term_desc;   // tables of last join levels
+  std::vector<uint64_t> term_start;  // their term ranges, back to back (+ total)
   for (uint32_t b = 0; b < nq; ++b) {
     QState& S = Q[b];
     if (!S.active) continue;
@@ -759,6 +761,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       d.off_base = off_words;
       d.list_base = list_words;
       d.list_cap = (uint32_t)S.cet_size[edge];
+      d.term_base = ~0ull;
       off_words += d.ncand + 1;
       list_words += S.cet_size[edge];
       jdesc.push_back(d);
@@ -774,6 +777,14 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       if (k >= 2) {
         jp.par[k] = pl.par_pos[k];
         jp.cet[k] = add_desc(pl.order[pl.par_pos[k]], pl.order[k], pl.par_edge[k]);
+        if (k + 1 == S.hq.n) {
+          CetDesc& d = jdesc.back();
+          d.term_base = term_words;
+          d.term_qid = pl.order[k];
+          term_desc.push_back(jp.cet[k]);
+          term_start.push_back(term_words);
+          term_words += S.cet_size[pl.par_edge[k]];
+        }
         for (uint32_t x : pl.backs[k]) jp.backs[nb++] = (uint8_t)x;
       }
     }
@@ -809,8 +820,11 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   }
 
   // ---------------- phase C: candidate-edge tables ----------------
-  if ((rc = upload(c, B_JDESC, jdesc)) || (rc = upload(c, B_PLANS, plans)) || (rc = upload(c, B_TASKS, tasks)))
+  term_start.push_back(term_words);
+  if ((rc = upload(c, B_JDESC, jdesc)) || (rc = upload(c, B_PLANS, plans)) || (rc = upload(c, B_TASKS, tasks)) ||
+      (rc = upload(c, B_TERM_DESC, term_desc)) || (rc = upload(c, B_TERM_START, term_start)))
     return rc;
+  if ((rc = c->bufs[B_TERMS].ensure(sizeof(uint64_t) * std::max<uint64_t>(term_words, 1)))) return rc;
   if ((rc = c->bufs[B_OFFSETS].ensure(sizeof(uint32_t) * std::max<uint64_t>(off_words, 1))) ||
       (rc = c->bufs[B_LISTS].ensure(sizeof(uint32_t) * std::max<uint64_t>(list_words, 1))) ||
       (rc = c->bufs[B_CHUNKS].ensure(sizeof(uint32_t) * (jdesc.size() + 2))))
@@ -838,6 +852,9 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     pa = pbegin();
     launch_cet_write(dg, bitmaps, dptr<uint32_t>(c, B_CLIST), dptr<CetDesc>(c, B_JDESC), (uint32_t)jdesc.size(),
                      dptr<uint32_t>(c, B_OFFSETS), dptr<uint32_t>(c, B_LISTS), dptr<uint32_t>(c, B_CHUNKS), cgrid, s);
+    launch_cet_terms(dptr<uint32_t>(c, B_LISTS), dptr<CetDesc>(c, B_JDESC), dptr<uint32_t>(c, B_TERM_DESC),
+                     dptr<uint64_t>(c, B_TERM_START), (uint32_t)term_desc.size(), term_words,
+                     dptr<unsigned long long>(c, B_TERMS), s);
     pend(K_CET_WRITE, pa, 0.0, -1);
     // ---------------- phase D: join ----------------
     launch_fill_u32(dptr<uint32_t>(c, B_JOUT), 0u, sizeof(JoinOut) / 4 * (uint64_t)std::max<uint32_t>(nplans, 1), s);
@@ -852,6 +869,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     ja.clist = dptr<uint32_t>(c, B_CLIST);
     ja.offsets = dptr<uint32_t>(c, B_OFFSETS);
     ja.lists = dptr<uint32_t>(c, B_LISTS);
+    ja.terms = dptr<unsigned long long>(c, B_TERMS);
     ja.cets = dptr<CetDesc>(c, B_JDESC);
     ja.plans = dptr<JoinPlan>(c, B_PLANS);
     ja.tasks = dptr<JoinTask>(c, B_TASKS);

@@ -44,6 +44,9 @@ enum BufId {
   B_EMIT,
   B_MLP,
   B_FSTATS,
+  B_TERMS,       // hash terms of the last join level's lists (u64)
+  B_TERM_DESC,   // tables that have them
+  B_TERM_START,  // their ranges
   B_COUNT_
 };
 

@@ -51,6 +51,11 @@ struct CetDesc {
   uint64_t off_base;   // offsets [ncand+1] (u32) in the offsets arena
   uint64_t list_base;  // lists in the list arena
   uint64_t total;      // |CET|
+  // the last join level's table also gets the hash term of every list entry
+  // (tuple_term(term_qid, list[e]) at term_base + e); ~0 for other tables
+  uint64_t term_base;
+  uint32_t term_qid;
+  uint32_t pad;
 };
 
 // Join plan of one query: vertex order and, per level, how to extend.

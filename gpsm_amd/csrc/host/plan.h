@@ -131,13 +131,14 @@ void candidate_orders(const HostQuery& q, const std::vector<uint64_t>& cet_size,
 // rows).  The final level's examined candidates are counted; its rows, the
 // embedding count, are the same for every order.  Evaluated one level per
 // round so that all candidates of a batch share one network call per round.
-// Join time of one back-edge check (a binary search in the shorter of two
-// adjacency lists) in units of one examined candidate, fitted on an MI355X
-// to the join kernel's time over batches of the 100 walk queries on the
-// 1M-vertex Chung-Lu graph, each query taking one of its candidate orders at
-// random (gpsm_amd/calibrate_planner.py: 24 batches, 24.3-29.9 ms, largest
-// error 3.3 ms; first-table rows and materialised rows got weight 0).
-constexpr double kBackWeight = 6.4;
+// Join time of one back-edge check (a hub-bitmap bit or a binary search in
+// the shorter of two adjacency lists) in units of one examined candidate,
+// fitted on an MI355X to the join kernel's time over batches of the 100 walk
+// queries on the 1M-vertex Chung-Lu graph, each query taking one of its
+// candidate orders at random (gpsm_amd/calibrate_planner.py: 24 batches,
+// 23.0-27.8 ms, largest error 3.1 ms; first-table rows and materialised rows
+// got weight 0).
+constexpr double kBackWeight = 3.9;
 
 struct PlanCost {
   const PlanContext* c = nullptr;

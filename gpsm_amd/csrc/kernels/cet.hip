@@ -153,6 +153,26 @@ __global__ void __launch_bounds__(256) cet_write_kernel(const uint32_t* __restri
   }
 }
 
+// One thread per term: the table by a binary search over the term ranges.
+__global__ void __launch_bounds__(256) cet_terms_kernel(const uint32_t* __restrict__ lists,
+                                                        const CetDesc* __restrict__ descs,
+                                                        const uint32_t* __restrict__ term_desc,
+                                                        const uint64_t* __restrict__ term_start, uint32_t nd,
+                                                        uint64_t nterms, unsigned long long* __restrict__ terms) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nterms; t += stride) {
+    uint32_t lo = 0, hi = nd;  // last range with start <= t
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (term_start[mid] <= t) lo = mid;
+      else hi = mid;
+    }
+    const CetDesc& d = descs[term_desc[lo]];
+    const uint64_t e = t - term_start[lo];
+    terms[t] = e < d.list_cap ? tuple_term(d.term_qid, lists[d.list_base + e]) : 0ull;
+  }
+}
+
 __global__ void chunk_base_kernel(const CetDesc* __restrict__ descs, uint32_t ndesc, uint32_t* __restrict__ base) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     uint32_t acc = 0;
@@ -178,6 +198,16 @@ void launch_cet_count(const DevGraph& g, const uint32_t* bitmaps, const uint32_t
 void launch_cet_offsets(CetDesc* descs, uint32_t ndesc, uint32_t* offsets, hipStream_t s) {
   if (!ndesc) return;
   hipLaunchKernelGGL(cet_offsets_kernel, dim3(ndesc), dim3(256), 0, s, descs, offsets);
+}
+
+void launch_cet_terms(const uint32_t* lists, const CetDesc* descs, const uint32_t* term_desc,
+                      const uint64_t* term_start, uint32_t nterm_desc, uint64_t nterms, unsigned long long* terms,
+                      hipStream_t s) {
+  if (!nterm_desc || !nterms) return;
+  uint64_t grid = (nterms + 255) / 256;
+  if (grid > 16384) grid = 16384;
+  hipLaunchKernelGGL(cet_terms_kernel, dim3((uint32_t)grid), dim3(256), 0, s, lists, descs, term_desc, term_start,
+                     nterm_desc, nterms, terms);
 }
 
 void launch_cet_write(const DevGraph& g, const uint32_t* bitmaps, const uint32_t* clist, const CetDesc* descs,

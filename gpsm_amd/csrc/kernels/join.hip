@@ -212,6 +212,7 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
         // flight together; rows of the next level are appended per wave (one
         // LDS atomic per wave; row order inside a level does not matter)
         const uint32_t* lst = a.lists + dn.list_base;
+        const unsigned long long* trm_n = final ? a.terms + dn.term_base : nullptr;
         const uint32_t lane = lane_id();
         const unsigned long long below = lane ? (~0ULL >> (WAVE - lane)) : 0ULL;
         uint32_t f = 0;
@@ -223,15 +224,20 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
             if (fl + WROWS * EXP_UNROLL > cap) break;  // uniform
           }
           uint32_t mvs[EXP_UNROLL], is[EXP_UNROLL];
+          unsigned long long trm[EXP_UNROLL];
 #pragma unroll
           for (int u = 0; u < EXP_UNROLL; ++u) {
             const uint32_t g = f + u * WROWS + tid;
             is[u] = WROWS;
             mvs[u] = 0;
+            trm[u] = 0;
             if (g < T) {
               const uint32_t i = seg_search(S.pre, nw, g);
               is[u] = i;
-              mvs[u] = lst[S.start[i] + (g - S.pre[i])];
+              const uint32_t e = S.start[i] + (g - S.pre[i]);
+              mvs[u] = lst[e];
+              // last level: the entry's hash term, computed once per list entry
+              if (final) trm[u] = trm_n[e];
             }
           }
 #pragma unroll
@@ -239,15 +245,16 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
             const uint32_t i = is[u], mv = mvs[u];
             bool ok = i != WROWS;
             if (ok) {
-              my_bytes += 4;
-              for (uint32_t c = 0; c <= k && ok; ++c) ok = val[c * WROWS + i] != mv;
+              my_bytes += 4;  // the list entry (its hash term is a derived copy of it)
+              // injectivity: k+1 comparisons, no early exit (uniform trip count)
+              for (uint32_t c = 0; c <= k; ++c) ok &= val[c * WROWS + i] != mv;
               for (uint32_t b = bo; b < be && ok; ++b)
                 ok = has_edge(a.row, a.col, a.hub_of, a.hub_bits, a.W, mv, val[P.backs[b] * WROWS + i], my_bytes);
             }
             if (final) {
               if (ok) {
                 ++my_cnt;
-                my_hash += fmix64(S.srow[i] + tuple_term(wn, mv));
+                my_hash += fmix64(S.srow[i] + trm[u]);
                 if (a.emit) {
                   const unsigned long long pos = atomicAdd(a.emit_ctr, 1ULL);
                   if (pos < a.emit_cap) {

@@ -88,6 +88,11 @@ void launch_cet_count(const DevGraph& g, const uint32_t* bitmaps, const uint32_t
                       hipStream_t s);
 // Segmented exclusive scan of the per-candidate counts of every desc.
 void launch_cet_offsets(CetDesc* descs, uint32_t ndesc, uint32_t* offsets, hipStream_t s);
+// Hash terms of the last join levels' lists: for table term_desc[i] (terms
+// [term_start[i], term_start[i+1]) of terms), tuple_term(term_qid, list[e]).
+void launch_cet_terms(const uint32_t* lists, const CetDesc* descs, const uint32_t* term_desc,
+                      const uint64_t* term_start, uint32_t nterm_desc, uint64_t nterms, unsigned long long* terms,
+                      hipStream_t s);
 // Writes the lists.
 void launch_cet_write(const DevGraph& g, const uint32_t* bitmaps, const uint32_t* clist, const CetDesc* descs,
                       uint32_t ndesc, const uint32_t* offsets, uint32_t* lists, uint32_t* chunk_base, uint32_t grid,
@@ -104,6 +109,7 @@ struct JoinArgs {
   const uint32_t* clist;
   const uint32_t* offsets;
   const uint32_t* lists;
+  const unsigned long long* terms;  // hash terms of the last level's lists (CetDesc::term_base)
   const CetDesc* cets;
   const JoinPlan* plans;
   const JoinTask* tasks;

@@ -18,7 +18,7 @@ from gpsm_amd import graphs as G
 from gpsm_amd import planner
 from gpsm_amd.calibrate_planner import backs_per_level
 
-BACK = 6.4  # kBackWeight in gpsm_amd/csrc/host/plan.h
+BACK = 3.9  # kBackWeight in gpsm_amd/csrc/host/plan.h
 
 pytestmark = pytest.mark.gpu
 

@@ -16,7 +16,7 @@ from gpsm_amd.train_planner import continuations
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 MARGIN = 0.9  # kLearnedMargin in plan.h
-BACK = 6.4  # kBackWeight in plan.h
+BACK = 3.9  # kBackWeight in plan.h
 
 
 @pytest.fixture(scope="module")
