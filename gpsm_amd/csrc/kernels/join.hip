@@ -225,14 +225,25 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
           }
           uint32_t mvs[EXP_UNROLL], is[EXP_UNROLL];
           unsigned long long trm[EXP_UNROLL];
+          // wave w takes the step's candidates g0 .. g0 + 255 (lane l: g0 +
+          // 64u + l), so one pair of searches per wave brackets the rows they
+          // fall in and each candidate searches only inside the bracket
+          const uint32_t g0 = f + (tid / WAVE) * (WAVE * EXP_UNROLL);
+          uint32_t rlo = 0, rn = 1;
+          if (g0 < T) {  // uniform per wave
+            const uint32_t glast = min(g0 + WAVE * EXP_UNROLL - 1, T - 1);
+            const uint32_t r = seg_search(S.pre, nw, lane == 0 ? g0 : glast);
+            rlo = __shfl(r, 0, WAVE);
+            rn = __shfl(r, 1, WAVE) - rlo + 1;
+          }
 #pragma unroll
           for (int u = 0; u < EXP_UNROLL; ++u) {
-            const uint32_t g = f + u * WROWS + tid;
+            const uint32_t g = g0 + u * WAVE + lane;
             is[u] = WROWS;
             mvs[u] = 0;
             trm[u] = 0;
             if (g < T) {
-              const uint32_t i = seg_search(S.pre, nw, g);
+              const uint32_t i = rlo + seg_search(S.pre + rlo, rn, g);
               is[u] = i;
               const uint32_t e = S.start[i] + (g - S.pre[i]);
               mvs[u] = lst[e];
