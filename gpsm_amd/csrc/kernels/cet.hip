@@ -19,6 +19,8 @@ namespace gpsm {
 
 namespace {
 
+constexpr int CET_UNROLL = 4;  // adjacency entries in flight per lane
+
 struct CetLds {
   uint32_t pre[WAVE + 1];
   uint32_t off[WAVE];
@@ -66,16 +68,24 @@ __global__ void __launch_bounds__(256) cet_count_kernel(const uint32_t* __restri
     load_chunk(row, clist, d, j, nch, L, mine, D);
     const uint32_t* bm_w = bitmaps + (size_t)d.w_slot * W;
     uint32_t tot = 0;
-    for (uint32_t f = 0; f < D; f += WAVE) {
-      const uint32_t g = f + lane;
-      if (g < D) {
-        const uint32_t ci = seg_search(L.pre, WAVE, g);
-        const uint32_t mv = col[L.off[ci] + (g - L.pre[ci])];
-        if (bit_of(bm_w, mv)) {
-          ++tot;
-          if (per_cand) atomicAdd(&L.cnt[ci], 1u);
+    for (uint32_t f = 0; f < D; f += WAVE * CET_UNROLL) {
+      // CET_UNROLL entries per lane: all adjacency loads, then all probes
+      uint32_t cis[CET_UNROLL], mvs[CET_UNROLL];
+#pragma unroll
+      for (int u = 0; u < CET_UNROLL; ++u) {
+        const uint32_t g = f + u * WAVE + lane;
+        cis[u] = WAVE;
+        if (g < D) {
+          cis[u] = seg_search(L.pre, WAVE, g);
+          mvs[u] = col[L.off[cis[u]] + (g - L.pre[cis[u]])];
         }
       }
+#pragma unroll
+      for (int u = 0; u < CET_UNROLL; ++u)
+        if (cis[u] != WAVE && bit_of(bm_w, mvs[u])) {
+          ++tot;
+          if (per_cand) atomicAdd(&L.cnt[cis[u]], 1u);
+        }
     }
     wave_sync();
     if (per_cand && mine) offsets[d.off_base + cand_index(j, nch, lane)] = L.cnt[lane];
@@ -128,27 +138,41 @@ __global__ void __launch_bounds__(256) cet_write_kernel(const uint32_t* __restri
     wave_sync();
     const uint32_t* bm_w = bitmaps + (size_t)d.w_slot * W;
     uint32_t* out = lists + d.list_base;
-    for (uint32_t f = 0; f < D; f += WAVE) {
-      const uint32_t g = f + lane;
-      bool hit = false;
-      uint32_t ci = 0, mv = 0;
-      if (g < D) {
-        ci = seg_search(L.pre, WAVE, g);
-        mv = col[L.off[ci] + (g - L.pre[ci])];
-        hit = bit_of(bm_w, mv) != 0;
+    for (uint32_t f = 0; f < D; f += WAVE * CET_UNROLL) {
+      // CET_UNROLL sub-steps of 64 entries: all adjacency loads, all probes,
+      // then the writes sub-step by sub-step (lists stay in adjacency order)
+      uint32_t cis[CET_UNROLL], mvs[CET_UNROLL];
+      bool hits[CET_UNROLL];
+#pragma unroll
+      for (int u = 0; u < CET_UNROLL; ++u) {
+        const uint32_t g = f + u * WAVE + lane;
+        cis[u] = 0;
+        mvs[u] = 0;
+        hits[u] = false;
+        if (g < D) {
+          cis[u] = seg_search(L.pre, WAVE, g);
+          mvs[u] = col[L.off[cis[u]] + (g - L.pre[cis[u]])];
+        }
       }
-      const unsigned long long hm = __ballot(hit);
-      if (hit) {
-        // lanes of this step that belong to candidate ci start at seg_lo
-        const uint32_t seg_lo = L.pre[ci] > f ? L.pre[ci] - f : 0u;
-        const unsigned long long below = (lane ? (~0ULL >> (WAVE - lane)) : 0ULL) & (~0ULL << seg_lo);
-        const uint32_t rank = __popcll(hm & below);
-        const uint32_t pos = L.cnt[ci] + rank;
-        if (pos < d.list_cap) out[pos] = mv;
+#pragma unroll
+      for (int u = 0; u < CET_UNROLL; ++u)
+        if (f + u * WAVE + lane < D) hits[u] = bit_of(bm_w, mvs[u]) != 0;
+#pragma unroll
+      for (int u = 0; u < CET_UNROLL; ++u) {
+        const uint32_t fu = f + u * WAVE, ci = cis[u];
+        const unsigned long long hm = __ballot(hits[u]);
+        if (hits[u]) {
+          // lanes of this sub-step that belong to candidate ci start at seg_lo
+          const uint32_t seg_lo = L.pre[ci] > fu ? L.pre[ci] - fu : 0u;
+          const unsigned long long below = (lane ? (~0ULL >> (WAVE - lane)) : 0ULL) & (~0ULL << seg_lo);
+          const uint32_t rank = __popcll(hm & below);
+          const uint32_t pos = L.cnt[ci] + rank;
+          if (pos < d.list_cap) out[pos] = mvs[u];
+        }
+        wave_sync();
+        if (hits[u]) atomicAdd(&L.cnt[ci], 1u);
+        wave_sync();
       }
-      wave_sync();
-      if (hit) atomicAdd(&L.cnt[ci], 1u);
-      wave_sync();
     }
   }
 }

@@ -48,24 +48,26 @@ __device__ __forceinline__ bool has_edge(const uint32_t* __restrict__ row, const
     bytes += 4;
     return ha != ~0u ? bit_of(hub_bits + (size_t)ha * W, b) != 0 : bit_of(hub_bits + (size_t)hb * W, a) != 0;
   }
-  uint32_t lo = row[a], hi = row[a + 1];
-  const uint32_t lo2 = row[b], hi2 = row[b + 1];
+  uint32_t lo = row[a], n = row[a + 1] - lo;
+  const uint32_t lo2 = row[b], n2 = row[b + 1] - lo2;
   uint32_t key = b;
-  if (hi2 - lo2 < hi - lo) {
+  if (n2 < n) {
     lo = lo2;
-    hi = hi2;
+    n = n2;
     key = a;
   }
   bytes += 16;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    const uint32_t v = col[mid];
-    bytes += 4;
-    if (v == key) return true;
-    if (v < key) lo = mid + 1;
-    else hi = mid;
+  if (n == 0) return false;
+  // branch-free search for the last entry <= key: ceil(log2 n) halvings, one
+  // comparison at the end (no early exit, so no per-step exec-mask updates)
+  bytes += 4ull * (1 + (n > 1 ? 32 - __clz(n - 1) : 0));  // ceil(log2 n) probes + the last read
+  const uint32_t* p = col + lo;
+  while (n > 1) {
+    const uint32_t half = n >> 1;
+    p = p[half] <= key ? p + half : p;
+    n -= half;
   }
-  return false;
+  return *p == key;
 }
 
 struct JoinShared {
