@@ -85,6 +85,9 @@ SIGNATURES = {
     "gpsm_graph_device_bytes": (C.c_uint64, [C.c_void_p]),
     "gpsm_set_hub_policy": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64]),
     "gpsm_graph_hubs": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "gpsm_graph_set_directed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gpsm_verify_directed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.POINTER(C.c_uint64)]),
     "gpsm_match": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(QueryS), C.POINTER(Opts), C.POINTER(Result)]),
     "gpsm_match_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(QueryS), C.POINTER(Opts),
                                    C.POINTER(Result)]),

@@ -420,11 +420,16 @@ int gpsm_graph_free(gpsm_graph* g) {
   hipSetDevice(g->ctx->device);
   hipStreamSynchronize(g->ctx->stream);
   free_graph_arrays(g->dg);
+  g->directed.release();
   delete g;
   return GPSM_OK;
 }
 
-uint64_t gpsm_graph_device_bytes(const gpsm_graph* g) { return g ? g->bytes : 0; }
+uint64_t gpsm_graph_device_bytes(const gpsm_graph* g) {
+  if (!g) return 0;
+  const uint64_t directed = g->directed.row ? 4 * ((uint64_t)g->dg.n + 1 + 2 * g->directed.arcs) : 0;
+  return g->bytes + directed;
+}
 
 int gpsm_set_hub_policy(gpsm_ctx* c, uint32_t min_degree, uint64_t max_bytes) {
   if (!c || !min_degree) return GPSM_EINVAL;

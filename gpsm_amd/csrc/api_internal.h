@@ -120,9 +120,29 @@ struct gpsm_ctx {
   uint64_t hub_max_bytes = 8ull << 30;
 };
 
+namespace gpsm {
+
+// Directed, edge-labelled out-adjacency of a resident graph, rows sorted by
+// (target, label): only the final-set check reads it.
+struct DirectedGraph {
+  uint32_t* row = nullptr;    // [n+1]
+  uint32_t* col = nullptr;    // [arcs]
+  uint32_t* label = nullptr;  // [arcs]
+  uint64_t arcs = 0;
+  void release() {
+    hipFree(row);
+    hipFree(col);
+    hipFree(label);
+    *this = DirectedGraph();
+  }
+};
+
+}  // namespace gpsm
+
 struct gpsm_graph {
   gpsm_ctx* ctx = nullptr;
   gpsm::DevGraph dg;
   gpsm::GraphStats stats;
   uint64_t bytes = 0;
+  gpsm::DirectedGraph directed;
 };

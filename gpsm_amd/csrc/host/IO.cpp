@@ -71,6 +71,12 @@ bool IO::input(std::vector<Graph*>& query_list) {
   }
 }
 
+Graph* IO::read_data() {
+  Graph* g = input(dfp);
+  if (g) g->transformToCSR();
+  return g;
+}
+
 bool IO::input(Graph*& data_graph) {
   data_graph = input(dfp);
   if (!data_graph) return false;

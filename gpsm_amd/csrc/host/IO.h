@@ -20,6 +20,9 @@ class IO {
   bool input(std::vector<Graph*>& query_list);
   // next data graph (CSR built); false at the terminator or on error
   bool input(Graph*& data_graph);
+  // the same without counting it (data ids are assigned with next_data):
+  // safe on another thread while this one writes output
+  Graph* read_data();
   // next graph of a stream; nullptr at the terminator, sets error() on failure
   Graph* input(FILE* fp);
   bool error() const { return err; }

@@ -2,13 +2,16 @@
 
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <map>
+#include <set>
 #include <vector>
 
 namespace {
 
 gpsm_ctx* g_ctx = nullptr;
 std::map<const Graph*, gpsm_graph*> g_resident;  // data graphs uploaded once
+std::set<const gpsm_graph*> g_directed;          // ... with their out-lists attached
 
 // The query as the reference lists it: every in-edge of every vertex
 // (io/IO.cpp:77-86 builds in/out lists in file order).
@@ -47,6 +50,7 @@ bool Match::initGPU(int dev) {
 void Match::releaseGPU() {
   for (auto& kv : g_resident) gpsm_graph_free(kv.second);
   g_resident.clear();
+  g_directed.clear();
   gpsm_shutdown(g_ctx);
   g_ctx = nullptr;
 }
@@ -54,6 +58,7 @@ void Match::releaseGPU() {
 void Match::forget(const Graph* data) {
   auto it = g_resident.find(data);
   if (it == g_resident.end()) return;
+  g_directed.erase(it->second);
   gpsm_graph_free(it->second);
   g_resident.erase(it);
 }
@@ -117,4 +122,45 @@ void Match::match(IO&, unsigned*& final_result, unsigned& result_row_num, unsign
     std::memcpy(final_result, r.tuples, sizeof(unsigned) * (size_t)result_row_num * result_col_num);
   }
   gpsm_result_free(&r);
+}
+
+int Match::verify_directed(unsigned* final_result, unsigned& result_row_num) {
+  if (!g_ctx) return status = GPSM_ENODEV;
+  gpsm_graph* g = resident();
+  if (!g) return status;
+  if (!g_directed.count(g)) {
+    const unsigned n = (unsigned)data->vertices.size();
+    if (n != data->vertex_num) return status = GPSM_EINVAL;  // no in/out lists (a binary cache)
+    std::vector<uint32_t> row(n + 1, 0), col, lab;
+    std::vector<std::pair<uint32_t, uint32_t>> arcs;
+    for (unsigned v = 0; v < n; ++v) {
+      arcs.clear();
+      for (const Neighbor& nb : data->vertices[v].out) arcs.emplace_back((uint32_t)nb.vid, (uint32_t)nb.elb);
+      std::sort(arcs.begin(), arcs.end());
+      for (const auto& a : arcs) {
+        col.push_back(a.first);
+        lab.push_back(a.second);
+      }
+      row[v + 1] = (uint32_t)col.size();
+    }
+    if ((status = gpsm_graph_set_directed(g, row.data(), col.data(), lab.data())) != GPSM_OK) return status;
+    g_directed.insert(g);
+  }
+  QueryArrays qa(query);
+  std::vector<uint32_t> elabel;
+  for (const Vertex& v : query->vertices)
+    for (const Neighbor& nb : v.out) elabel.push_back((uint32_t)nb.elb);
+  const unsigned q = query->vertex_num;
+  std::vector<uint8_t> keep(result_row_num);
+  status = gpsm_verify_directed(g_ctx, g, q, qa.q.m, qa.src.data(), qa.dst.data(), elabel.data(), final_result,
+                                result_row_num, keep.data(), nullptr);
+  if (status != GPSM_OK) return status;
+  unsigned w = 0;
+  for (unsigned r = 0; r < result_row_num; ++r)
+    if (keep[r]) {
+      if (w != r) std::memmove(final_result + (size_t)w * q, final_result + (size_t)r * q, sizeof(unsigned) * q);
+      ++w;
+    }
+  result_row_num = w;
+  return status;
 }

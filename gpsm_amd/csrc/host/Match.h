@@ -22,6 +22,11 @@ class Match {
   // count-only variant: embedding count and set hash, nothing materialised
   int count(uint64_t& embeddings, uint64_t& hash);
   void match(IO& io, unsigned*& final_result, unsigned& result_row_num, unsigned& result_col_num, int*& id_map);
+  // The reference's directed, edge-labelled final check (io/IO.cpp:146-180)
+  // on the device: keeps the rows of final_result (as match returns them)
+  // whose query arcs are data arcs with the same label, in place.  Needs the
+  // data graph's in/out lists (a text input, not a binary cache).
+  int verify_directed(unsigned* final_result, unsigned& result_row_num);
   // Selects the device; returns false (never exits) when none is usable.
   static bool initGPU(int dev);
   static void releaseGPU();

@@ -3,12 +3,17 @@
 // data graph of the data file and writes the embeddings; "match used" covers
 // matching and output of all queries of a data graph, excluding reading and
 // CSR construction, as in the reference.  With --count, only embedding
-// counts and set hashes are printed (no tuples are materialised).
+// counts and set hashes are printed (no tuples are materialised).  With
+// --directed, embeddings are written only when every query arc maps to a
+// data arc with the same edge label (the reference's IO::verify,
+// io/IO.cpp:146-180, done on the device); without it the semantics are
+// undirected and edge labels are ignored.
 #include <sys/time.h>
 
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <future>
 #include <string>
 #include <vector>
 
@@ -52,16 +57,17 @@ static int write_cache(const std::string& data, const std::string& out) {
 }
 
 int main(int argc, const char* argv[]) {
-  bool count_only = false;
+  bool count_only = false, directed = false;
   std::vector<std::string> args;
   for (int i = 1; i < argc; ++i) {
     if (std::strcmp(argv[i], "--count") == 0) count_only = true;
+    else if (std::strcmp(argv[i], "--directed") == 0) directed = true;
     else args.push_back(argv[i]);
   }
   if (args.size() == 3 && args[0] == "--write-cache") return write_cache(args[1], args[2]);
   if (args.size() < 2 || args.size() > 4) {
     std::fprintf(stderr,
-                 "usage: GpSM [--count] <data> <query> [output=ans.txt] [dev=0]\n"
+                 "usage: GpSM [--count | --directed] <data> <query> [output=ans.txt] [dev=0]\n"
                  "       GpSM --write-cache <data.g> <out.csr>\n"
                  "<data> is a text graph stream or a binary CSR cache (one data graph).\n");
     return 2;
@@ -69,6 +75,10 @@ int main(int argc, const char* argv[]) {
   const std::string output = args.size() >= 3 ? args[2] : "ans.txt";
   const int dev = args.size() == 4 ? std::atoi(args[3].c_str()) : 0;
   const bool cache = is_cache(args[0]);
+  if (directed && (count_only || cache)) {
+    std::fprintf(stderr, "--directed needs the text data format and tuples (not --count)\n");
+    return 2;
+  }
   if (!Match::initGPU(dev)) return 1;
   // a cache is read below; the text reader then only serves the queries
   IO io(args[1], cache ? args[1] : args[0], output);
@@ -97,6 +107,11 @@ int main(int argc, const char* argv[]) {
       int* id_map = nullptr;
       m.match(io, result, rows, cols, id_map);
       if (m.last_status() != GPSM_OK) rc = 1;
+      else if (directed && rows && m.verify_directed(result, rows) != GPSM_OK) {
+        std::fprintf(stderr, "verify: %s\n", gpsm_strerror(m.last_status()));
+        rc = 1;
+        rows = 0;
+      }
       io.output(result, rows, cols, id_map, queries[i], data);
       io.flush();
       delete[] result;
@@ -118,8 +133,18 @@ int main(int argc, const char* argv[]) {
     io.next_data();
     run_data(data);
   } else {
-    Graph* data = nullptr;
-    while (io.input(data)) run_data(data);
+    // the next data graph is read and its CSR built on another thread while
+    // the device matches the current one (the reference reads, builds and
+    // matches them one after another, main/run.cpp:60-98)
+    auto read = [&io]() { return io.read_data(); };
+    std::future<Graph*> next = std::async(std::launch::async, read);
+    for (;;) {
+      Graph* data = next.get();
+      if (!data) break;
+      next = std::async(std::launch::async, read);
+      io.next_data();
+      run_data(data);
+    }
     if (io.error()) {
       std::fprintf(stderr, "data file: malformed or missing terminator\n");
       rc = 1;

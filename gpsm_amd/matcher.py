@@ -231,6 +231,41 @@ class ResidentGraph:
     def device_bytes(self):
         return int(_lib.lib().gpsm_graph_device_bytes(self.h))
 
+    def set_directed(self, src, dst, elabel):
+        """Attaches the directed, edge-labelled arcs src[i] -> dst[i] (label
+        elabel[i]) of the data graph, the out-lists that Graph::isEdgeContained
+        scans (graph/Graph.cpp:107-118), for verify_directed."""
+        src = np.asarray(src, dtype=np.uint32)
+        dst = np.asarray(dst, dtype=np.uint32)
+        lab = np.asarray(elabel, dtype=np.uint32)
+        nn, adj = C.c_uint32(), C.c_uint64()
+        check(_lib.lib().gpsm_graph_info(self.h, C.byref(nn), C.byref(adj)), "gpsm_graph_info")
+        n = nn.value
+        if len(src) and int(max(src.max(), dst.max())) >= n:
+            raise ValueError("arc endpoint >= n")
+        order = np.lexsort((lab, dst, src))
+        row = np.zeros(n + 1, dtype=np.uint32)
+        np.cumsum(np.bincount(src, minlength=n), out=row[1:])
+        col = np.ascontiguousarray(dst[order])
+        lab = np.ascontiguousarray(lab[order])
+        check(_lib.lib().gpsm_graph_set_directed(self.h, ptr(row), ptr(col), ptr(lab)), "gpsm_graph_set_directed")
+
+    def verify_directed(self, q, arcs, tuples):
+        """IO::verify (io/IO.cpp:146-180) on the device: a bool mask over the
+        rows of tuples (k x q.n, column j = query vertex j) that are injective
+        and map every query arc (src, dst, label) of arcs to a data arc with
+        that label.  Needs set_directed first."""
+        a = np.asarray(arcs, dtype=np.uint32).reshape(-1, 3)
+        src, dst, lab = (np.ascontiguousarray(a[:, i]) for i in range(3))
+        t = np.ascontiguousarray(np.asarray(tuples, dtype=np.uint32).reshape(-1, q.n))
+        keep = np.zeros(max(len(t), 1), dtype=np.uint8)
+        kept = C.c_uint64()
+        check(_lib.lib().gpsm_verify_directed(self.ctx.h, self.h, q.n, len(a), ptr(src), ptr(dst), ptr(lab), ptr(t),
+                                              len(t), ptr(keep), C.byref(kept)), "gpsm_verify_directed")
+        mask = keep[:len(t)].astype(bool)
+        assert int(mask.sum()) == kept.value
+        return mask
+
     def cet_edges(self, q):
         """Query edges (from < to, the planner's edge ids) and |CET| of each
         (gpsm_candidate_edges without the lists)."""

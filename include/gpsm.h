@@ -138,6 +138,25 @@ int gpsm_set_hub_policy(gpsm_ctx* ctx, uint32_t min_degree, uint64_t max_bytes);
  * a vertex is one (0 when there are none). */
 int gpsm_graph_hubs(const gpsm_graph* g, uint32_t* nhub, uint32_t* hub_degree);
 
+/* --- directed, edge-labelled final check -------------------------------- */
+
+/* Attaches the directed, edge-labelled out-adjacency of the data graph to a
+ * resident graph (Graph::vertices[v].out, graph/Graph.h:91-158, as read by
+ * io/IO.cpp:77-86): out_row[n+1], out_col/out_label[out_row[n]], each row
+ * sorted by (target, label); duplicate arcs and self-loops allowed.  Replaces
+ * an earlier attachment.  GPSM_EINVAL on unsorted rows or targets >= n. */
+int gpsm_graph_set_directed(gpsm_graph* g, const uint32_t* out_row, const uint32_t* out_col,
+                            const uint32_t* out_label);
+/* Replaces IO::verify (io/IO.cpp:146-180) with Graph::isEdgeContained
+ * (graph/Graph.cpp:107-118) on the device: tuple i (tuples[i*q .. i*q+q),
+ * column j = query vertex j) is kept iff its q vertices are distinct and for
+ * every query arc e the data graph has the arc tuple[src[e]] -> tuple[dst[e]]
+ * with label elabel[e].  keep[n_tuples] receives 1/0, *kept (may be NULL)
+ * the number kept.  Needs gpsm_graph_set_directed first (GPSM_EINVAL). */
+int gpsm_verify_directed(gpsm_ctx* ctx, const gpsm_graph* g, uint32_t q, uint32_t m, const uint32_t* src,
+                         const uint32_t* dst, const uint32_t* elabel, const uint32_t* tuples, uint64_t n_tuples,
+                         uint8_t* keep, uint64_t* kept);
+
 /* --- matching ----------------------------------------------------------- */
 
 /* Replaces Match::match (match/Match.cu:1710-1966) together with the

@@ -197,3 +197,21 @@ def test_cli_writes_the_binary_cache(tmp_path):
     bad = subprocess.run([exe, "--write-cache", str(tmp_path / "missing.g"), str(out)], capture_output=True,
                          timeout=60)
     assert bad.returncode != 0
+
+
+def test_cli_directed_refuses_count_and_cache(tmp_path):
+    """--directed needs tuples and the text format's arc lists: with --count
+    or a binary cache the driver stops with a usage error before touching a
+    device."""
+    g = G.small_graph(5, n=300, m=900, labels=2, kind="er")
+    p = tmp_path / "data.g"
+    src, dst = g.edges()
+    _lib.lib().gpsm_io_write_graph(str(p).encode(), 0, 0, g.n, _lib.ptr(g.labels), len(src), _lib.ptr(src),
+                                   _lib.ptr(dst))
+    _lib.lib().gpsm_io_write_end(str(p).encode())
+    cache = tmp_path / "data.csr"
+    G.save_csr(g, cache)
+    exe = os.path.join(os.path.dirname(_lib.LIB_PATH), "GpSM")
+    for args in (["--directed", "--count", str(p), str(p)], ["--directed", str(cache), str(p)]):
+        r = subprocess.run([exe] + args + [str(tmp_path / "ans.txt")], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2 and "--directed" in r.stderr, r.stderr

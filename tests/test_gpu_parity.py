@@ -351,3 +351,36 @@ def test_cli_counts_from_text_and_from_cache_agree(tmp_path):
     for line, q in zip(outs[0], qs):
         ref = O.o1_enumerate(g, q)
         assert line.endswith("%d embeddings, hash %016x" % (ref["count"], ref["hash"])), line
+
+
+def test_cli_matches_every_data_graph_of_a_file(tmp_path):
+    """A data file of several graphs (the reference's main loop,
+    main/run.cpp:60-98): the driver reads the next graph on another thread
+    while the current one is matched, and frees each graph's device copy
+    before the next can reuse its address; every graph's counts equal the
+    enumerator's."""
+    import os
+    import subprocess
+    graphs = [G.small_graph(20 + i, n=1500 + 500 * i, m=6000 + 1000 * i, labels=3, kind="cl") for i in range(3)]
+    data, query = tmp_path / "d.g", tmp_path / "q.g"
+    for i, g in enumerate(graphs):
+        src, dst = g.edges()
+        _lib.lib().gpsm_io_write_graph(str(data).encode(), i, i, g.n, _lib.ptr(g.labels), len(src),
+                                       _lib.ptr(src), _lib.ptr(dst))
+    _lib.lib().gpsm_io_write_end(str(data).encode())
+    qs = G.queries(graphs[0], 3, 3, 4, _lib.Q_WALK_HALF, 41, 10 ** 6)
+    for i, q in enumerate(qs):
+        q.struct()
+        _lib.lib().gpsm_io_write_graph(str(query).encode(), i, i, q.n, _lib.ptr(q.labels), q.m, _lib.ptr(q.src),
+                                       _lib.ptr(q.dst))
+    _lib.lib().gpsm_io_write_end(str(query).encode())
+    exe = os.path.join(os.path.dirname(_lib.LIB_PATH), "GpSM")
+    r = subprocess.run([exe, "--count", str(data), str(query), str(tmp_path / "ans.txt")], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = [l for l in r.stdout.splitlines() if l.startswith("query ")]
+    assert len(lines) == len(graphs) * len(qs)
+    for gi, g in enumerate(graphs):
+        for qi, q in enumerate(qs):
+            ref = O.o1_enumerate(g, q)
+            assert lines[gi * len(qs) + qi] == "query %d: %d embeddings, hash %016x" % (qi, ref["count"], ref["hash"])
