@@ -258,6 +258,7 @@ void free_graph_arrays(DevGraph& d) {
   hipFree(d.row);
   hipFree(d.col);
   hipFree(d.key);
+  hipFree(d.ckey);
   hipFree(d.fchunk);
   hipFree(d.hub_of);
   hipFree(d.hub_bits);
@@ -300,18 +301,20 @@ int finish_graph(gpsm_ctx* c, gpsm_graph* g, const uint32_t* row_offset, const u
   filter_chunks(n, row_offset, fch);
   d.nfchunk = (uint32_t)fch.size() - 1;
   if (hipMalloc(&d.key, sizeof(uint32_t) * n) != hipSuccess ||
+      hipMalloc(&d.ckey, sizeof(uint32_t) * std::max<uint64_t>(d.adj, 1)) != hipSuccess ||
       hipMalloc(&d.fchunk, sizeof(uint32_t) * fch.size()) != hipSuccess) {
     (void)hipGetLastError();
     return fail(GPSM_ENOMEM);
   }
   if (hipMemcpyAsync(d.key, key.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
       hipMemcpyAsync(d.fchunk, fch.data(), sizeof(uint32_t) * fch.size(), hipMemcpyHostToDevice, c->stream) !=
-          hipSuccess ||
-      hipStreamSynchronize(c->stream) != hipSuccess) {
+          hipSuccess) {
     (void)hipGetLastError();
     return fail(GPSM_EDEVICE);
   }
-  g->bytes = sizeof(uint32_t) * ((uint64_t)n + 1 + d.adj + n + fch.size());
+  launch_neighbour_keys(d, c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess || hipGetLastError() != hipSuccess) return fail(GPSM_EDEVICE);
+  g->bytes = sizeof(uint32_t) * ((uint64_t)n + 1 + 2 * d.adj + n + fch.size());
   if (int rc = build_hubs(c, row_offset, g)) return fail(rc);
   return GPSM_OK;
 }
@@ -581,6 +584,8 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       (rc = upload(c, B_NBR_LO, nbr_lo)) || (rc = upload(c, B_NBR_HI, nbr_hi)))
     return rc;
   if ((rc = c->bufs[B_COUNTS].ensure(sizeof(uint32_t) * std::max<uint32_t>(nslots, 1)))) return rc;
+  if ((rc = c->bufs[B_SEGC].ensure(sizeof(uint32_t) * std::max<uint64_t>((uint64_t)nslots * segs_of(W), 1))))
+    return rc;
   const size_t fstat_step = (size_t)STAT_COPIES * STAT_STRIDE;
   if ((rc = c->bufs[B_FSTATS].ensure(sizeof(unsigned long long) * fstat_step * (maxsteps + 1)))) return rc;
   uint32_t* bitmaps = dptr<uint32_t>(c, B_BITMAPS);
@@ -605,7 +610,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
                          counters ? fstats + fstat_step * sidx : nullptr, s);
       pend(K_FILTER, a, 4.0 * W * nops_s, counters ? (int)sidx : -1);
     }
-    launch_row_popcount(dg, bitmaps, nslots, dptr<uint32_t>(c, B_COUNTS), s);
+    launch_row_popcount(dg, bitmaps, nslots, dptr<uint32_t>(c, B_SEGC), dptr<uint32_t>(c, B_COUNTS), s);
     if (captured) {
       if (hipStreamEndCapture(s, &graph) != hipSuccess || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) !=
                                                               hipSuccess)
@@ -642,8 +647,8 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   if ((rc = c->bufs[B_BPREFIX].ensure(sizeof(uint32_t) * (size_t)std::max<uint32_t>(nslots, 1) * NB))) return rc;
   {
     hipEvent_t a = pbegin();
-    launch_compact_rows(dg, bitmaps, nslots, dptr<uint64_t>(c, B_COFF), dptr<uint32_t>(c, B_CLIST),
-                        dptr<uint32_t>(c, B_BPREFIX), s);
+    launch_compact_rows(dg, bitmaps, nslots, dptr<uint32_t>(c, B_SEGC), dptr<uint64_t>(c, B_COFF),
+                        dptr<uint32_t>(c, B_CLIST), dptr<uint32_t>(c, B_BPREFIX), s);
     pend(K_COMPACT, a, (double)nslots * 4.0 * (W + NB) + 4.0 * coff[nslots], -1);
   }
   std::vector<CetDesc> fdesc;
@@ -930,8 +935,9 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
           unsigned long long f[5] = {0, 0, 0, 0, 0};
           for (uint32_t cp = 0; cp < STAT_COPIES; ++cp)
             for (int i = 0; i < 5; ++i) f[i] += fs[fstat_step * pnd.stats_slot + cp * STAT_STRIDE + i];
-          // candidates: two row offsets; entries: column index; key gathers;
-          // bitmap probes and bit writes count one bit each
+          // candidates: two row offsets; entries: column index or neighbour
+          // key; neighbour keys read beside the column index (explore's
+          // second pass); bitmap probes and bit writes count one bit each
           bytes += 8.0 * f[0] + 4.0 * f[1] + 4.0 * f[2] + (f[3] + f[4]) / 8.0;
         }
         Prof::Acc& acc = c->prof.acc[pnd.kind];

@@ -47,6 +47,7 @@ enum BufId {
   B_TERMS,       // hash terms of the last join level's lists (u64)
   B_TERM_DESC,   // tables that have them
   B_TERM_START,  // their ranges
+  B_SEGC,        // popcounts of the candidate bitmaps' segments
   B_COUNT_
 };
 

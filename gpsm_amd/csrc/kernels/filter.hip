@@ -57,17 +57,16 @@ struct WaveLds {
   uint32_t lim[WAVE];
 };
 
-// Neighbour constraints of one adjacency entry mv: refine probes C(v_i)
-// for the constraints not yet met; explore compares mv's key with v_i's
-// label/degree range.
+// Neighbour constraints of one adjacency entry: refine probes C(v_i) with
+// the entry's vertex mv for the constraints not yet met; explore compares
+// the entry's key (mv holds ckey[e] then) with v_i's label/degree range.
 __device__ __forceinline__ unsigned long long constraint_bits(uint32_t type, uint32_t nv, uint32_t nb, uint32_t mv,
                                                               unsigned long long known,
-                                                              const uint32_t* __restrict__ key,
                                                               const uint32_t* __restrict__ nbr_slot,
                                                               const uint32_t* __restrict__ nbr_lo,
                                                               const uint32_t* __restrict__ nbr_hi,
                                                               const uint32_t* __restrict__ bitmaps, uint32_t W,
-                                                              uint32_t& t_key, uint32_t& t_probe) {
+                                                              uint32_t& t_probe) {
   unsigned long long bits = 0;
   if (type == OP_REFINE) {
     for (uint32_t i = 0; i < nv; ++i) {
@@ -76,8 +75,7 @@ __device__ __forceinline__ unsigned long long constraint_bits(uint32_t type, uin
       if (bit_of(bitmaps + (size_t)nbr_slot[nb + i] * W, mv)) bits |= 1ULL << i;
     }
   } else {
-    ++t_key;
-    const uint32_t k = key[mv];
+    const uint32_t k = mv;  // read in place of the column index: no extra bytes
     for (uint32_t i = 0; i < nv; ++i)
       if (k >= nbr_lo[nb + i] && k <= nbr_hi[nb + i]) bits |= 1ULL << i;
   }
@@ -86,7 +84,7 @@ __device__ __forceinline__ unsigned long long constraint_bits(uint32_t type, uin
 
 __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __restrict__ row,
                                                           const uint32_t* __restrict__ col,
-                                                          const uint32_t* __restrict__ key, uint32_t W,
+                                                          const uint32_t* __restrict__ ckey, uint32_t W,
                                                           const uint32_t* __restrict__ fchunk, uint32_t chunks,
                                                           const FilterOp* __restrict__ ops, uint32_t nops,
                                                           const uint32_t* __restrict__ nbr_slot,
@@ -161,6 +159,8 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
         L.pre[lane] = rpre;
         L.base[lane] = off + cur;
         wave_sync();
+        // refine needs the neighbour ids, explore only their keys
+        const uint32_t* __restrict__ src = type == OP_REFINE ? col : ckey;
         uint32_t mvs[UNROLL], cis[UNROLL];
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
           if (g < T) {
             const uint32_t ci = seg_search(L.pre, WAVE, g);
             cis[u] = ci;
-            mvs[u] = col[L.base[ci] + (g - L.pre[ci])];
+            mvs[u] = src[L.base[ci] + (g - L.pre[ci])];
           }
         }
 #pragma unroll
@@ -177,8 +177,7 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
           if (cis[u] == WAVE) continue;
           ++t_ent;
           const unsigned long long bits =
-              constraint_bits(type, nv, nb, mvs[u], L.hit[cis[u]], key, nbr_slot, nbr_lo, nbr_hi, bitmaps, W, t_key,
-                              t_probe);
+              constraint_bits(type, nv, nb, mvs[u], L.hit[cis[u]], nbr_slot, nbr_lo, nbr_hi, bitmaps, W, t_probe);
           if (bits) atomicOr(&L.hit[cis[u]], bits);
         }
         wave_sync();
@@ -205,7 +204,7 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
         wave_sync();
         // pass 2: every matching data neighbour of a surviving prefix enters C(v_i)
         for (uint32_t f = 0; f < T; f += UNROLL * WAVE) {
-          uint32_t mvs[UNROLL], cis[UNROLL];
+          uint32_t mvs[UNROLL], ks[UNROLL], cis[UNROLL];
 #pragma unroll
           for (int u = 0; u < UNROLL; ++u) {
             const uint32_t g = f + u * WAVE + lane;
@@ -213,13 +212,15 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
             if (g < T) {
               const uint32_t ci = seg_search(L.pre, WAVE, g);
               cis[u] = ci;
-              mvs[u] = col[L.base[ci] + (g - L.pre[ci])];
+              const uint32_t e = L.base[ci] + (g - L.pre[ci]);
+              mvs[u] = col[e];
+              ks[u] = ckey[e];
             }
           }
 #pragma unroll
           for (int u = 0; u < UNROLL; ++u) {
             if (cis[u] == WAVE) continue;
-            const uint32_t k = key[mvs[u]];
+            const uint32_t k = ks[u];
             const uint32_t lm = L.lim[cis[u]];
             ++t_ent;
             ++t_key;
@@ -272,7 +273,7 @@ void launch_filter_step(const DevGraph& g, const FilterOp* ops, uint32_t nops, c
   uint64_t grid = (items + wpb - 1) / wpb;
   if (grid > 65536) grid = 65536;
   const size_t lds = sizeof(WaveLds) * wpb;
-  hipLaunchKernelGGL(filter_step_kernel, dim3((uint32_t)grid), dim3(block), lds, s, g.row, g.col, g.key, g.W,
+  hipLaunchKernelGGL(filter_step_kernel, dim3((uint32_t)grid), dim3(block), lds, s, g.row, g.col, g.ckey, g.W,
                      g.fchunk, g.nfchunk, ops, nops, nbr_slot, nbr_lo, nbr_hi, bitmaps, stats);
 }
 

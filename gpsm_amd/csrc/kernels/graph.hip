@@ -1,5 +1,8 @@
 // Graph-resident structures built once at upload.
 //
+// Neighbour keys: ckey[e] = key[col[e]], the label/degree key of every
+// adjacency entry's target, stored beside col (4 bytes per entry).
+//
 // Hub adjacency bitmaps: one n-bit bitmap per hub (a vertex above the hub
 // degree), so that the join's edge-membership test of a pair touching a hub
 // is one bit instead of a binary search in an adjacency list of up to 10^5
@@ -33,7 +36,20 @@ __global__ void __launch_bounds__(256) hub_bits_kernel(const uint32_t* __restric
   }
 }
 
+__global__ void neighbour_keys_kernel(const uint32_t* __restrict__ col, const uint32_t* __restrict__ key,
+                                     uint64_t adj, uint32_t* __restrict__ ckey) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < adj; e += stride) ckey[e] = key[col[e]];
+}
+
 }  // namespace
+
+void launch_neighbour_keys(const DevGraph& g, hipStream_t s) {
+  if (!g.adj) return;
+  uint64_t grid = (g.adj + 255) / 256;
+  if (grid > 16384) grid = 16384;
+  hipLaunchKernelGGL(neighbour_keys_kernel, dim3((uint32_t)grid), dim3(256), 0, s, g.col, g.key, g.adj, g.ckey);
+}
 
 void launch_hub_bits(const DevGraph& g, const uint32_t* hubs, const uint32_t* block_base, uint32_t nblocks,
                      hipStream_t s) {

@@ -17,6 +17,9 @@ struct DevGraph {
   uint32_t* row = nullptr;
   uint32_t* col = nullptr;
   uint32_t* key = nullptr;
+  // ckey[e] = key[col[e]]: the filter's label/degree tests stream the keys
+  // of a row's neighbours beside the row instead of gathering them
+  uint32_t* ckey = nullptr;
   // filter work items: vertex ranges [fchunk[i], fchunk[i+1]) of at most
   // FCHUNK_VERTS vertices and FCHUNK_ENTRIES adjacency entries (a vertex
   // with more entries is a range of its own), so that no wave of the filter
@@ -48,6 +51,7 @@ int build_csr_from_edges(uint32_t n, uint64_t m, const uint32_t* src, const uint
 // Sets the adjacency bitmaps of the hubs: hub_bits (nhub * W words) must be
 // zero; hubs[h] is the vertex of bitmap h; block_base[h] = 64-entry blocks
 // of the adjacency of hubs before h (nhub + 1 entries, last = nblocks).
+void launch_neighbour_keys(const DevGraph& g, hipStream_t s);
 void launch_hub_bits(const DevGraph& g, const uint32_t* hubs, const uint32_t* block_base, uint32_t nblocks,
                      hipStream_t s);
 
@@ -65,17 +69,25 @@ void launch_filter_step(const DevGraph& g, const FilterOp* ops, uint32_t nops, c
                         unsigned long long* stats, hipStream_t s);
 
 // scan.hip ---------------------------------------------------------------------
-// counts[r] = popcount of bitmap row rows[r] (all rows when rows == nullptr).
-void launch_row_popcount(const DevGraph& g, const uint32_t* bitmaps, uint32_t nrows, uint32_t* counts,
-                         hipStream_t s);
-// Compaction of bitmap rows to ascending id lists plus the rank index.
-void launch_compact_rows(const DevGraph& g, const uint32_t* bitmaps, uint32_t nrows, const uint64_t* coff,
-                         uint32_t* clist, uint32_t* bprefix, hipStream_t s);
+// Bitmap rows are compacted in segments of SEG_WORDS words, one workgroup
+// per (row, segment).
+constexpr uint32_t SEG_WORDS = 2048;
+inline uint32_t segs_of(uint32_t W) { return (W + SEG_WORDS - 1) / SEG_WORDS; }
+// segc[r * segs_of(W) + i] = popcount of segment i of bitmap row r;
+// counts[r] = popcount of row r.
+void launch_row_popcount(const DevGraph& g, const uint32_t* bitmaps, uint32_t nrows, uint32_t* segc,
+                         uint32_t* counts, hipStream_t s);
+// Compaction of bitmap rows to ascending id lists (row r at clist + coff[r])
+// plus the rank index, from the segment counts of launch_row_popcount.
+void launch_compact_rows(const DevGraph& g, const uint32_t* bitmaps, uint32_t nrows, const uint32_t* segc,
+                         const uint64_t* coff, uint32_t* clist, uint32_t* bprefix, hipStream_t s);
 // Exclusive scan of n u32 values; out may alias in.  total (device, u64) optional.
 void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, unsigned long long* total,
                            uint32_t* scratch, uint64_t scratch_words, hipStream_t s);
 uint64_t scan_scratch_words(uint64_t n);
-// Ascending ids of set bits of one bitmap; count written to *count (device u64).
+// Ascending ids of set bits of one bitmap; count written to *count (device
+// u64).  scratch: compact_scratch_words(nbits) words, 8-byte aligned.
+uint64_t compact_scratch_words(uint64_t nbits);
 void launch_compact_bitmap(const uint32_t* bits, uint64_t nbits, uint32_t* ids, unsigned long long* count,
                            uint32_t* scratch, hipStream_t s);
 

@@ -3,12 +3,17 @@
 // match/Match.cu:463-526) and its unbuilt Blelloch prefix scan / stream
 // compaction (util/prefix_scan.cu:158-427, util/compact_scan.cu:7-70).
 //
-// Candidate sets are bitmaps here, so "collect" is a popcount scan: a
-// workgroup walks one bitmap row in tiles of 256 words, each thread
-// popcounts one word, a wave64 shuffle scan plus a per-wave LDS carry gives
-// every word its output position, ids are staged in LDS and written back
-// coalesced.  The same pass writes the rank index (set bits before each
-// 256-bit block) the join uses to find a vertex's candidate-edge list.
+// Candidate sets are bitmaps here, so "collect" is a popcount scan.  Each
+// bitmap row is cut into segments of SEG_WORDS words, one workgroup per
+// (row, segment), so that a batch of a few hundred rows still fills 256 CUs:
+// a first pass popcounts every segment (and sums a row's segments into its
+// candidate count), the second gives a segment its row-relative base from
+// the counts of the segments before it and walks it in tiles of 256 words:
+// each thread popcounts one word, a wave64 shuffle scan plus a per-wave LDS
+// carry gives every word its output position, ids are staged in LDS and
+// written back coalesced.  The same pass writes the rank index (set bits
+// before each 256-bit block, row-relative) the join uses to find a vertex's
+// candidate-edge list.
 #include "launch.h"
 #include "wave.h"
 
@@ -17,36 +22,63 @@ namespace gpsm {
 namespace {
 
 constexpr uint32_t TILE_WORDS = 256;  // one word per thread
+static_assert(SEG_WORDS % TILE_WORDS == 0 && TILE_WORDS % 8 == 0, "segments are whole tiles and rank blocks");
 
-__global__ void __launch_bounds__(256) row_popcount_kernel(const uint32_t* __restrict__ bitmaps, uint32_t W,
-                                                           uint32_t* __restrict__ counts) {
+__global__ void __launch_bounds__(256) seg_popcount_kernel(const uint32_t* __restrict__ bitmaps, uint32_t W,
+                                                           uint32_t nseg, uint32_t* __restrict__ segc) {
   __shared__ uint32_t part[4];
-  const uint32_t* bm = bitmaps + (size_t)blockIdx.x * W;
+  const uint32_t r = blockIdx.x / nseg, sg = blockIdx.x % nseg;
+  const uint32_t* bm = bitmaps + (size_t)r * W;
+  const uint32_t base = sg * SEG_WORDS;
   uint32_t c = 0;
-  for (uint32_t i = threadIdx.x; i < W; i += blockDim.x) c += __popc(bm[i]);
+#pragma unroll
+  for (uint32_t j = 0; j < SEG_WORDS / 256; ++j) {
+    const uint32_t i = base + j * 256 + threadIdx.x;
+    if (i < W) c += __popc(bm[i]);
+  }
   c = wave_sum(c);
   if (lane_id() == 0) part[threadIdx.x / WAVE] = c;
   __syncthreads();
-  if (threadIdx.x == 0) counts[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+  if (threadIdx.x == 0) segc[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+__global__ void row_totals_kernel(const uint32_t* __restrict__ segc, uint32_t nseg, uint32_t nrows,
+                                  uint32_t* __restrict__ counts) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  uint32_t t = 0;
+  for (uint32_t i = 0; i < nseg; ++i) t += segc[(size_t)r * nseg + i];
+  counts[r] = t;
 }
 
 __global__ void __launch_bounds__(256) compact_rows_kernel(const uint32_t* __restrict__ bitmaps, uint32_t W,
-                                                           uint32_t NB, const uint64_t* __restrict__ coff,
+                                                           uint32_t NB, uint32_t nseg,
+                                                           const uint32_t* __restrict__ segc,
+                                                           const uint64_t* __restrict__ coff,
                                                            uint32_t* __restrict__ clist,
                                                            uint32_t* __restrict__ bprefix) {
   __shared__ uint32_t stage[TILE_WORDS * 32];
   __shared__ uint32_t tmp[8];
-  const uint32_t r = blockIdx.x;
+  __shared__ uint32_t rel;
+  const uint32_t r = blockIdx.x / nseg, sg = blockIdx.x % nseg;
   const uint32_t* bm = bitmaps + (size_t)r * W;
   uint32_t* bp = bprefix + (size_t)r * NB;
   uint32_t* out = clist + coff[r];
-  uint32_t carry = 0;
-  for (uint32_t t0 = 0; t0 < W; t0 += TILE_WORDS) {
+  if (threadIdx.x < WAVE) {  // set bits of the row's earlier segments
+    uint32_t x = 0;
+    for (uint32_t i = threadIdx.x; i < sg; i += WAVE) x += segc[(size_t)r * nseg + i];
+    x = wave_sum(x);
+    if (threadIdx.x == 0) rel = x;
+  }
+  __syncthreads();
+  uint32_t carry = rel;
+  const uint32_t end = min(W, (sg + 1) * SEG_WORDS);
+  for (uint32_t t0 = sg * SEG_WORDS; t0 < end; t0 += TILE_WORDS) {
     const uint32_t wi = t0 + threadIdx.x;
-    const uint32_t w = wi < W ? bm[wi] : 0u;
+    const uint32_t w = wi < end ? bm[wi] : 0u;
     uint32_t total;
     const uint32_t ex = block_excl_scan(__popc(w), tmp, &total);
-    if ((wi & 7) == 0 && wi < W) bp[wi >> 3] = carry + ex;
+    if ((wi & 7) == 0 && wi < end) bp[wi >> 3] = carry + ex;
     uint32_t pos = ex, x = w;
     while (x) {
       const uint32_t b = __ffs(x) - 1;
@@ -58,7 +90,7 @@ __global__ void __launch_bounds__(256) compact_rows_kernel(const uint32_t* __res
     carry += total;
     __syncthreads();
   }
-  if (threadIdx.x == 0) bp[NB - 1] = carry;
+  if (sg == nseg - 1 && threadIdx.x == 0) bp[NB - 1] = carry;
 }
 
 // Reduce-then-scan for generic u32 arrays (wrapping u32 values, u64 total).
@@ -129,16 +161,20 @@ void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
   hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)grid), dim3(256), 0, s, p, v, n);
 }
 
-void launch_row_popcount(const DevGraph& g, const uint32_t* bitmaps, uint32_t nrows, uint32_t* counts,
-                         hipStream_t s) {
+void launch_row_popcount(const DevGraph& g, const uint32_t* bitmaps, uint32_t nrows, uint32_t* segc,
+                         uint32_t* counts, hipStream_t s) {
   if (!nrows) return;
-  hipLaunchKernelGGL(row_popcount_kernel, dim3(nrows), dim3(256), 0, s, bitmaps, g.W, counts);
+  const uint32_t nseg = segs_of(g.W);
+  hipLaunchKernelGGL(seg_popcount_kernel, dim3(nrows * nseg), dim3(256), 0, s, bitmaps, g.W, nseg, segc);
+  hipLaunchKernelGGL(row_totals_kernel, dim3((nrows + 255) / 256), dim3(256), 0, s, segc, nseg, nrows, counts);
 }
 
-void launch_compact_rows(const DevGraph& g, const uint32_t* bitmaps, uint32_t nrows, const uint64_t* coff,
-                         uint32_t* clist, uint32_t* bprefix, hipStream_t s) {
+void launch_compact_rows(const DevGraph& g, const uint32_t* bitmaps, uint32_t nrows, const uint32_t* segc,
+                         const uint64_t* coff, uint32_t* clist, uint32_t* bprefix, hipStream_t s) {
   if (!nrows) return;
-  hipLaunchKernelGGL(compact_rows_kernel, dim3(nrows), dim3(256), 0, s, bitmaps, g.W, g.NB, coff, clist, bprefix);
+  const uint32_t nseg = segs_of(g.W);
+  hipLaunchKernelGGL(compact_rows_kernel, dim3(nrows * nseg), dim3(256), 0, s, bitmaps, g.W, g.NB, nseg, segc, coff,
+                     clist, bprefix);
 }
 
 uint64_t scan_scratch_words(uint64_t n) { return 2 * ((n + SCAN_TILE - 1) / SCAN_TILE) + 2; }
@@ -153,17 +189,26 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, unsign
   hipLaunchKernelGGL(scan_tile_kernel, dim3((uint32_t)nb), dim3(256), 0, s, in, out, n, sums);
 }
 
+uint64_t compact_scratch_words(uint64_t nbits) {
+  const uint64_t W = (nbits + 31) / 32;
+  return 2 + ((W + 7) / 8 + 1) + segs_of((uint32_t)W) + 1;
+}
+
 void launch_compact_bitmap(const uint32_t* bits, uint64_t nbits, uint32_t* ids, unsigned long long* count,
                            uint32_t* scratch, hipStream_t s) {
   // one row of ceil(nbits/32) words through the row compaction; scratch
-  // holds the row offset (2 words) and the rank index.
+  // holds the row offset (2 words), the rank index, the segment counts and
+  // the row count
   DevGraph g;
   g.W = (uint32_t)((nbits + 31) / 32);
   g.NB = (g.W + 7) / 8 + 1;
   uint64_t* coff = reinterpret_cast<uint64_t*>(scratch);
-  hipMemsetAsync(coff, 0, sizeof(uint64_t), s);
   uint32_t* bp = scratch + 2;
-  launch_compact_rows(g, bits, 1, coff, ids, bp, s);
+  uint32_t* segc = bp + g.NB;
+  uint32_t* rowc = segc + segs_of(g.W);
+  hipMemsetAsync(coff, 0, sizeof(uint64_t), s);
+  launch_row_popcount(g, bits, 1, segc, rowc, s);
+  launch_compact_rows(g, bits, 1, segc, coff, ids, bp, s);
   hipMemsetAsync(count, 0, sizeof(unsigned long long), s);
   hipMemcpyAsync(count, bp + g.NB - 1, sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
 }

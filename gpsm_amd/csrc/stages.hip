@@ -186,14 +186,14 @@ int gpsm_compact_bitmap(gpsm_ctx* c, const uint32_t* bits, uint64_t nbits, uint3
   const uint64_t W = (nbits + 31) / 32;
   std::vector<uint32_t> h(bits, bits + W);
   if (nbits & 31) h[W - 1] &= (1u << (nbits & 31)) - 1u;  // bits past nbits are not part of the input
-  const uint64_t NB = (W + 7) / 8 + 1;
+  const uint64_t sw = (compact_scratch_words(nbits) + 1) & ~1ull;  // dcount after it, 8-byte aligned
   if (c->bufs[B_BITMAPS].ensure(sizeof(uint32_t) * W) || c->bufs[B_CLIST].ensure(sizeof(uint32_t) * nbits) ||
-      c->bufs[B_SCRATCH].ensure(sizeof(uint32_t) * (NB + 8)))
+      c->bufs[B_SCRATCH].ensure(sizeof(uint32_t) * (sw + 2)))
     return GPSM_ENOMEM;
   uint32_t* db = reinterpret_cast<uint32_t*>(c->bufs[B_BITMAPS].ptr);
   uint32_t* dids = reinterpret_cast<uint32_t*>(c->bufs[B_CLIST].ptr);
   uint32_t* scr = reinterpret_cast<uint32_t*>(c->bufs[B_SCRATCH].ptr);
-  unsigned long long* dcount = reinterpret_cast<unsigned long long*>(scr + NB + 4);
+  unsigned long long* dcount = reinterpret_cast<unsigned long long*>(scr + sw);
   if (hipMemcpyAsync(db, h.data(), sizeof(uint32_t) * W, hipMemcpyHostToDevice, s) != hipSuccess)
     return GPSM_EDEVICE;
   launch_compact_bitmap(db, nbits, dids, dcount, scr, s);
