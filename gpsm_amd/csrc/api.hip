@@ -663,6 +663,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       d.w_slot = S.vbase + S.hq.et[e];
       d.ncand = counts[d.x_slot];
       d.clist_off = coff[d.x_slot];
+      key_range(S.hq, st, S.hq.et[e], d.w_lo, d.w_hi);
       fdesc.push_back(d);
     }
   }
@@ -772,6 +773,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       d.list_base = list_words;
       d.list_cap = (uint32_t)S.cet_size[edge];
       d.term_base = ~0ull;
+      key_range(S.hq, st, wq, d.w_lo, d.w_hi);
       off_words += d.ncand + 1;
       list_words += S.cet_size[edge];
       jdesc.push_back(d);

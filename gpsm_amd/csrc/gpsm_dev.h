@@ -55,6 +55,10 @@ struct CetDesc {
   // (tuple_term(term_qid, list[e]) at term_base + e); ~0 for other tables
   uint64_t term_base;
   uint32_t term_qid;
+  // key range of w (as the filter used it): an entry whose neighbour key is
+  // outside it is not in C(w), so its bitmap probe is skipped; w_hi == 0
+  // means no range is known (every entry is probed)
+  uint32_t w_lo, w_hi;
   uint32_t pad;
 };
 

@@ -8,7 +8,9 @@
 // a power-law graph (its lowest ids, the first candidates) land on
 // different waves.  The wave walks the concatenated adjacency of its
 // candidates 64 entries per step (balanced across hub and leaf candidates)
-// and tests C(w) membership in the bitmap.  Outputs are per candidate (its
+// and tests C(w) membership in the bitmap, probing it only for entries
+// whose neighbour key (ckey, read beside col) is in w's label/degree range:
+// C(w) holds no vertex outside it.  Outputs are per candidate (its
 // count, its list at its offset), so the assignment does not change them.
 // The write pass ranks each hit inside its candidate's run with a ballot,
 // so lists stay in adjacency order without a second scan.
@@ -49,6 +51,7 @@ __device__ __forceinline__ void load_chunk(const uint32_t* __restrict__ row, con
 
 __global__ void __launch_bounds__(256) cet_count_kernel(const uint32_t* __restrict__ row,
                                                         const uint32_t* __restrict__ col,
+                                                        const uint32_t* __restrict__ ckey,
                                                         const uint32_t* __restrict__ bitmaps, uint32_t W,
                                                         const uint32_t* __restrict__ clist, CetDesc* descs,
                                                         const uint32_t* __restrict__ chunk_base, uint32_t ndesc,
@@ -67,22 +70,26 @@ __global__ void __launch_bounds__(256) cet_count_kernel(const uint32_t* __restri
     uint32_t D;
     load_chunk(row, clist, d, j, nch, L, mine, D);
     const uint32_t* bm_w = bitmaps + (size_t)d.w_slot * W;
+    const uint32_t wlo = d.w_hi ? d.w_lo : 0u, whi = d.w_hi ? d.w_hi : ~0u;
     uint32_t tot = 0;
     for (uint32_t f = 0; f < D; f += WAVE * CET_UNROLL) {
-      // CET_UNROLL entries per lane: all adjacency loads, then all probes
-      uint32_t cis[CET_UNROLL], mvs[CET_UNROLL];
+      // CET_UNROLL entries per lane: all adjacency and key loads, then the
+      // probes of the entries whose key is in range
+      uint32_t cis[CET_UNROLL], mvs[CET_UNROLL], ks[CET_UNROLL];
 #pragma unroll
       for (int u = 0; u < CET_UNROLL; ++u) {
         const uint32_t g = f + u * WAVE + lane;
         cis[u] = WAVE;
         if (g < D) {
           cis[u] = seg_search(L.pre, WAVE, g);
-          mvs[u] = col[L.off[cis[u]] + (g - L.pre[cis[u]])];
+          const uint32_t e = L.off[cis[u]] + (g - L.pre[cis[u]]);
+          mvs[u] = col[e];
+          ks[u] = ckey[e];
         }
       }
 #pragma unroll
       for (int u = 0; u < CET_UNROLL; ++u)
-        if (cis[u] != WAVE && bit_of(bm_w, mvs[u])) {
+        if (cis[u] != WAVE && ks[u] >= wlo && ks[u] <= whi && bit_of(bm_w, mvs[u])) {
           ++tot;
           if (per_cand) atomicAdd(&L.cnt[cis[u]], 1u);
         }
@@ -114,6 +121,7 @@ __global__ void __launch_bounds__(256) cet_offsets_kernel(const CetDesc* __restr
 
 __global__ void __launch_bounds__(256) cet_write_kernel(const uint32_t* __restrict__ row,
                                                         const uint32_t* __restrict__ col,
+                                                        const uint32_t* __restrict__ ckey,
                                                         const uint32_t* __restrict__ bitmaps, uint32_t W,
                                                         const uint32_t* __restrict__ clist,
                                                         const CetDesc* __restrict__ descs,
@@ -137,26 +145,31 @@ __global__ void __launch_bounds__(256) cet_write_kernel(const uint32_t* __restri
     L.cnt[lane_id()] = mine ? offsets[d.off_base + cand_index(j, nch, lane_id())] : 0u;
     wave_sync();
     const uint32_t* bm_w = bitmaps + (size_t)d.w_slot * W;
+    const uint32_t wlo = d.w_hi ? d.w_lo : 0u, whi = d.w_hi ? d.w_hi : ~0u;
     uint32_t* out = lists + d.list_base;
     for (uint32_t f = 0; f < D; f += WAVE * CET_UNROLL) {
-      // CET_UNROLL sub-steps of 64 entries: all adjacency loads, all probes,
-      // then the writes sub-step by sub-step (lists stay in adjacency order)
-      uint32_t cis[CET_UNROLL], mvs[CET_UNROLL];
+      // CET_UNROLL sub-steps of 64 entries: all adjacency and key loads, the
+      // probes of in-range entries, then the writes sub-step by sub-step
+      // (lists stay in adjacency order)
+      uint32_t cis[CET_UNROLL], mvs[CET_UNROLL], ks[CET_UNROLL];
       bool hits[CET_UNROLL];
 #pragma unroll
       for (int u = 0; u < CET_UNROLL; ++u) {
         const uint32_t g = f + u * WAVE + lane;
         cis[u] = 0;
         mvs[u] = 0;
+        ks[u] = 0;
         hits[u] = false;
         if (g < D) {
           cis[u] = seg_search(L.pre, WAVE, g);
-          mvs[u] = col[L.off[cis[u]] + (g - L.pre[cis[u]])];
+          const uint32_t e = L.off[cis[u]] + (g - L.pre[cis[u]]);
+          mvs[u] = col[e];
+          ks[u] = ckey[e];
         }
       }
 #pragma unroll
       for (int u = 0; u < CET_UNROLL; ++u)
-        if (f + u * WAVE + lane < D) hits[u] = bit_of(bm_w, mvs[u]) != 0;
+        if (f + u * WAVE + lane < D && ks[u] >= wlo && ks[u] <= whi) hits[u] = bit_of(bm_w, mvs[u]) != 0;
 #pragma unroll
       for (int u = 0; u < CET_UNROLL; ++u) {
         const uint32_t fu = f + u * WAVE, ci = cis[u];
@@ -215,8 +228,8 @@ void launch_cet_count(const DevGraph& g, const uint32_t* bitmaps, const uint32_t
                       hipStream_t s) {
   if (!ndesc) return;
   hipLaunchKernelGGL(chunk_base_kernel, dim3(1), dim3(64), 0, s, descs, ndesc, chunk_base);
-  hipLaunchKernelGGL(cet_count_kernel, dim3(grid), dim3(256), 0, s, g.row, g.col, bitmaps, g.W, clist, descs,
-                     chunk_base, ndesc, offsets, per_cand);
+  hipLaunchKernelGGL(cet_count_kernel, dim3(grid), dim3(256), 0, s, g.row, g.col, g.ckey, bitmaps, g.W, clist,
+                     descs, chunk_base, ndesc, offsets, per_cand);
 }
 
 void launch_cet_offsets(CetDesc* descs, uint32_t ndesc, uint32_t* offsets, hipStream_t s) {
@@ -239,8 +252,8 @@ void launch_cet_write(const DevGraph& g, const uint32_t* bitmaps, const uint32_t
                       hipStream_t s) {
   if (!ndesc) return;
   hipLaunchKernelGGL(chunk_base_kernel, dim3(1), dim3(64), 0, s, descs, ndesc, chunk_base);
-  hipLaunchKernelGGL(cet_write_kernel, dim3(grid), dim3(256), 0, s, g.row, g.col, bitmaps, g.W, clist, descs,
-                     chunk_base, ndesc, offsets, lists);
+  hipLaunchKernelGGL(cet_write_kernel, dim3(grid), dim3(256), 0, s, g.row, g.col, g.ckey, bitmaps, g.W, clist,
+                     descs, chunk_base, ndesc, offsets, lists);
 }
 
 }  // namespace gpsm
