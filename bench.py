@@ -141,6 +141,9 @@ def main():
         assert [(r.count, r.hash) for r in rp] == [(r.count, r.hash) for r in res]
         extra["speedup_vs_plain"] = qps / extra["plain_heuristic_qps"]
 
+    # host wall-clock phases of a batch (synchronisations and host work included)
+    phases = {k[5:]: round(v["ms"] / args.steps, 3) for k, v in stats.items() if k.startswith("wall:")}
+    stats = {k: v for k, v in stats.items() if not k.startswith("wall:")}
     # roofline of the dominant kernel (by time in the timed region)
     dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
     name, st = dom
@@ -211,6 +214,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "phase_ms_per_step": phases,
         "workload_gen_s": round(t_gen, 1),
     }
     out.update({k: (round(v, 3) if isinstance(v, float) else v) for k, v in extra.items()})

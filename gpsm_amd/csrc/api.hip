@@ -72,8 +72,9 @@ void DevBuf::release() {
   bytes = 0;
 }
 
-const char* const kProfNames[K_KINDS_] = {"filter_step", "join", "cet_count", "cet_write",
-                                           "compact_rows", "init", "other"};
+const char* const kProfNames[K_KINDS_] = {"filter_step", "join",         "cet_count",   "cet_write",
+                                           "compact_rows", "init",         "other",       "wall:filter",
+                                           "wall:cet_count", "wall:plan", "wall:join"};
 
 hipEvent_t Prof::get() {
   if (!pool.empty()) {
@@ -552,6 +553,15 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   };
   auto t0 = std::chrono::steady_clock::now();
   hipEventRecord(c->ev0, s);
+  auto tphase = t0;
+  auto wall = [&](int kind) {
+    if (!prof) return;
+    const auto now = std::chrono::steady_clock::now();
+    Prof::Acc& acc = c->prof.acc[kind];
+    acc.launches += 1;
+    acc.ms += std::chrono::duration<double, std::milli>(now - tphase).count();
+    tphase = now;
+  };
 
   // ---------------- phase A: filter ----------------
   std::vector<InitOp> init;
@@ -626,6 +636,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     if (exec) hipGraphExecDestroy(exec);
     if (graph) hipGraphDestroy(graph);
   }
+  wall(K_WALL_FILTER);
   const std::vector<uint32_t>& counts = c->h_counts;
   if (hooks && hooks->after_filter) {
     rc = hooks->after_filter(hooks->user, c, bitmaps, counts.data(), nslots, W);
@@ -681,6 +692,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     HIPCHK(hipMemcpyAsync(fdesc.data(), dptr<CetDesc>(c, B_FDESC), sizeof(CetDesc) * fdesc.size(),
                           hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  wall(K_WALL_CET);
   if (hooks && hooks->after_cet_count) {
     rc = hooks->after_cet_count(hooks->user, c, fdesc.data(), (uint32_t)fdesc.size());
     if (rc) return rc;
@@ -832,6 +844,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   }
 
   // ---------------- phase C: candidate-edge tables ----------------
+  wall(K_WALL_PLAN);
   term_start.push_back(term_words);
   if ((rc = upload(c, B_JDESC, jdesc)) || (rc = upload(c, B_PLANS, plans)) || (rc = upload(c, B_TASKS, tasks)) ||
       (rc = upload(c, B_TERM_DESC, term_desc)) || (rc = upload(c, B_TERM_START, term_start)))
@@ -900,6 +913,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       ja.emit_cap = emit_cap;
     }
     ja.budget = o.work_budget;
+    ja.count_bytes = counters ? 1u : 0u;
     pa = pbegin();
     if (!tasks.empty()) launch_join(ja, std::min<uint32_t>(nwg, (uint32_t)tasks.size()), 256, s);
     pend(K_JOIN, pa, 0.0, -1);
@@ -917,6 +931,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       HIPCHK(hipMemcpyAsync(&emitted, dptr<uint32_t>(c, B_CTR) + 16, sizeof(emitted), hipMemcpyDeviceToHost, s));
     hipEventRecord(c->ev1, s);
     HIPCHK(hipStreamSynchronize(s));
+    wall(K_WALL_JOIN);
     if (exec) hipGraphExecDestroy(exec);
     if (graph) hipGraphDestroy(graph);
     float ms = 0;

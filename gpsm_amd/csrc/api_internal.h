@@ -95,11 +95,27 @@ struct Prof {
     uint64_t launches = 0;
     double ms = 0, bytes = 0;
   };
-  Acc acc[8];
+  Acc acc[12];
   hipEvent_t get();
 };
 
-enum ProfKind { K_FILTER = 0, K_JOIN, K_CET_COUNT, K_CET_WRITE, K_COMPACT, K_INIT, K_OTHER, K_KINDS_ };
+// K_WALL_*: host wall-clock time of a batch's phases, synchronisations and
+// host work included (filter; compaction + forward |CET|; join orders;
+// candidate-edge tables + join + results)
+enum ProfKind {
+  K_FILTER = 0,
+  K_JOIN,
+  K_CET_COUNT,
+  K_CET_WRITE,
+  K_COMPACT,
+  K_INIT,
+  K_OTHER,
+  K_WALL_FILTER,
+  K_WALL_CET,
+  K_WALL_PLAN,
+  K_WALL_JOIN,
+  K_KINDS_
+};
 extern const char* const kProfNames[K_KINDS_];
 
 }  // namespace gpsm

@@ -18,6 +18,8 @@
 //    shorter adjacency) and injectivity is checked in the same pass;
 //  * the last level is never written: each embedding adds its tuple hash
 //    (additive over vertices, one fmix64 per emit) and 1 to the counters.
+#include <type_traits>
+
 #include "launch.h"
 #include "wave.h"
 
@@ -39,13 +41,14 @@ __device__ __forceinline__ uint32_t rank_of(const uint32_t* __restrict__ bm, con
 
 // Edge {a, b}: one bit of a hub's adjacency bitmap when either end is a hub,
 // else a binary search in the shorter of the two (short) adjacency lists.
+template <bool BYTES>
 __device__ __forceinline__ bool has_edge(const uint32_t* __restrict__ row, const uint32_t* __restrict__ col,
                                          const uint32_t* __restrict__ hub_of, const uint32_t* __restrict__ hub_bits,
                                          uint32_t W, uint32_t a, uint32_t b, unsigned long long& bytes) {
   const uint32_t ha = hub_of[a], hb = hub_of[b];
-  bytes += 8;
+  if (BYTES) bytes += 8;
   if (ha != ~0u || hb != ~0u) {
-    bytes += 4;
+    if (BYTES) bytes += 4;
     return ha != ~0u ? bit_of(hub_bits + (size_t)ha * W, b) != 0 : bit_of(hub_bits + (size_t)hb * W, a) != 0;
   }
   uint32_t lo = row[a], n = row[a + 1] - lo;
@@ -56,11 +59,11 @@ __device__ __forceinline__ bool has_edge(const uint32_t* __restrict__ row, const
     n = n2;
     key = a;
   }
-  bytes += 16;
+  if (BYTES) bytes += 16;
   if (n == 0) return false;
   // branch-free search for the last entry <= key: ceil(log2 n) halvings, one
   // comparison at the end (no early exit, so no per-step exec-mask updates)
-  bytes += 4ull * (1 + (n > 1 ? 32 - __clz(n - 1) : 0));  // ceil(log2 n) probes + the last read
+  if (BYTES) bytes += 4ull * (1 + (n > 1 ? 32 - __clz(n - 1) : 0));  // ceil(log2 n) probes + the last read
   const uint32_t* p = col + lo;
   while (n > 1) {
     const uint32_t half = n >> 1;
@@ -83,6 +86,9 @@ struct JoinShared {
   // followed by the row window: val[c * WROWS + r], c < qmax
 };
 
+// BYTES: account the algorithmic bytes read and written (JoinOut::bytes);
+// the kernel without it is the one timed.
+template <bool BYTES>
 __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
   extern __shared__ unsigned char lds_raw[];
   JoinShared& S = *reinterpret_cast<JoinShared*>(lds_raw);
@@ -194,7 +200,7 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
           for (uint32_t c = 0; c <= k; ++c) s += tuple_term(P.order[c], val[c * WROWS + tid]);
           S.srow[tid] = s;
           // row values, rank index (prefix + words) and the offset pair
-          my_bytes += (k == 1 ? 8 : 4 * (k + 1)) + 4 + 4 * (((val[p * WROWS + tid] >> 5) & 7) + 1) + 8;
+          if (BYTES) my_bytes += (k == 1 ? 8 : 4 * (k + 1)) + 4 + 4 * (((val[p * WROWS + tid] >> 5) & 7) + 1) + 8;
           const uint32_t j = rank_of(bm_x, bp_x, val[p * WROWS + tid]);
           uint32_t st = offn[j];
           ub = offn[j + 1] - st;
@@ -212,86 +218,96 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
 
         // expand: EXP_UNROLL candidates per thread per step, list loads in
         // flight together; rows of the next level are appended per wave (one
-        // LDS atomic per wave; row order inside a level does not matter)
+        // LDS atomic per wave; row order inside a level does not matter).
+        // The last level and the others are separate instantiations.
         const uint32_t* lst = a.lists + dn.list_base;
-        const unsigned long long* trm_n = final ? a.terms + dn.term_base : nullptr;
         const uint32_t lane = lane_id();
         const unsigned long long below = lane ? (~0ULL >> (WAVE - lane)) : 0ULL;
-        uint32_t f = 0;
-        for (; f < T; f += WROWS * EXP_UNROLL) {
-          if (!final) {
-            // every thread reads fill before any wave appends in this step
-            const uint32_t fl = S.fill[nk];
-            __syncthreads();
-            if (fl + WROWS * EXP_UNROLL > cap) break;  // uniform
-          }
-          uint32_t mvs[EXP_UNROLL], is[EXP_UNROLL];
-          unsigned long long trm[EXP_UNROLL];
-          // wave w takes the step's candidates g0 .. g0 + 255 (lane l: g0 +
-          // 64u + l), so one pair of searches per wave brackets the rows they
-          // fall in and each candidate searches only inside the bracket
-          const uint32_t g0 = f + (tid / WAVE) * (WAVE * EXP_UNROLL);
-          uint32_t rlo = 0, rn = 1;
-          if (g0 < T) {  // uniform per wave
-            const uint32_t glast = min(g0 + WAVE * EXP_UNROLL - 1, T - 1);
-            const uint32_t r = seg_search(S.pre, nw, lane == 0 ? g0 : glast);
-            rlo = __shfl(r, 0, WAVE);
-            rn = __shfl(r, 1, WAVE) - rlo + 1;
-          }
-#pragma unroll
-          for (int u = 0; u < EXP_UNROLL; ++u) {
-            const uint32_t g = g0 + u * WAVE + lane;
-            is[u] = WROWS;
-            mvs[u] = 0;
-            trm[u] = 0;
-            if (g < T) {
-              const uint32_t i = rlo + seg_search(S.pre + rlo, rn, g);
-              is[u] = i;
-              const uint32_t e = S.start[i] + (g - S.pre[i]);
-              mvs[u] = lst[e];
-              // last level: the entry's hash term, computed once per list entry
-              if (final) trm[u] = trm_n[e];
+        auto expand = [&](auto fin) -> uint32_t {
+          constexpr bool FINAL = decltype(fin)::value;
+          const unsigned long long* trm_n = FINAL ? a.terms + dn.term_base : nullptr;
+          uint32_t f = 0;
+          for (; f < T; f += WROWS * EXP_UNROLL) {
+            if constexpr (!FINAL) {
+              // every thread reads fill before any wave appends in this step
+              const uint32_t fl = S.fill[nk];
+              __syncthreads();
+              if (fl + WROWS * EXP_UNROLL > cap) break;  // uniform
             }
-          }
-#pragma unroll
-          for (int u = 0; u < EXP_UNROLL; ++u) {
-            const uint32_t i = is[u], mv = mvs[u];
-            bool ok = i != WROWS;
-            if (ok) {
-              my_bytes += 4;  // the list entry (its hash term is a derived copy of it)
-              // injectivity: k+1 comparisons, no early exit (uniform trip count)
-              for (uint32_t c = 0; c <= k; ++c) ok &= val[c * WROWS + i] != mv;
-              for (uint32_t b = bo; b < be && ok; ++b)
-                ok = has_edge(a.row, a.col, a.hub_of, a.hub_bits, a.W, mv, val[P.backs[b] * WROWS + i], my_bytes);
+            uint32_t mvs[EXP_UNROLL], is[EXP_UNROLL];
+            unsigned long long trm[EXP_UNROLL];
+            // wave w takes the step's candidates g0 .. g0 + 255 (lane l: g0 +
+            // 64u + l), so one pair of searches per wave brackets the rows they
+            // fall in and each candidate searches only inside the bracket
+            const uint32_t g0 = f + (tid / WAVE) * (WAVE * EXP_UNROLL);
+            uint32_t rlo = 0, rn = 1;
+            if (g0 < T) {  // uniform per wave
+              const uint32_t glast = min(g0 + WAVE * EXP_UNROLL - 1, T - 1);
+              const uint32_t r = seg_search(S.pre, nw, lane == 0 ? g0 : glast);
+              rlo = __shfl(r, 0, WAVE);
+              rn = __shfl(r, 1, WAVE) - rlo + 1;
             }
-            if (final) {
+#pragma unroll
+            for (int u = 0; u < EXP_UNROLL; ++u) {
+              const uint32_t g = g0 + u * WAVE + lane;
+              is[u] = WROWS;
+              mvs[u] = 0;
+              trm[u] = 0;
+              if (g < T) {
+                const uint32_t i = rlo + seg_search(S.pre + rlo, rn, g);
+                is[u] = i;
+                const uint32_t e = S.start[i] + (g - S.pre[i]);
+                mvs[u] = lst[e];
+                // last level: the entry's hash term, computed once per list entry
+                if constexpr (FINAL) trm[u] = trm_n[e];
+              }
+            }
+#pragma unroll
+            for (int u = 0; u < EXP_UNROLL; ++u) {
+              const uint32_t i = is[u], mv = mvs[u];
+              bool ok = i != WROWS;
               if (ok) {
-                ++my_cnt;
-                my_hash += fmix64(S.srow[i] + trm[u]);
-                if (a.emit) {
-                  const unsigned long long pos = atomicAdd(a.emit_ctr, 1ULL);
-                  if (pos < a.emit_cap) {
-                    uint32_t* t = a.emit + pos * q;
-                    for (uint32_t c = 0; c <= k; ++c) t[P.order[c]] = val[c * WROWS + i];
-                    t[wn] = mv;
+                if (BYTES) my_bytes += 4;  // the list entry (its hash term is a derived copy of it)
+                // injectivity: the first four columns read together (a
+                // repeated column repeats a comparison), then the rest
+#pragma unroll
+                for (uint32_t c = 0; c < 4; ++c) ok &= val[min(c, k) * WROWS + i] != mv;
+                for (uint32_t c = 4; c <= k; ++c) ok &= val[c * WROWS + i] != mv;
+                for (uint32_t b = bo; b < be && ok; ++b)
+                  ok = has_edge<BYTES>(a.row, a.col, a.hub_of, a.hub_bits, a.W, mv, val[P.backs[b] * WROWS + i],
+                                       my_bytes);
+              }
+              if constexpr (FINAL) {
+                if (ok) {
+                  ++my_cnt;
+                  my_hash += fmix64(S.srow[i] + trm[u]);
+                  if (a.emit) {
+                    const unsigned long long pos = atomicAdd(a.emit_ctr, 1ULL);
+                    if (pos < a.emit_cap) {
+                      uint32_t* t = a.emit + pos * q;
+                      for (uint32_t c = 0; c <= k; ++c) t[P.order[c]] = val[c * WROWS + i];
+                      t[wn] = mv;
+                    }
                   }
                 }
-              }
-            } else {
-              const unsigned long long bal = __ballot(ok);
-              uint32_t wbase = 0;
-              if (lane == 0 && bal) wbase = atomicAdd(&S.fill[nk], (uint32_t)__popcll(bal));
-              wbase = __shfl(wbase, 0, WAVE);
-              if (ok) {
-                const uint64_t r = wbase + __popcll(bal & below);
-                for (uint32_t c = 0; c <= k; ++c) Ln[(uint64_t)c * cap + r] = val[c * WROWS + i];
-                Ln[(uint64_t)nk * cap + r] = mv;
-                my_bytes += 4 * (nk + 1);
+              } else {
+                const unsigned long long bal = __ballot(ok);
+                uint32_t wbase = 0;
+                if (lane == 0 && bal) wbase = atomicAdd(&S.fill[nk], (uint32_t)__popcll(bal));
+                wbase = __shfl(wbase, 0, WAVE);
+                if (ok) {
+                  const uint64_t r = wbase + __popcll(bal & below);
+                  for (uint32_t c = 0; c <= k; ++c) Ln[(uint64_t)c * cap + r] = val[c * WROWS + i];
+                  Ln[(uint64_t)nk * cap + r] = mv;
+                  if (BYTES) my_bytes += 4 * (nk + 1);
+                }
               }
             }
+            if constexpr (!FINAL) __syncthreads();
           }
-          if (!final) __syncthreads();
-        }
+          return f;
+        };
+        const uint32_t f = final ? expand(std::true_type{}) : expand(std::false_type{});
         if (!final && tid == 0) S.lrows[nk] += S.fill[nk];
         // advance level k's cursor past what was expanded
         if (tid == 0) {
@@ -344,13 +360,15 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
       atomicAdd(&a.out[task.query].hash, h);
     }
     __syncthreads();
-    my_bytes = wave_sum64(my_bytes);
-    if (lane_id() == 0) S.red[tid / WAVE] = my_bytes;
-    __syncthreads();
-    if (tid == 0) {
-      unsigned long long bsum = 0;
-      for (uint32_t w = 0; w < blockDim.x / WAVE; ++w) bsum += S.red[w];
-      atomicAdd(&a.out[task.query].bytes, bsum);
+    if constexpr (BYTES) {
+      my_bytes = wave_sum64(my_bytes);
+      if (lane_id() == 0) S.red[tid / WAVE] = my_bytes;
+      __syncthreads();
+      if (tid == 0) {
+        unsigned long long bsum = 0;
+        for (uint32_t w = 0; w < blockDim.x / WAVE; ++w) bsum += S.red[w];
+        atomicAdd(&a.out[task.query].bytes, bsum);
+      }
     }
     if (tid >= 1 && tid < q && S.lrows[tid]) atomicAdd(&a.out[task.query].level_rows[tid], S.lrows[tid]);
     if (tid >= 1 && tid < q && S.lwork[tid]) atomicAdd(&a.out[task.query].level_work[tid], S.lwork[tid]);
@@ -370,10 +388,16 @@ uint64_t join_level_words(uint32_t qmax, uint32_t cap) {
 void launch_join(const JoinArgs& a, uint32_t nwg, uint32_t block, hipStream_t s) {
   (void)block;
   const size_t lds = ((sizeof(JoinShared) + 15) & ~(size_t)15) + sizeof(uint32_t) * WROWS * a.qmax;
-  if (lds > 65536)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(join_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
-  hipLaunchKernelGGL(join_kernel, dim3(nwg), dim3(WROWS), lds, s, a);
+  if (lds > 65536) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(join_kernel<true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(join_kernel<false>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  }
+  if (a.count_bytes)
+    hipLaunchKernelGGL(join_kernel<true>, dim3(nwg), dim3(WROWS), lds, s, a);
+  else
+    hipLaunchKernelGGL(join_kernel<false>, dim3(nwg), dim3(WROWS), lds, s, a);
 }
 
 }  // namespace gpsm

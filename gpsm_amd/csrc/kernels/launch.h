@@ -139,6 +139,8 @@ struct JoinArgs {
   uint64_t emit_cap;
   // 0 or the candidate rows a query may examine before it is abandoned
   unsigned long long budget;
+  // nonzero: account algorithmic bytes in JoinOut::bytes (a slower variant)
+  uint32_t count_bytes;
 };
 void launch_join(const JoinArgs& a, uint32_t nwg, uint32_t block, hipStream_t s);
 uint64_t join_level_words(uint32_t qmax, uint32_t cap);

@@ -281,11 +281,13 @@ typedef struct gpsm_kstat {
 } gpsm_kstat;
 
 /* Per-launch statistics on the context's stream: mode 0 off, 1 HIP-event
- * timing of every launch, 2 timing plus the filter kernel's device counters
- * of the work it did (candidates, adjacency entries, gathers, probes; a
- * few atomics per workgroup), from which its algorithmic bytes are
- * computed.  Setting a mode clears the statistics.  Launches captured in a
- * hipGraph are not timed. */
+ * timing of every launch, 2 timing plus device counters of the work the
+ * filter and join kernels did (candidates, adjacency entries, key reads,
+ * probes, list entries, searches; a few atomics per workgroup), from which
+ * their algorithmic bytes are computed (the join runs a variant that counts
+ * them).  Setting a mode clears the statistics.  Launches captured in a
+ * hipGraph are not timed.  Entries named "wall:<phase>" are host wall-clock
+ * times of a batch's phases, synchronisations included. */
 int gpsm_profile_enable(gpsm_ctx* ctx, int mode);
 /* Copies up to max kernel-family statistics; *n receives the number. */
 int gpsm_profile_read(gpsm_ctx* ctx, gpsm_kstat* out, uint32_t max, uint32_t* n);
