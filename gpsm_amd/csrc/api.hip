@@ -810,6 +810,12 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
           term_words += S.cet_size[pl.par_edge[k]];
         }
         for (uint32_t x : pl.backs[k]) jp.backs[nb++] = (uint8_t)x;
+        uint64_t inj = 0;
+        for (uint32_t c = 0; c < k; ++c)
+          if (S.hq.lab[pl.order[c]] == S.hq.lab[pl.order[k]]) inj |= 1ull << c;
+        inj &= ~(1ull << pl.par_pos[k]);
+        for (uint32_t x : pl.backs[k]) inj &= ~(1ull << x);
+        jp.inj[k] = inj;
       }
     }
     jp.back_off[S.hq.n] = nb;

@@ -68,6 +68,11 @@ struct JoinPlan {
   uint32_t vbase;             // slot of query vertex 0
   uint32_t first_cet;         // CetDesc index of the first table (pi0 -> pi1)
   uint32_t nback_total;
+  // injectivity checks of level k: positions c < k whose query vertex has
+  // the label of position k's, other than the parent and the back
+  // positions.  A candidate of another label cannot equal the vertex bound
+  // there, and one adjacent to the vertex at c cannot equal it (no self-loops).
+  uint64_t inj[MAXQ];
   uint32_t order[MAXQ];       // position -> query id
   uint32_t par[MAXQ];         // parent position of level k (k >= 2)
   uint32_t cet[MAXQ];         // CetDesc index used to extend level k (k >= 2)

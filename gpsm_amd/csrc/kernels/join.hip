@@ -75,7 +75,7 @@ __device__ __forceinline__ bool has_edge(const uint32_t* __restrict__ row, const
 
 struct JoinShared {
   uint32_t pre[WROWS + 1];
-  uint32_t start[WROWS];
+  uint32_t delta[WROWS];  // list position of a row's candidate g is delta[row] + g
   unsigned long long srow[WROWS];
   uint32_t tmp[8];
   uint32_t fill[MAXQ], cur_row[MAXQ], cur_cand[MAXQ];
@@ -109,6 +109,7 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
     const uint32_t* cl0 = a.clist + d0.clist_off;
     const uint32_t* ls0 = a.lists + d0.list_base;
     unsigned long long my_cnt = 0, my_hash = 0, my_bytes = 0;
+    unsigned long long wave_cnt = 0;  // last-level embeddings of this wave (uniform)
 
     if (tid < MAXQ) {
       S.lrows[tid] = 0;
@@ -178,6 +179,7 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
         const uint32_t* bp_x = a.bprefix + (size_t)dn.x_slot * a.NB;
         const uint32_t* offn = a.offsets + dn.off_base;
         const uint32_t bo = P.back_off[nk], be = P.back_off[nk + 1];
+        const uint64_t inj = P.inj[nk];
         const uint32_t wn = P.order[nk];
         const uint32_t cap = a.cap;
         // level j (j >= 2, j+1 columns) starts at cap * (j(j+1)/2 - 3)
@@ -185,7 +187,7 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
         uint32_t* Ln = final ? nullptr : lvl_base + (uint64_t)cap * ((nk * (nk + 1)) / 2 - 3);
 
         // load the window and each row's candidate range
-        uint32_t ub = 0;
+        uint32_t ub = 0, st = 0;
         if (tid < nw) {
           const uint32_t r = r0 + tid;
           if (k == 1) {
@@ -202,17 +204,17 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
           // row values, rank index (prefix + words) and the offset pair
           if (BYTES) my_bytes += (k == 1 ? 8 : 4 * (k + 1)) + 4 + 4 * (((val[p * WROWS + tid] >> 5) & 7) + 1) + 8;
           const uint32_t j = rank_of(bm_x, bp_x, val[p * WROWS + tid]);
-          uint32_t st = offn[j];
+          st = offn[j];
           ub = offn[j + 1] - st;
           if (tid == 0) {
             ub -= S.cur_cand[k];
             st += S.cur_cand[k];
           }
-          S.start[tid] = st;
         }
         uint32_t T;
         const uint32_t ex = block_excl_scan(ub, S.tmp, &T);
         S.pre[tid] = tid < nw ? ex : T;
+        S.delta[tid] = st - ex;  // wraps; delta + g is in range for g in the row's span
         if (tid == 0 && !final) S.fill[nk] = 0;
         __syncthreads();
 
@@ -256,7 +258,7 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
               if (g < T) {
                 const uint32_t i = rlo + seg_search(S.pre + rlo, rn, g);
                 is[u] = i;
-                const uint32_t e = S.start[i] + (g - S.pre[i]);
+                const uint32_t e = S.delta[i] + g;
                 mvs[u] = lst[e];
                 // last level: the entry's hash term, computed once per list entry
                 if constexpr (FINAL) trm[u] = trm_n[e];
@@ -268,18 +270,16 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
               bool ok = i != WROWS;
               if (ok) {
                 if (BYTES) my_bytes += 4;  // the list entry (its hash term is a derived copy of it)
-                // injectivity: the first four columns read together (a
-                // repeated column repeats a comparison), then the rest
-#pragma unroll
-                for (uint32_t c = 0; c < 4; ++c) ok &= val[min(c, k) * WROWS + i] != mv;
-                for (uint32_t c = 4; c <= k; ++c) ok &= val[c * WROWS + i] != mv;
+                // injectivity: only the positions bound to a vertex of the
+                // same label that no edge test already separates
+                for (uint64_t m = inj; m; m &= m - 1) ok &= val[(uint32_t)(__ffsll((long long)m) - 1) * WROWS + i] != mv;
                 for (uint32_t b = bo; b < be && ok; ++b)
                   ok = has_edge<BYTES>(a.row, a.col, a.hub_of, a.hub_bits, a.W, mv, val[P.backs[b] * WROWS + i],
                                        my_bytes);
               }
               if constexpr (FINAL) {
+                wave_cnt += __popcll(__ballot(ok));
                 if (ok) {
-                  ++my_cnt;
                   my_hash += fmix64(S.srow[i] + trm[u]);
                   if (a.emit) {
                     const unsigned long long pos = atomicAdd(a.emit_ctr, 1ULL);
@@ -340,7 +340,7 @@ __global__ void __launch_bounds__(256) join_kernel(JoinArgs a) {
     }
 
     // flush this task's counters
-    my_cnt = wave_sum64(my_cnt);
+    my_cnt = wave_sum64(my_cnt) + wave_cnt;
     my_hash = wave_sum64(my_hash);
     __syncthreads();
     if (lane_id() == 0) S.red[tid / WAVE] = my_cnt;
