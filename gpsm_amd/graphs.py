@@ -50,11 +50,18 @@ class Query:
         return len(self.src)
 
     def struct(self):
-        """ctypes gpsm_query pointing at this query's arrays (keep self alive)."""
+        """ctypes gpsm_query pointing at this query's arrays (keep self alive).
+        Cached while the three arrays stay the same objects."""
+        key = (id(self.labels), id(self.src), id(self.dst))
+        if self.__dict__.get("_skey") == key:
+            return self.__dict__["_s"]
         self.labels = u32(self.labels)
         self.src = u32(self.src)
         self.dst = u32(self.dst)
-        return _lib.QueryS(self.n, self.m, ptr(self.labels), ptr(self.src), ptr(self.dst))
+        s = _lib.QueryS(self.n, self.m, ptr(self.labels), ptr(self.src), ptr(self.dst))
+        self.__dict__["_skey"] = (id(self.labels), id(self.src), id(self.dst))
+        self.__dict__["_s"] = s
+        return s
 
 
 class MappedGraph(DataGraph):
