@@ -86,6 +86,10 @@ SIGNATURES = {
     "gpsm_set_hub_policy": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64]),
     "gpsm_graph_hubs": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "gpsm_graph_set_directed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gpsm_format_tuples": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p,
+                                     C.c_uint64, C.POINTER(C.c_uint64)]),
+    "gpsm_write_tuples": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p,
+                                    C.POINTER(C.c_uint64)]),
     "gpsm_verify_directed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.POINTER(C.c_uint64)]),
     "gpsm_match": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(QueryS), C.POINTER(Opts), C.POINTER(Result)]),

@@ -48,6 +48,10 @@ enum BufId {
   B_TERM_DESC,   // tables that have them
   B_TERM_START,  // their ranges
   B_SEGC,        // popcounts of the candidate bitmaps' segments
+  B_TEXT_IN,     // embedding writer: tuples of a chunk
+  B_TEXT_KEEP,   // ... their keep flags
+  B_TEXT_WG,     // ... scan scratch, per-workgroup lengths and offsets
+  B_TEXT_OUT,    // ... the chunk's text
   B_COUNT_
 };
 

@@ -33,6 +33,11 @@ class IO {
   bool output(const unsigned* final_result, unsigned rows, unsigned cols, const int* id_map, Graph* query,
               Graph* data);
   void flush();
+  // Formats rows (query-id order) into a file descriptor, e.g. on the device
+  // (gpsm_write_tuples); output() uses it for identity id maps.  Returns 0 on
+  // success.
+  using RowWriter = int (*)(int fd, const unsigned* rows, unsigned n, unsigned cols);
+  void set_row_writer(RowWriter w) { writer = w; }
 
  private:
   int data_id = -1;
@@ -40,5 +45,6 @@ class IO {
   FILE* dfp = nullptr;
   FILE* ofp = nullptr;
   bool err = false;
+  RowWriter writer = nullptr;
   std::string line = "============================================================";
 };

@@ -164,3 +164,10 @@ int Match::verify_directed(unsigned* final_result, unsigned& result_row_num) {
   result_row_num = w;
   return status;
 }
+
+int Match::write_rows(int fd, const unsigned* rows, unsigned n, unsigned cols) {
+  if (!g_ctx) return GPSM_ENODEV;
+  const int rc = gpsm_write_tuples(g_ctx, fd, rows, n, cols, nullptr, nullptr);
+  if (rc != GPSM_OK) std::fprintf(stderr, "output: %s\n", gpsm_strerror(rc));
+  return rc;
+}

@@ -27,6 +27,9 @@ class Match {
   // whose query arcs are data arcs with the same label, in place.  Needs the
   // data graph's in/out lists (a text input, not a binary cache).
   int verify_directed(unsigned* final_result, unsigned& result_row_num);
+  // IO::output's embedding lines formatted on the device into fd
+  // (gpsm_write_tuples); an IO::RowWriter.
+  static int write_rows(int fd, const unsigned* rows, unsigned n, unsigned cols);
   // Selects the device; returns false (never exits) when none is usable.
   static bool initGPU(int dev);
   static void releaseGPU();

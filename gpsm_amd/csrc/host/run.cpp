@@ -83,6 +83,7 @@ int main(int argc, const char* argv[]) {
   // a cache is read below; the text reader then only serves the queries
   IO io(args[1], cache ? args[1] : args[0], output);
   if (!io.ok()) return 1;
+  io.set_row_writer(&Match::write_rows);  // embedding lines formatted on the device
   std::vector<Graph*> queries;
   if (!io.input(queries)) {
     std::fprintf(stderr, "query file: malformed or missing terminator\n");
@@ -112,7 +113,7 @@ int main(int argc, const char* argv[]) {
         rc = 1;
         rows = 0;
       }
-      io.output(result, rows, cols, id_map, queries[i], data);
+      if (!io.output(result, rows, cols, id_map, queries[i], data)) rc = 1;
       io.flush();
       delete[] result;
     }

@@ -91,6 +91,17 @@ uint64_t compact_scratch_words(uint64_t nbits);
 void launch_compact_bitmap(const uint32_t* bits, uint64_t nbits, uint32_t* ids, unsigned long long* count,
                            uint32_t* scratch, hipStream_t s);
 
+// text.hip ---------------------------------------------------------------------
+// Embedding lines "(j, k) ... \n" of IO::output.  R = text_rows_per_wg(q)
+// rows per workgroup; wg_len / wg_off: ceil(rows / R) words.
+constexpr uint32_t TEXT_STAGE_BYTES = 61440;
+uint32_t text_fixed_length(uint32_t q);
+uint32_t text_rows_per_wg(uint32_t q);
+void launch_text_len(const uint32_t* tuples, uint64_t rows, uint32_t q, const uint8_t* keep, uint32_t* wg_len,
+                     hipStream_t s);
+void launch_text_write(const uint32_t* tuples, uint64_t rows, uint32_t q, const uint8_t* keep,
+                       const uint32_t* wg_off, char* out, hipStream_t s);
+
 // cet.hip ----------------------------------------------------------------------
 // Per CetDesc: |CET| into desc.total (atomic); per-candidate counts into
 // offsets (when per_cand != 0) at desc.off_base.

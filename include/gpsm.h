@@ -138,6 +138,22 @@ int gpsm_set_hub_policy(gpsm_ctx* ctx, uint32_t min_degree, uint64_t max_bytes);
  * a vertex is one (0 when there are none). */
 int gpsm_graph_hubs(const gpsm_graph* g, uint32_t* nhub, uint32_t* hub_degree);
 
+/* --- embedding text ------------------------------------------------------ */
+
+/* Replaces the line loop of IO::output (io/IO.cpp:207-213) with device
+ * formatting: for every row i of tuples (n_rows x q, column j = query vertex
+ * j) with keep == NULL or keep[i] != 0, the line "(0, t0) (1, t1) ... \n"
+ * ("(%u, %u) " per column, then a newline).  Rows are formatted in chunks
+ * (bounded device and host staging).  gpsm_format_tuples copies the text to
+ * out[0..cap) and returns GPSM_EOVERFLOW when it is longer (out holds the
+ * first cap bytes); *bytes (may be NULL) receives the full length. */
+int gpsm_format_tuples(gpsm_ctx* ctx, const uint32_t* tuples, uint64_t n_rows, uint32_t q, const uint8_t* keep,
+                       char* out, uint64_t cap, uint64_t* bytes);
+/* The same text written to a file descriptor chunk by chunk (GPSM_EINVAL
+ * when a write fails). */
+int gpsm_write_tuples(gpsm_ctx* ctx, int fd, const uint32_t* tuples, uint64_t n_rows, uint32_t q,
+                      const uint8_t* keep, uint64_t* bytes);
+
 /* --- directed, edge-labelled final check -------------------------------- */
 
 /* Attaches the directed, edge-labelled out-adjacency of the data graph to a
