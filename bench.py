@@ -122,11 +122,24 @@ def main():
     # the heuristic order in the same batched launch, and the plain path
     extra = {}
     if planner_used:
-        t0 = time.perf_counter()
-        for _ in range(max(1, args.steps // 2)):
-            rh = rg.match_batch(qs, order_mode=_lib.ORDER_HEURISTIC)
-        dt = (time.perf_counter() - t0) / max(1, args.steps // 2)
-        extra["batched_heuristic_qps"] = len(qs) / dt
+        # learned vs heuristic order on the same batched path: the two
+        # alternate batch by batch, so that drift of the device's clocks
+        # affects both alike; same timing mode as the timed steps
+        cmp = {"learned": [0.0, 0.0], "heuristic": [0.0, 0.0]}
+        for _ in range(args.steps):
+            for name, m in (("learned", mode), ("heuristic", _lib.ORDER_HEURISTIC)):
+                ctx.profile(1)
+                t0 = time.perf_counter()
+                r = rg.match_batch(qs, order_mode=m)
+                cmp[name][0] += time.perf_counter() - t0
+                cmp[name][1] += ctx.kernel_stats()["join"]["ms"]
+                if name == "heuristic":
+                    rh = r
+        ctx.profile(0)
+        extra["batched_heuristic_qps"] = len(qs) * args.steps / cmp["heuristic"][0]
+        extra["learned_over_heuristic"] = cmp["heuristic"][0] / cmp["learned"][0]
+        extra["alternating_qps"] = {k: round(len(qs) * args.steps / v[0], 3) for k, v in cmp.items()}
+        extra["join_ms_per_step"] = {k: round(v[1] / args.steps, 3) for k, v in cmp.items()}
         assert [(r.count, r.hash) for r in rh] == [(r.count, r.hash) for r in res]
         # join work (first-table rows + candidates examined) of both orders
         extra["join_work_learned"] = sum(r.join_work for r in res)

@@ -72,6 +72,27 @@ void DevBuf::release() {
   bytes = 0;
 }
 
+int HostBuf::ensure(size_t nbytes) {
+  if (nbytes <= bytes) return GPSM_OK;
+  if (ptr) hipHostFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+  size_t want = std::max<size_t>(nbytes + nbytes / 4, 4096);
+  if (hipHostMalloc(&ptr, want, hipHostMallocDefault) != hipSuccess) {
+    ptr = nullptr;
+    (void)hipGetLastError();
+    return GPSM_ENOMEM;
+  }
+  bytes = want;
+  return GPSM_OK;
+}
+
+void HostBuf::release() {
+  if (ptr) hipHostFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
 const char* const kProfNames[K_KINDS_] = {"filter_step", "join",         "cet_count",   "cet_write",
                                            "compact_rows", "init",         "other",       "wall:filter",
                                            "wall:cet_count", "wall:plan", "wall:join"};
@@ -243,6 +264,7 @@ void gpsm_shutdown(gpsm_ctx* c) {
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
   for (auto& b : c->bufs) b.release();
+  for (auto& b : c->h_stage) b.release();
   hipEventDestroy(c->ev0);
   hipEventDestroy(c->ev1);
   hipStreamDestroy(c->stream);
@@ -596,8 +618,14 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   if ((rc = c->bufs[B_COUNTS].ensure(sizeof(uint32_t) * std::max<uint32_t>(nslots, 1)))) return rc;
   if ((rc = c->bufs[B_SEGC].ensure(sizeof(uint32_t) * std::max<uint64_t>((uint64_t)nslots * segs_of(W), 1))))
     return rc;
+  uint32_t max_ops = 0;  // ops of the largest superstep
+  for (uint32_t sidx = 0; sidx < maxsteps; ++sidx) max_ops = std::max(max_ops, step_off[sidx + 1] - step_off[sidx]);
+  if ((rc = c->bufs[B_FITEMS].ensure(sizeof(uint2) * std::max<uint64_t>((uint64_t)max_ops * dg.nfchunk, 1))) ||
+      (rc = c->bufs[B_FNITEMS].ensure(sizeof(uint32_t) * std::max<uint32_t>(maxsteps, 1))))
+    return rc;
   const size_t fstat_step = (size_t)STAT_COPIES * STAT_STRIDE;
   if ((rc = c->bufs[B_FSTATS].ensure(sizeof(unsigned long long) * fstat_step * (maxsteps + 1)))) return rc;
+  if ((rc = c->h_stage[0].ensure(sizeof(uint32_t) * std::max<uint32_t>(nslots, 1)))) return rc;
   uint32_t* bitmaps = dptr<uint32_t>(c, B_BITMAPS);
   unsigned long long* fstats = dptr<unsigned long long>(c, B_FSTATS);
   {
@@ -606,6 +634,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     hipGraphExec_t exec = nullptr;
     if (o.use_graph && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) captured = true;
     launch_fill_u32(bitmaps, 0u, (uint64_t)nslots * W, s);
+    if (maxsteps) launch_fill_u32(dptr<uint32_t>(c, B_FNITEMS), 0u, maxsteps, s);
     if (counters) launch_fill_u32(reinterpret_cast<uint32_t*>(fstats), 0u, 2ull * fstat_step * (maxsteps + 1), s);
     {
       hipEvent_t a = pbegin();
@@ -617,6 +646,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       hipEvent_t a = pbegin();
       launch_filter_step(dg, dptr<FilterOp>(c, B_FOPS) + step_off[sidx], nops_s, dptr<uint32_t>(c, B_NBR_SLOT),
                          dptr<uint32_t>(c, B_NBR_LO), dptr<uint32_t>(c, B_NBR_HI), bitmaps, fblock,
+                         c->ncu * 16, dptr<uint2>(c, B_FITEMS), dptr<uint32_t>(c, B_FNITEMS) + sidx,
                          counters ? fstats + fstat_step * sidx : nullptr, s);
       pend(K_FILTER, a, 4.0 * W * nops_s, counters ? (int)sidx : -1);
     }
@@ -630,9 +660,10 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     HIPCHK(hipGetLastError());
     std::vector<uint32_t>& counts = c->h_counts;
     counts.assign(std::max<uint32_t>(nslots, 1), 0);
-    HIPCHK(hipMemcpyAsync(counts.data(), dptr<uint32_t>(c, B_COUNTS), sizeof(uint32_t) * nslots,
+    HIPCHK(hipMemcpyAsync(c->h_stage[0].ptr, dptr<uint32_t>(c, B_COUNTS), sizeof(uint32_t) * nslots,
                           hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    std::memcpy(counts.data(), c->h_stage[0].ptr, sizeof(uint32_t) * nslots);
     if (exec) hipGraphExecDestroy(exec);
     if (graph) hipGraphDestroy(graph);
   }
@@ -678,20 +709,35 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       fdesc.push_back(d);
     }
   }
+  // The forward counts are kept per candidate (offset regions of every
+  // query edge, back to back): a plan edge taken in the forward direction
+  // reuses them in phase C, and only reversed plan edges are counted again.
+  // The buffer is sized here for those too (at most one region of |C(to)|+1
+  // words per query edge), so that phase C does not reallocate it.
+  uint64_t fwd_words = 0, rev_bound = 0;
+  for (auto& d : fdesc) {
+    d.off_base = fwd_words;
+    fwd_words += d.ncand + 1;
+    rev_bound += counts[d.w_slot] + 1;
+  }
   const uint32_t cgrid = std::max<uint32_t>(c->ncu * 8, 64);
   if ((rc = upload(c, B_FDESC, fdesc))) return rc;
   if ((rc = c->bufs[B_CHUNKS].ensure(sizeof(uint32_t) * (fdesc.size() + 2)))) return rc;
+  if ((rc = c->bufs[B_OFFSETS].ensure(sizeof(uint32_t) * std::max<uint64_t>(fwd_words + rev_bound, 1)))) return rc;
+  const uint64_t off_capacity = c->bufs[B_OFFSETS].bytes / sizeof(uint32_t);
   {
     hipEvent_t a = pbegin();
     launch_cet_count(dg, bitmaps, dptr<uint32_t>(c, B_CLIST), dptr<CetDesc>(c, B_FDESC), (uint32_t)fdesc.size(),
-                     nullptr, 0, dptr<uint32_t>(c, B_CHUNKS), cgrid, s);
+                     dptr<uint32_t>(c, B_OFFSETS), 1, dptr<uint32_t>(c, B_CHUNKS), cgrid, s);
     pend(K_CET_COUNT, a, 0.0, -1);
   }
   HIPCHK(hipGetLastError());
+  if ((rc = c->h_stage[1].ensure(sizeof(CetDesc) * std::max<size_t>(fdesc.size(), 1)))) return rc;
   if (!fdesc.empty())
-    HIPCHK(hipMemcpyAsync(fdesc.data(), dptr<CetDesc>(c, B_FDESC), sizeof(CetDesc) * fdesc.size(),
+    HIPCHK(hipMemcpyAsync(c->h_stage[1].ptr, dptr<CetDesc>(c, B_FDESC), sizeof(CetDesc) * fdesc.size(),
                           hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  if (!fdesc.empty()) std::memcpy(fdesc.data(), c->h_stage[1].ptr, sizeof(CetDesc) * fdesc.size());
   wall(K_WALL_CET);
   if (hooks && hooks->after_cet_count) {
     rc = hooks->after_cet_count(hooks->user, c, fdesc.data(), (uint32_t)fdesc.size());
@@ -703,7 +749,8 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   std::vector<JoinPlan> plans;
   std::vector<JoinTask> tasks;
   std::vector<uint32_t> plan_of_q(nq, ~0u);
-  uint64_t off_words = 0, list_words = 0, term_words = 0;
+  uint64_t off_words = fwd_words, list_words = 0, term_words = 0;  // reversed tables' regions follow the forward ones
+  std::vector<uint32_t> recount;  // tables of reversed plan edges (count again)
   std::vector<uint32_t> term_desc;   // tables of last join levels
   std::vector<uint64_t> term_start;  // their term ranges, back to back (+ total)
   for (uint32_t b = 0; b < nq; ++b) {
@@ -730,6 +777,17 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     live_b.push_back(b);
   }
   if (learned) {
+    // a query whose heuristic plan predicts little join work keeps it (its
+    // join takes less time than ranking its candidate orders would); the
+    // network still prices it, so that every query's task share (below)
+    // comes from the same cost model
+    std::vector<uint8_t> rank(live_b.size(), 0);
+    for (size_t i = 0; i < live_b.size(); ++i) {
+      QState& S = Q[live_b[i]];
+      OrderPlan h;
+      heuristic_order(S.hq, S.cet_size, st.adj + 1, h);
+      rank[i] = analytic_cost(pcs[live_b[i]], h) >= kLearnedMinWork;
+    }
     // every planning round of the whole batch is one network call
     std::vector<OrderPlan> chosen_plan;
     std::vector<double> cost;
@@ -738,7 +796,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
         [](void* u, const float* x, uint32_t B, float* y) {
           return gpsm_mlp_forward(static_cast<gpsm_mlp*>(u), x, B, y);
         },
-        c->planner, st.adj + 1, chosen_plan, nullptr, &cost);
+        c->planner, st.adj + 1, chosen_plan, nullptr, &cost, &rank);
     if (prc) return prc;
     for (size_t i = 0; i < live_b.size(); ++i) {
       Q[live_b[i]].plan = std::move(chosen_plan[i]);
@@ -781,12 +839,17 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       d.w_slot = S.vbase + wq;
       d.ncand = counts[d.x_slot];
       d.clist_off = coff[d.x_slot];
-      d.off_base = off_words;
+      if (xq == S.hq.ef[edge]) {
+        d.off_base = fdesc[fdesc_of_q[b] + edge].off_base;  // its counts are there since phase B
+      } else {
+        d.off_base = off_words;
+        off_words += d.ncand + 1;
+        recount.push_back((uint32_t)jdesc.size());
+      }
       d.list_base = list_words;
       d.list_cap = (uint32_t)S.cet_size[edge];
       d.term_base = ~0ull;
       key_range(S.hq, st, wq, d.w_lo, d.w_hi);
-      off_words += d.ncand + 1;
       list_words += S.cet_size[edge];
       jdesc.push_back(d);
       return (uint32_t)jdesc.size() - 1;
@@ -856,12 +919,17 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       (rc = upload(c, B_TERM_DESC, term_desc)) || (rc = upload(c, B_TERM_START, term_start)))
     return rc;
   if ((rc = c->bufs[B_TERMS].ensure(sizeof(uint64_t) * std::max<uint64_t>(term_words, 1)))) return rc;
-  if ((rc = c->bufs[B_OFFSETS].ensure(sizeof(uint32_t) * std::max<uint64_t>(off_words, 1))) ||
+  std::vector<CetDesc> cdesc;
+  for (uint32_t i : recount) cdesc.push_back(jdesc[i]);
+  if (off_words > off_capacity) return GPSM_EINVAL;  // cannot happen: rev_bound covers every reversed table
+  if ((rc = upload(c, B_CDESC, cdesc)) ||
       (rc = c->bufs[B_LISTS].ensure(sizeof(uint32_t) * std::max<uint64_t>(list_words, 1))) ||
       (rc = c->bufs[B_CHUNKS].ensure(sizeof(uint32_t) * (jdesc.size() + 2))))
     return rc;
   const uint32_t nplans = (uint32_t)plans.size();
-  if ((rc = c->bufs[B_JOUT].ensure(sizeof(JoinOut) * std::max<uint32_t>(nplans, 1)))) return rc;
+  if ((rc = c->bufs[B_JOUT].ensure(sizeof(JoinOut) * std::max<uint32_t>(nplans, 1))) ||
+      (rc = c->h_stage[2].ensure(sizeof(JoinOut) * std::max<uint32_t>(nplans, 1))))
+    return rc;
   if ((rc = c->bufs[B_CTR].ensure(256))) return rc;
   const uint64_t lw = join_level_words(qmax, cap);
   if ((rc = c->bufs[B_LEVELS].ensure(sizeof(uint32_t) * std::max<uint64_t>(lw * nwg, 1)))) return rc;
@@ -876,8 +944,9 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     hipGraphExec_t exec = nullptr;
     if (o.use_graph && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) captured = true;
     hipEvent_t pa = pbegin();
-    launch_cet_count(dg, bitmaps, dptr<uint32_t>(c, B_CLIST), dptr<CetDesc>(c, B_JDESC), (uint32_t)jdesc.size(),
-                     dptr<uint32_t>(c, B_OFFSETS), 1, dptr<uint32_t>(c, B_CHUNKS), cgrid, s);
+    if (!cdesc.empty())
+      launch_cet_count(dg, bitmaps, dptr<uint32_t>(c, B_CLIST), dptr<CetDesc>(c, B_CDESC), (uint32_t)cdesc.size(),
+                       dptr<uint32_t>(c, B_OFFSETS), 1, dptr<uint32_t>(c, B_CHUNKS), cgrid, s);
     pend(K_CET_COUNT, pa, 0.0, -1);
     launch_cet_offsets(dptr<CetDesc>(c, B_JDESC), (uint32_t)jdesc.size(), dptr<uint32_t>(c, B_OFFSETS), s);
     pa = pbegin();
@@ -931,12 +1000,14 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     }
     HIPCHK(hipGetLastError());
     std::vector<JoinOut> jo(std::max<uint32_t>(nplans, 1));
-    HIPCHK(hipMemcpyAsync(jo.data(), dptr<JoinOut>(c, B_JOUT), sizeof(JoinOut) * nplans, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(c->h_stage[2].ptr, dptr<JoinOut>(c, B_JOUT), sizeof(JoinOut) * nplans,
+                          hipMemcpyDeviceToHost, s));
     unsigned long long emitted = 0;
     if (emit_cap)
       HIPCHK(hipMemcpyAsync(&emitted, dptr<uint32_t>(c, B_CTR) + 16, sizeof(emitted), hipMemcpyDeviceToHost, s));
     hipEventRecord(c->ev1, s);
     HIPCHK(hipStreamSynchronize(s));
+    if (nplans) std::memcpy(jo.data(), c->h_stage[2].ptr, sizeof(JoinOut) * nplans);
     wall(K_WALL_JOIN);
     if (exec) hipGraphExecDestroy(exec);
     if (graph) hipGraphDestroy(graph);

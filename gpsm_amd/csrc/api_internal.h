@@ -19,6 +19,15 @@ struct DevBuf {
   void release();
 };
 
+// Pinned host staging (hipHostMalloc): copies from and to it are queued on
+// the stream like kernels, where a pageable copy waits on the host.
+struct HostBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t nbytes);
+  void release();
+};
+
 enum BufId {
   B_SCRATCH,
   B_BITMAPS,
@@ -52,6 +61,9 @@ enum BufId {
   B_TEXT_KEEP,   // ... their keep flags
   B_TEXT_WG,     // ... scan scratch, per-workgroup lengths and offsets
   B_TEXT_OUT,    // ... the chunk's text
+  B_FITEMS,      // filter superstep: (op, vertex range) pairs with candidates
+  B_FNITEMS,     // ... their number, one counter per superstep
+  B_CDESC,       // candidate-edge tables of reversed plan edges (counted again)
   B_COUNT_
 };
 
@@ -135,6 +147,7 @@ struct gpsm_ctx {
   gpsm::Tuning tune;
   gpsm_mlp* planner = nullptr;
   std::vector<uint32_t> h_counts;
+  gpsm::HostBuf h_stage[3];  // pinned readbacks of a batch: counts, |CET| descriptors, join results
   // hub adjacency bitmaps of later uploads: vertices of degree > the hub
   // degree, starting at hub_min_degree and doubled until they fit hub_max_bytes
   uint32_t hub_min_degree = 512;

@@ -595,18 +595,20 @@ double analytic_cost(const PlanContext& c, const OrderPlan& plan) {
 }
 
 int learned_plans(const std::vector<const PlanContext*>& qs, PredictFn predict, void* user, uint64_t big,
-                  std::vector<OrderPlan>& out, std::vector<uint32_t>* chosen, std::vector<double>* cost) {
+                  std::vector<OrderPlan>& out, std::vector<uint32_t>* chosen, std::vector<double>* cost,
+                  const std::vector<uint8_t>* rank) {
   const size_t nq = qs.size();
   out.assign(nq, OrderPlan());
   if (chosen) chosen->assign(nq, 0);
   if (cost) cost->assign(nq, 0.0);
   std::vector<float> feats, y;
+  auto ranked = [&](size_t b) { return !rank || (*rank)[b]; };
   // 1. greedy learned orders, one decision round at a time for the batch
   std::vector<LearnedState> ls(nq);
   std::vector<size_t> live, rows_of(nq);
   for (size_t b = 0; b < nq; ++b) {
     learned_start(*qs[b], ls[b]);
-    live.push_back(b);
+    if (ranked(b)) live.push_back(b);
   }
   while (!live.empty()) {
     feats.clear();
@@ -631,7 +633,13 @@ int learned_plans(const std::vector<const PlanContext*>& qs, PredictFn predict, 
   std::vector<std::vector<PlanCost>> pc(nq);
   for (size_t b = 0; b < nq; ++b) {
     const PlanContext& c = *qs[b];
-    candidate_orders(*c.q, *c.cet_size, big, cands[b]);
+    if (ranked(b)) {
+      candidate_orders(*c.q, *c.cet_size, big, cands[b]);
+    } else {
+      OrderPlan h;
+      heuristic_order(*c.q, *c.cet_size, big, h);
+      cands[b].assign(1, h.order);
+    }
     if (ls[b].order.size() == c.q->n &&
         std::find(cands[b].begin(), cands[b].end(), ls[b].order) == cands[b].end())
       cands[b].push_back(ls[b].order);

@@ -158,13 +158,28 @@ void cost_consume(PlanCost& s, const float* y);
 // predicted cost is below this fraction of the heuristic order's.
 constexpr double kLearnedMargin = 0.9;
 
+// Queries whose heuristic plan has a predicted cost (analytic_cost) below
+// this many examined candidates keep the heuristic order without ranking
+// (learned_plans' rank flags).
+// On an MI355X the join examines about 3e8 candidates per millisecond
+// (3.56e-9 ms each, fitted by gpsm_amd/calibrate_planner.py on the 100 walk
+// queries of the 1M-vertex Chung-Lu graph), so such a query's whole join
+// takes a few microseconds, less than ranking its candidate orders on the
+// host.
+constexpr double kLearnedMinWork = 1e6;
+
 // The learned planner for a batch: greedy learned orders, then the cheapest
 // of each query's candidates (candidate_orders plus the greedy order) by
 // chained predicted cost.  out[b] is always a valid plan; chosen[b] (if
 // given) is the index of the candidate taken, 0 = the heuristic order.
 // Returns 0 or the first non-zero code of predict.
+// rank (optional, one flag per query): queries with a zero flag skip the
+// greedy search and the candidate orders and keep the heuristic order, whose
+// chained cost is still evaluated (so that every query's cost comes from the
+// same network).
 int learned_plans(const std::vector<const PlanContext*>& qs, PredictFn predict, void* user, uint64_t big,
-                  std::vector<OrderPlan>& out, std::vector<uint32_t>* chosen, std::vector<double>* cost = nullptr);
+                  std::vector<OrderPlan>& out, std::vector<uint32_t>* chosen, std::vector<double>* cost = nullptr,
+                  const std::vector<uint8_t>* rank = nullptr);
 
 // The same cost of a complete plan with the analytic chain estimate in place
 // of the network: each level's log rows = previous log rows + log(|CET| of

@@ -47,6 +47,69 @@ __global__ void init_kernel(const uint32_t* __restrict__ key, uint32_t n, uint32
   }
 }
 
+// The (op, vertex range) pairs of a superstep whose range holds at least one
+// candidate of the op's C(u), appended to items[] (wave-aggregated, order
+// arbitrary: a superstep's updates are atomic bit operations on distinct
+// slots per query, so the order of its items does not change the result).
+// Once the sets are small most ranges are empty, and the superstep then
+// costs one coalesced pass over the bitmap words instead of one wave per
+// pair.  One wave per (op, 64 consecutive ranges): the lanes read the
+// ranges' words 64 at a time, and each set bit marks its range (a binary
+// search in the 65 range bounds), after which the bits of that range are
+// skipped.
+constexpr uint32_t AR_GROUP = WAVE;  // vertex ranges per wave
+
+__global__ void __launch_bounds__(256) active_ranges_kernel(const uint32_t* __restrict__ fchunk, uint32_t chunks,
+                                                             const FilterOp* __restrict__ ops, uint32_t nops,
+                                                             const uint32_t* __restrict__ bitmaps, uint32_t W,
+                                                             uint2* __restrict__ items, uint32_t* __restrict__ nitems) {
+  __shared__ uint32_t s_lo[4][AR_GROUP + 1];
+  __shared__ uint32_t s_act[4][AR_GROUP];
+  const uint32_t lane = lane_id(), wv = threadIdx.x / WAVE;
+  const uint32_t groups = (chunks + AR_GROUP - 1) / AR_GROUP;
+  const uint64_t gw = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  if (gw >= (uint64_t)nops * groups) return;  // the whole wave
+  const uint32_t o = uniform((uint32_t)(gw / groups)), c0 = uniform((uint32_t)(gw % groups)) * AR_GROUP;
+  const uint32_t nc = min(AR_GROUP, chunks - c0);
+  uint32_t* lo_b = s_lo[wv];
+  uint32_t* act = s_act[wv];
+  if (lane < nc) lo_b[lane] = fchunk[c0 + lane];
+  if (lane == 0) lo_b[nc] = fchunk[c0 + nc];
+  act[lane] = 0;
+  wave_sync();
+  const uint32_t v_begin = lo_b[0], v_end = lo_b[nc];
+  const uint32_t w_begin = v_begin >> 5, w_last = (v_end - 1) >> 5;
+  const uint32_t* bm = bitmaps + (size_t)uniform(ops[o].u_slot) * W;
+  for (uint32_t w0 = w_begin; w0 <= w_last; w0 += WAVE) {
+    const uint32_t w = w0 + lane;
+    if (w > w_last) continue;
+    uint32_t x = bm[w];
+    if (w == w_begin) x &= ~0u << (v_begin & 31);
+    if (w == w_last && (v_end & 31)) x &= (1u << (v_end & 31)) - 1u;
+    while (x) {
+      const uint32_t v = w * 32 + (uint32_t)(__ffs(x) - 1);
+      uint32_t a = 0, b = nc;  // range a: lo_b[a] <= v < lo_b[a + 1]
+      while (b - a > 1) {
+        const uint32_t mid = (a + b) >> 1;
+        if (lo_b[mid] <= v) a = mid;
+        else b = mid;
+      }
+      act[a] = 1;
+      const uint32_t next = lo_b[a + 1];  // first vertex after range a (> v)
+      if (next >= w * 32 + 32) break;
+      x &= ~0u << (next - w * 32);
+    }
+  }
+  wave_sync();
+  const bool on = lane < nc && act[lane] != 0;
+  const unsigned long long m = __ballot(on);
+  if (!m) return;
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(nitems, (uint32_t)__popcll(m));
+  base = __shfl(base, 0, WAVE);
+  if (on) items[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = make_uint2(o, c0 + lane);
+}
+
 constexpr int UNROLL = 4;  // adjacency entries in flight per lane
 
 struct WaveLds {
@@ -85,8 +148,10 @@ __device__ __forceinline__ unsigned long long constraint_bits(uint32_t type, uin
 __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __restrict__ row,
                                                           const uint32_t* __restrict__ col,
                                                           const uint32_t* __restrict__ ckey, uint32_t W,
-                                                          const uint32_t* __restrict__ fchunk, uint32_t chunks,
-                                                          const FilterOp* __restrict__ ops, uint32_t nops,
+                                                          const uint32_t* __restrict__ fchunk,
+                                                          const uint2* __restrict__ list,
+                                                          const uint32_t* __restrict__ nlist,
+                                                          const FilterOp* __restrict__ ops,
                                                           const uint32_t* __restrict__ nbr_slot,
                                                           const uint32_t* __restrict__ nbr_lo,
                                                           const uint32_t* __restrict__ nbr_hi,
@@ -95,16 +160,17 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
   extern __shared__ unsigned char lds_raw[];
   WaveLds& L = reinterpret_cast<WaveLds*>(lds_raw)[threadIdx.x / WAVE];
   const uint32_t lane = lane_id();
-  const uint64_t items = (uint64_t)nops * chunks;
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
+  const uint32_t items = uniform(*nlist);
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) / WAVE;
   // per-lane tallies for the optional byte accounting, over all items
   unsigned long long t_cand = 0;
   uint32_t t_ent = 0, t_key = 0, t_probe = 0, t_set = 0;
 
-  for (uint64_t it = wave; it < items; it += nwaves) {
-    const uint32_t o = uniform((uint32_t)(it / chunks));
-    const uint32_t c = uniform((uint32_t)(it % chunks));
+  for (uint32_t it = wave; it < items; it += nwaves) {
+    const uint2 item = list[it];
+    const uint32_t o = uniform(item.x);
+    const uint32_t c = uniform(item.y);
     const uint32_t type = uniform(ops[o].type);
     const uint32_t nv = uniform(ops[o].nv);
     const uint32_t nb = uniform(ops[o].nbr_off);
@@ -265,16 +331,21 @@ void launch_init(const DevGraph& g, const InitOp* ops, uint32_t nops, uint32_t* 
 
 void launch_filter_step(const DevGraph& g, const FilterOp* ops, uint32_t nops, const uint32_t* nbr_slot,
                         const uint32_t* nbr_lo, const uint32_t* nbr_hi, uint32_t* bitmaps, uint32_t block,
-                        unsigned long long* stats, hipStream_t s) {
+                        uint32_t max_wg, uint2* items, uint32_t* nitems, unsigned long long* stats, hipStream_t s) {
   if (!nops) return;
   if (block != 64 && block != 128 && block != 256) block = 256;
-  const uint64_t items = (uint64_t)nops * g.nfchunk;
+  const uint64_t pairs = (uint64_t)nops * g.nfchunk;
+  const uint64_t awaves = (uint64_t)nops * ((g.nfchunk + AR_GROUP - 1) / AR_GROUP);
+  hipLaunchKernelGGL(active_ranges_kernel, dim3((uint32_t)((awaves + 3) / 4)), dim3(256), 0, s, g.fchunk,
+                     g.nfchunk, ops, nops, bitmaps, g.W, items, nitems);
+  // the list's length is known on the device only: a grid of at most
+  // max_wg workgroups strides over it (waves past its end leave at once)
   const uint32_t wpb = block / WAVE;
-  uint64_t grid = (items + wpb - 1) / wpb;
-  if (grid > 65536) grid = 65536;
+  uint64_t grid = (pairs + wpb - 1) / wpb;
+  if (grid > max_wg) grid = max_wg;
   const size_t lds = sizeof(WaveLds) * wpb;
   hipLaunchKernelGGL(filter_step_kernel, dim3((uint32_t)grid), dim3(block), lds, s, g.row, g.col, g.ckey, g.W,
-                     g.fchunk, g.nfchunk, ops, nops, nbr_slot, nbr_lo, nbr_hi, bitmaps, stats);
+                     g.fchunk, items, nitems, ops, nbr_slot, nbr_lo, nbr_hi, bitmaps, stats);
 }
 
 void filter_chunks(uint32_t n, const uint32_t* row_offset, std::vector<uint32_t>& bounds) {

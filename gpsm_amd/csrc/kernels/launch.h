@@ -63,10 +63,14 @@ void launch_init(const DevGraph& g, const InitOp* ops, uint32_t nops, uint32_t* 
 // copy c, entries 0..4 = candidates, adjacency entries read, key gathers,
 // bitmap probes, bits set, added by the workgroups with blockIdx % copies
 // == c (the sum over copies is the launch's total).
+// The superstep visits only the (op, vertex range) pairs whose range holds a
+// candidate of C(u): a first kernel lists them in items (nops x g.nfchunk
+// capacity) and counts them in *nitems (zeroed by the caller), the second
+// processes the list with at most max_wg workgroups.
 constexpr uint32_t STAT_COPIES = 32, STAT_STRIDE = 8;
 void launch_filter_step(const DevGraph& g, const FilterOp* ops, uint32_t nops, const uint32_t* nbr_slot,
                         const uint32_t* nbr_lo, const uint32_t* nbr_hi, uint32_t* bitmaps, uint32_t block,
-                        unsigned long long* stats, hipStream_t s);
+                        uint32_t max_wg, uint2* items, uint32_t* nitems, unsigned long long* stats, hipStream_t s);
 
 // scan.hip ---------------------------------------------------------------------
 // Bitmap rows are compacted in segments of SEG_WORDS words, one workgroup

@@ -291,6 +291,7 @@ struct gpsm_mlp {
   float* loss = nullptr;
   float* norm = nullptr;  // mu[F], inv_sd[F]
   DevBuf x, t, y;
+  HostBuf hx, hy;  // pinned staging of forward's features and outputs
   int step = 0;
 };
 
@@ -378,6 +379,8 @@ void gpsm_mlp_free(gpsm_mlp* mm) {
   hipFree(mm->loss);
   hipFree(mm->norm);
   mm->x.release();
+  mm->hx.release();
+  mm->hy.release();
   mm->t.release();
   mm->y.release();
   delete mm;
@@ -404,18 +407,24 @@ int gpsm_mlp_get_params(gpsm_mlp* mm, float* out) {
   return GPSM_OK;
 }
 
+// The planner calls this once per planning round with a few thousand rows:
+// features and outputs go through pinned staging so that the copies queue
+// on the stream with the kernel and the round costs one synchronisation.
 int gpsm_mlp_forward(gpsm_mlp* mm, const float* x, uint32_t B, float* y) {
   if (!mm || (B && (!x || !y))) return GPSM_EINVAL;
   if (!B) return GPSM_OK;
-  int rc = stage(mm, x, nullptr, B);
-  if (rc) return rc;
+  const size_t xb = sizeof(float) * (size_t)B * F, yb = sizeof(float) * B;
+  if (mm->x.ensure(xb) || mm->y.ensure(yb) || mm->hx.ensure(xb) || mm->hy.ensure(yb)) return GPSM_ENOMEM;
   hipStream_t s = mm->ctx->stream;
+  std::memcpy(mm->hx.ptr, x, xb);
+  if (hipMemcpyAsync(mm->x.ptr, mm->hx.ptr, xb, hipMemcpyHostToDevice, s) != hipSuccess) return GPSM_EDEVICE;
   hipLaunchKernelGGL(mlp_forward_kernel, dim3((B + 31) / 32), dim3(64), 0, s, weights_of(mm),
                      reinterpret_cast<const float*>(mm->x.ptr), B, reinterpret_cast<float*>(mm->y.ptr));
   if (hipGetLastError() != hipSuccess ||
-      hipMemcpyAsync(y, mm->y.ptr, sizeof(float) * B, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(mm->hy.ptr, mm->y.ptr, yb, hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return GPSM_EDEVICE;
+  std::memcpy(y, mm->hy.ptr, yb);
   return GPSM_OK;
 }
 

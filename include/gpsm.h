@@ -59,7 +59,8 @@ typedef struct gpsm_query {
 
 /* join-order selection */
 #define GPSM_ORDER_HEURISTIC 0 /* the reference's greedy min-|CET| order (match/Match.cu:1599-1661) */
-#define GPSM_ORDER_LEARNED 1   /* greedy over MLP-predicted log cardinalities; needs gpsm_set_planner */
+#define GPSM_ORDER_LEARNED 1   /* candidate orders ranked by MLP-predicted join cost (queries with little
+                                  predicted join work keep the heuristic order); needs gpsm_set_planner */
 #define GPSM_ORDER_EXPLICIT 2  /* caller-given vertex order in gpsm_opts.explicit_order */
 
 typedef struct gpsm_opts {
