@@ -283,6 +283,7 @@ void free_graph_arrays(DevGraph& d) {
   hipFree(d.key);
   hipFree(d.ckey);
   hipFree(d.fchunk);
+  hipFree(d.wrange);
   hipFree(d.hub_of);
   hipFree(d.hub_bits);
   d = DevGraph();
@@ -323,21 +324,28 @@ int finish_graph(gpsm_ctx* c, gpsm_graph* g, const uint32_t* row_offset, const u
   std::vector<uint32_t> fch;
   filter_chunks(n, row_offset, fch);
   d.nfchunk = (uint32_t)fch.size() - 1;
+  std::vector<uint32_t> wr(d.W);
+  for (uint32_t w = 0, r = 0; w < d.W; ++w) {
+    while (fch[r + 1] <= 32 * w) ++r;
+    wr[w] = r;
+  }
   if (hipMalloc(&d.key, sizeof(uint32_t) * n) != hipSuccess ||
       hipMalloc(&d.ckey, sizeof(uint32_t) * std::max<uint64_t>(d.adj, 1)) != hipSuccess ||
-      hipMalloc(&d.fchunk, sizeof(uint32_t) * fch.size()) != hipSuccess) {
+      hipMalloc(&d.fchunk, sizeof(uint32_t) * fch.size()) != hipSuccess ||
+      hipMalloc(&d.wrange, sizeof(uint32_t) * d.W) != hipSuccess) {
     (void)hipGetLastError();
     return fail(GPSM_ENOMEM);
   }
   if (hipMemcpyAsync(d.key, key.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
       hipMemcpyAsync(d.fchunk, fch.data(), sizeof(uint32_t) * fch.size(), hipMemcpyHostToDevice, c->stream) !=
-          hipSuccess) {
+          hipSuccess ||
+      hipMemcpyAsync(d.wrange, wr.data(), sizeof(uint32_t) * d.W, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
     (void)hipGetLastError();
     return fail(GPSM_EDEVICE);
   }
   launch_neighbour_keys(d, c->stream);
   if (hipStreamSynchronize(c->stream) != hipSuccess || hipGetLastError() != hipSuccess) return fail(GPSM_EDEVICE);
-  g->bytes = sizeof(uint32_t) * ((uint64_t)n + 1 + 2 * d.adj + n + fch.size());
+  g->bytes = sizeof(uint32_t) * ((uint64_t)n + 1 + 2 * d.adj + n + fch.size() + d.W);
   if (int rc = build_hubs(c, row_offset, g)) return fail(rc);
   return GPSM_OK;
 }
@@ -602,6 +610,11 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       op.u_slot = S.vbase + fsx.u;
       op.nv = (uint32_t)fsx.v.size();
       op.nbr_off = (uint32_t)nbr_slot.size();
+      op.mark = op.need = ~0u;
+      if (fsx.type == OP_REFINE) {
+        op.mark = b * 4 + fsx.round;
+        if (fsx.round > 0) op.need = b * 4 + fsx.round - 1;
+      }
       for (size_t i = 0; i < fsx.v.size(); ++i) {
         nbr_slot.push_back(S.vbase + fsx.v[i]);
         nbr_lo.push_back(fsx.type == OP_EXPLORE ? fsx.lo[i] : 0);
@@ -621,7 +634,8 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   uint32_t max_ops = 0;  // ops of the largest superstep
   for (uint32_t sidx = 0; sidx < maxsteps; ++sidx) max_ops = std::max(max_ops, step_off[sidx + 1] - step_off[sidx]);
   if ((rc = c->bufs[B_FITEMS].ensure(sizeof(uint2) * std::max<uint64_t>((uint64_t)max_ops * dg.nfchunk, 1))) ||
-      (rc = c->bufs[B_FNITEMS].ensure(sizeof(uint32_t) * std::max<uint32_t>(maxsteps, 1))))
+      (rc = c->bufs[B_FNITEMS].ensure(sizeof(uint32_t) * std::max<uint32_t>(maxsteps, 1))) ||
+      (rc = c->bufs[B_RFLAGS].ensure(sizeof(uint32_t) * 4 * (uint64_t)std::max<uint32_t>(nq, 1))))
     return rc;
   const size_t fstat_step = (size_t)STAT_COPIES * STAT_STRIDE;
   if ((rc = c->bufs[B_FSTATS].ensure(sizeof(unsigned long long) * fstat_step * (maxsteps + 1)))) return rc;
@@ -635,6 +649,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     if (o.use_graph && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) captured = true;
     launch_fill_u32(bitmaps, 0u, (uint64_t)nslots * W, s);
     if (maxsteps) launch_fill_u32(dptr<uint32_t>(c, B_FNITEMS), 0u, maxsteps, s);
+    launch_fill_u32(dptr<uint32_t>(c, B_RFLAGS), 0u, 4ull * std::max<uint32_t>(nq, 1), s);
     if (counters) launch_fill_u32(reinterpret_cast<uint32_t*>(fstats), 0u, 2ull * fstat_step * (maxsteps + 1), s);
     {
       hipEvent_t a = pbegin();
@@ -646,7 +661,8 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       hipEvent_t a = pbegin();
       launch_filter_step(dg, dptr<FilterOp>(c, B_FOPS) + step_off[sidx], nops_s, dptr<uint32_t>(c, B_NBR_SLOT),
                          dptr<uint32_t>(c, B_NBR_LO), dptr<uint32_t>(c, B_NBR_HI), bitmaps, fblock,
-                         c->ncu * 16, dptr<uint2>(c, B_FITEMS), dptr<uint32_t>(c, B_FNITEMS) + sidx,
+                         65536, dptr<uint2>(c, B_FITEMS), dptr<uint32_t>(c, B_FNITEMS) + sidx,
+                         dptr<uint32_t>(c, B_RFLAGS),
                          counters ? fstats + fstat_step * sidx : nullptr, s);
       pend(K_FILTER, a, 4.0 * W * nops_s, counters ? (int)sidx : -1);
     }

@@ -64,6 +64,7 @@ enum BufId {
   B_FITEMS,      // filter superstep: (op, vertex range) pairs with candidates
   B_FNITEMS,     // ... their number, one counter per superstep
   B_CDESC,       // candidate-edge tables of reversed plan edges (counted again)
+  B_RFLAGS,      // refine rounds that removed a candidate, 4 per query
   B_COUNT_
 };
 

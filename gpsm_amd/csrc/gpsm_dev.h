@@ -34,6 +34,13 @@ struct FilterOp {
   uint32_t u_slot;   // C(u) iterated
   uint32_t nv;       // neighbour entries
   uint32_t nbr_off;  // into nbr_slot / nbr_lo / nbr_hi
+  // Refine rounds (match/Match.cu:863-887) after one that removed nothing
+  // would remove nothing either: the state is then a fixed point of the
+  // round.  mark: flag word set when this op removes a candidate (~0u: none);
+  // need: flag word of the query's previous round, which must be set for
+  // this op to run (~0u: always runs).
+  uint32_t mark;
+  uint32_t need;
 };
 
 // Initial label/degree check of a query's start vertex.

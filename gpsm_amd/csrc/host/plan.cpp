@@ -151,6 +151,7 @@ void make_filter_schedule(const HostQuery& q, const GraphStats& s, FilterSchedul
       FilterStep st;
       st.type = OP_REFINE;
       st.u = u;
+      st.round = (uint32_t)round;
       for (uint32_t id : q.in[u])
         if (valid[id]) st.v.push_back(id);
       for (uint32_t id : q.out[u])

@@ -37,6 +37,7 @@ struct HostQuery {
 struct FilterStep {
   uint32_t type;  // OP_EXPLORE / OP_REFINE
   uint32_t u;
+  uint32_t round = 0;            // refine: round 0..3
   std::vector<uint32_t> v;       // neighbour query ids, in the reference's order
   std::vector<uint32_t> lo, hi;  // explore: key range of each neighbour
 };

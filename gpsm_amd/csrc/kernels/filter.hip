@@ -48,66 +48,104 @@ __global__ void init_kernel(const uint32_t* __restrict__ key, uint32_t n, uint32
 }
 
 // The (op, vertex range) pairs of a superstep whose range holds at least one
-// candidate of the op's C(u), appended to items[] (wave-aggregated, order
-// arbitrary: a superstep's updates are atomic bit operations on distinct
-// slots per query, so the order of its items does not change the result).
-// Once the sets are small most ranges are empty, and the superstep then
-// costs one coalesced pass over the bitmap words instead of one wave per
-// pair.  One wave per (op, 64 consecutive ranges): the lanes read the
-// ranges' words 64 at a time, and each set bit marks its range (a binary
-// search in the 65 range bounds), after which the bits of that range are
-// skipped.
-constexpr uint32_t AR_GROUP = WAVE;  // vertex ranges per wave
+// candidate of the op's C(u), appended to items[] (order arbitrary: a
+// superstep's updates are atomic bit operations on distinct slots per
+// query, so the order of its items does not change the result).  Once the
+// sets are small most ranges are empty, and the superstep then costs one
+// coalesced pass over the bitmap words instead of one wave per pair.
+// One wave per (op, 256 consecutive ranges); lane l reads words l, l + 64,
+// ... of the ranges' span, eight loads in flight, together with the range of
+// each word's first vertex (wrange), and a nonzero word marks its range
+// (and, when it runs past that range's end, the next ones it has bits in).
+// The list is appended to with one atomic per workgroup: appends to a
+// single counter serialise, and one per wave cost as much as the scan.
+constexpr uint32_t AR_GROUP = 4 * WAVE;  // vertex ranges per wave
+constexpr int AR_UNROLL = 8;             // bitmap words in flight per lane
 
-__global__ void __launch_bounds__(256) active_ranges_kernel(const uint32_t* __restrict__ fchunk, uint32_t chunks,
+__global__ void __launch_bounds__(256) active_ranges_kernel(const uint32_t* __restrict__ fchunk,
+                                                             const uint32_t* __restrict__ wrange, uint32_t chunks,
                                                              const FilterOp* __restrict__ ops, uint32_t nops,
                                                              const uint32_t* __restrict__ bitmaps, uint32_t W,
+                                                             const uint32_t* __restrict__ rflags,
                                                              uint2* __restrict__ items, uint32_t* __restrict__ nitems) {
   __shared__ uint32_t s_lo[4][AR_GROUP + 1];
   __shared__ uint32_t s_act[4][AR_GROUP];
+  __shared__ uint32_t s_wtot[4], s_wbase[4];
   const uint32_t lane = lane_id(), wv = threadIdx.x / WAVE;
   const uint32_t groups = (chunks + AR_GROUP - 1) / AR_GROUP;
   const uint64_t gw = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-  if (gw >= (uint64_t)nops * groups) return;  // the whole wave
-  const uint32_t o = uniform((uint32_t)(gw / groups)), c0 = uniform((uint32_t)(gw % groups)) * AR_GROUP;
-  const uint32_t nc = min(AR_GROUP, chunks - c0);
   uint32_t* lo_b = s_lo[wv];
   uint32_t* act = s_act[wv];
-  if (lane < nc) lo_b[lane] = fchunk[c0 + lane];
-  if (lane == 0) lo_b[nc] = fchunk[c0 + nc];
-  act[lane] = 0;
-  wave_sync();
-  const uint32_t v_begin = lo_b[0], v_end = lo_b[nc];
-  const uint32_t w_begin = v_begin >> 5, w_last = (v_end - 1) >> 5;
-  const uint32_t* bm = bitmaps + (size_t)uniform(ops[o].u_slot) * W;
-  for (uint32_t w0 = w_begin; w0 <= w_last; w0 += WAVE) {
-    const uint32_t w = w0 + lane;
-    if (w > w_last) continue;
-    uint32_t x = bm[w];
-    if (w == w_begin) x &= ~0u << (v_begin & 31);
-    if (w == w_last && (v_end & 31)) x &= (1u << (v_end & 31)) - 1u;
-    while (x) {
-      const uint32_t v = w * 32 + (uint32_t)(__ffs(x) - 1);
-      uint32_t a = 0, b = nc;  // range a: lo_b[a] <= v < lo_b[a + 1]
-      while (b - a > 1) {
-        const uint32_t mid = (a + b) >> 1;
-        if (lo_b[mid] <= v) a = mid;
-        else b = mid;
+#pragma unroll
+  for (uint32_t j = 0; j < AR_GROUP / WAVE; ++j) act[j * WAVE + lane] = 0;
+  uint32_t o = 0, c0 = 0, nc = 0;
+  if (gw < (uint64_t)nops * groups) {  // uniform per wave
+    o = uniform((uint32_t)(gw / groups));
+    c0 = uniform((uint32_t)(gw % groups)) * AR_GROUP;
+    const uint32_t need = uniform(ops[o].need);
+    // an op whose previous refine round removed nothing lists no range
+    if (need == ~0u || uniform(rflags[need]) != 0) nc = min(AR_GROUP, chunks - c0);
+  }
+  if (nc) {
+    for (uint32_t j = lane; j <= nc; j += WAVE) lo_b[j] = fchunk[c0 + j];
+    wave_sync();
+    const uint32_t v_begin = lo_b[0], v_end = lo_b[nc];
+    const uint32_t w_begin = v_begin >> 5, w_last = (v_end - 1) >> 5;
+    const uint32_t* bm = bitmaps + (size_t)uniform(ops[o].u_slot) * W;
+    for (uint32_t w0 = w_begin; w0 <= w_last; w0 += WAVE * AR_UNROLL) {
+      uint32_t xs[AR_UNROLL], rs[AR_UNROLL];
+#pragma unroll
+      for (int u = 0; u < AR_UNROLL; ++u) {
+        const uint32_t w = w0 + u * WAVE + lane;
+        xs[u] = 0;
+        rs[u] = 0;
+        if (w <= w_last) {
+          xs[u] = bm[w];
+          rs[u] = wrange[w];
+        }
       }
-      act[a] = 1;
-      const uint32_t next = lo_b[a + 1];  // first vertex after range a (> v)
-      if (next >= w * 32 + 32) break;
-      x &= ~0u << (next - w * 32);
+#pragma unroll
+      for (int u = 0; u < AR_UNROLL; ++u) {
+        const uint32_t w = w0 + u * WAVE + lane;
+        uint32_t x = xs[u];
+        if (w == w_begin) x &= ~0u << (v_begin & 31);
+        if (w == w_last && (v_end & 31)) x &= (1u << (v_end & 31)) - 1u;
+        // range a holds the lowest remaining set bit: the word's first range
+        // (c0 or later once the bits below v_begin are masked), then forward
+        uint32_t a = rs[u] > c0 ? rs[u] - c0 : 0u;
+        while (x) {
+          const uint32_t v = w * 32 + (uint32_t)(__ffs(x) - 1);
+          while (lo_b[a + 1] <= v) ++a;
+          act[a] = 1;
+          const uint32_t next = lo_b[a + 1];  // first vertex after range a (> v)
+          if (next >= w * 32 + 32) break;
+          x &= ~0u << (next - w * 32);
+        }
+      }
     }
   }
   wave_sync();
-  const bool on = lane < nc && act[lane] != 0;
-  const unsigned long long m = __ballot(on);
-  if (!m) return;
-  uint32_t base = 0;
-  if (lane == 0) base = atomicAdd(nitems, (uint32_t)__popcll(m));
-  base = __shfl(base, 0, WAVE);
-  if (on) items[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = make_uint2(o, c0 + lane);
+  // lane l owns ranges l, l + 64, l + 128, l + 192 of the wave's group
+  uint32_t mine = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < AR_GROUP / WAVE; ++j) mine += act[j * WAVE + lane] != 0;
+  const uint32_t incl = wave_incl_scan(mine);
+  if (lane == WAVE - 1) s_wtot[wv] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (uint32_t x = 0; x < blockDim.x / WAVE; ++x) {
+      s_wbase[x] = tot;
+      tot += s_wtot[x];
+    }
+    const uint32_t base = tot ? atomicAdd(nitems, tot) : 0u;
+    for (uint32_t x = 0; x < blockDim.x / WAVE; ++x) s_wbase[x] += base;
+  }
+  __syncthreads();
+  uint32_t pos = s_wbase[wv] + incl - mine;
+#pragma unroll
+  for (uint32_t j = 0; j < AR_GROUP / WAVE; ++j)
+    if (act[j * WAVE + lane]) items[pos++] = make_uint2(o, c0 + j * WAVE + lane);
 }
 
 constexpr int UNROLL = 4;  // adjacency entries in flight per lane
@@ -156,6 +194,7 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
                                                           const uint32_t* __restrict__ nbr_lo,
                                                           const uint32_t* __restrict__ nbr_hi,
                                                           uint32_t* __restrict__ bitmaps,
+                                                          uint32_t* __restrict__ rflags,
                                                           unsigned long long* __restrict__ stats) {
   extern __shared__ unsigned char lds_raw[];
   WaveLds& L = reinterpret_cast<WaveLds*>(lds_raw)[threadIdx.x / WAVE];
@@ -254,7 +293,9 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
       }
 
       if (type == OP_REFINE) {
-        if (mine && h != full) clear_bit(bm_u, id);
+        const bool drop = mine && h != full;
+        if (drop) clear_bit(bm_u, id);
+        if (__ballot(drop) && lane == 0) rflags[uniform(ops[o].mark)] = 1u;
       } else {
         // first neighbour without a match: mu leaves C(u); neighbours before
         // it received all of mu's matching data neighbours (match/Match.cu:547-600)
@@ -331,13 +372,15 @@ void launch_init(const DevGraph& g, const InitOp* ops, uint32_t nops, uint32_t* 
 
 void launch_filter_step(const DevGraph& g, const FilterOp* ops, uint32_t nops, const uint32_t* nbr_slot,
                         const uint32_t* nbr_lo, const uint32_t* nbr_hi, uint32_t* bitmaps, uint32_t block,
-                        uint32_t max_wg, uint2* items, uint32_t* nitems, unsigned long long* stats, hipStream_t s) {
+                        uint32_t max_wg, uint2* items, uint32_t* nitems, uint32_t* rflags,
+                        unsigned long long* stats, hipStream_t s) {
   if (!nops) return;
   if (block != 64 && block != 128 && block != 256) block = 256;
   const uint64_t pairs = (uint64_t)nops * g.nfchunk;
   const uint64_t awaves = (uint64_t)nops * ((g.nfchunk + AR_GROUP - 1) / AR_GROUP);
-  hipLaunchKernelGGL(active_ranges_kernel, dim3((uint32_t)((awaves + 3) / 4)), dim3(256), 0, s, g.fchunk,
-                     g.nfchunk, ops, nops, bitmaps, g.W, items, nitems);
+  // four waves per workgroup: the kernel's LDS and its per-workgroup append assume 256 threads
+  hipLaunchKernelGGL(active_ranges_kernel, dim3((uint32_t)((awaves + 3) / 4)), dim3(256), 0, s, g.fchunk, g.wrange,
+                     g.nfchunk, ops, nops, bitmaps, g.W, rflags, items, nitems);
   // the list's length is known on the device only: a grid of at most
   // max_wg workgroups strides over it (waves past its end leave at once)
   const uint32_t wpb = block / WAVE;
@@ -345,7 +388,7 @@ void launch_filter_step(const DevGraph& g, const FilterOp* ops, uint32_t nops, c
   if (grid > max_wg) grid = max_wg;
   const size_t lds = sizeof(WaveLds) * wpb;
   hipLaunchKernelGGL(filter_step_kernel, dim3((uint32_t)grid), dim3(block), lds, s, g.row, g.col, g.ckey, g.W,
-                     g.fchunk, items, nitems, ops, nbr_slot, nbr_lo, nbr_hi, bitmaps, stats);
+                     g.fchunk, items, nitems, ops, nbr_slot, nbr_lo, nbr_hi, bitmaps, rflags, stats);
 }
 
 void filter_chunks(uint32_t n, const uint32_t* row_offset, std::vector<uint32_t>& bounds) {

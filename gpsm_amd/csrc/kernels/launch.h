@@ -26,6 +26,8 @@ struct DevGraph {
   // walks far more adjacency than the others (power-law hubs)
   uint32_t* fchunk = nullptr;
   uint32_t nfchunk = 0;
+  // wrange[w]: the vertex range holding vertex 32w (one per bitmap word)
+  uint32_t* wrange = nullptr;
   // hubs (degree above hub_degree): hub_of[v] = index of v's adjacency
   // bitmap (W words each, in hub_bits) or ~0u, so an edge test touching a
   // hub is one bit; other edge tests search the shorter adjacency list
@@ -66,11 +68,14 @@ void launch_init(const DevGraph& g, const InitOp* ops, uint32_t nops, uint32_t* 
 // The superstep visits only the (op, vertex range) pairs whose range holds a
 // candidate of C(u): a first kernel lists them in items (nops x g.nfchunk
 // capacity) and counts them in *nitems (zeroed by the caller), the second
-// processes the list with at most max_wg workgroups.
+// processes the list with at most max_wg workgroups.  rflags: the refine
+// rounds' change flags (FilterOp::mark / need), zeroed by the caller once
+// per batch; an op whose need flag is clear lists no range.
 constexpr uint32_t STAT_COPIES = 32, STAT_STRIDE = 8;
 void launch_filter_step(const DevGraph& g, const FilterOp* ops, uint32_t nops, const uint32_t* nbr_slot,
                         const uint32_t* nbr_lo, const uint32_t* nbr_hi, uint32_t* bitmaps, uint32_t block,
-                        uint32_t max_wg, uint2* items, uint32_t* nitems, unsigned long long* stats, hipStream_t s);
+                        uint32_t max_wg, uint2* items, uint32_t* nitems, uint32_t* rflags,
+                        unsigned long long* stats, hipStream_t s);
 
 // scan.hip ---------------------------------------------------------------------
 // Bitmap rows are compacted in segments of SEG_WORDS words, one workgroup
