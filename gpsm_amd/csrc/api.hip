@@ -93,6 +93,25 @@ void HostBuf::release() {
   bytes = 0;
 }
 
+void* PinnedArena::take(size_t nbytes) {
+  nbytes = (nbytes + 255) & ~(size_t)255;
+  while (cur < blocks.size() && used + nbytes > blocks[cur].bytes) {
+    ++cur;
+    used = 0;
+  }
+  if (cur == blocks.size()) {
+    blocks.emplace_back();
+    if (blocks.back().ensure(std::max<size_t>(nbytes, 1u << 20))) {
+      blocks.pop_back();
+      return nullptr;
+    }
+    used = 0;
+  }
+  void* p = static_cast<char*>(blocks[cur].ptr) + used;
+  used += nbytes;
+  return p;
+}
+
 const char* const kProfNames[K_KINDS_] = {"filter_step", "join",         "cet_count",   "cet_write",
                                            "compact_rows", "init",         "other",       "wall:filter",
                                            "wall:cet_count", "wall:plan", "wall:join"};
@@ -265,6 +284,7 @@ void gpsm_shutdown(gpsm_ctx* c) {
   hipStreamSynchronize(c->stream);
   for (auto& b : c->bufs) b.release();
   for (auto& b : c->h_stage) b.release();
+  c->arena.release();
   hipEventDestroy(c->ev0);
   hipEventDestroy(c->ev1);
   hipStreamDestroy(c->stream);
@@ -506,8 +526,12 @@ template <typename T>
 int upload(gpsm_ctx* c, int b, const std::vector<T>& v) {
   int rc = c->bufs[b].ensure(sizeof(T) * std::max<size_t>(v.size(), 1));
   if (rc) return rc;
-  if (!v.empty() && hipMemcpyAsync(c->bufs[b].ptr, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice,
-                                   c->stream) != hipSuccess)
+  if (v.empty()) return GPSM_OK;
+  // staged in the batch's pinned arena: the copy is queued, not waited for
+  void* h = c->arena.take(sizeof(T) * v.size());
+  if (!h) return GPSM_ENOMEM;
+  std::memcpy(h, v.data(), sizeof(T) * v.size());
+  if (hipMemcpyAsync(c->bufs[b].ptr, h, sizeof(T) * v.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess)
     return GPSM_EDEVICE;
   return GPSM_OK;
 }
@@ -535,6 +559,9 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   const GraphStats& st = g->stats;
   hipSetDevice(c->device);
   hipStream_t s = c->stream;
+  // the upload arena is reused from its start: no earlier copy may still read it
+  HIPCHK(hipStreamSynchronize(s));
+  c->arena.reset();
   gpsm_opts o{};
   if (opts) o = *opts;
   const uint32_t fblock = o.block ? o.block : c->tune.filter_block;
@@ -654,7 +681,8 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     {
       hipEvent_t a = pbegin();
       launch_init(dg, dptr<InitOp>(c, B_INIT), (uint32_t)init.size(), bitmaps, s);
-      pend(K_INIT, a, (double)init.size() * (4.0 * dg.n + 4.0 * W), -1);
+      // the keys are read once for all ops, each op's bitmap written once
+      pend(K_INIT, a, 4.0 * dg.n + (double)init.size() * 4.0 * W, -1);
     }
     for (uint32_t sidx = 0; sidx < maxsteps; ++sidx) {
       const uint32_t nops_s = step_off[sidx + 1] - step_off[sidx];

@@ -28,6 +28,22 @@ struct HostBuf {
   void release();
 };
 
+// Pinned staging for a batch's uploads: take() hands out fresh regions (in
+// blocks of at least 1 MB, kept for later batches) until reset(), which the
+// batch calls at its start once the stream is idle, so no queued copy still
+// reads a region that is handed out again.
+struct PinnedArena {
+  std::vector<HostBuf> blocks;
+  size_t cur = 0, used = 0;
+  void* take(size_t nbytes);
+  void reset() { cur = used = 0; }
+  void release() {
+    for (auto& b : blocks) b.release();
+    blocks.clear();
+    reset();
+  }
+};
+
 enum BufId {
   B_SCRATCH,
   B_BITMAPS,
@@ -149,6 +165,7 @@ struct gpsm_ctx {
   gpsm_mlp* planner = nullptr;
   std::vector<uint32_t> h_counts;
   gpsm::HostBuf h_stage[3];  // pinned readbacks of a batch: counts, |CET| descriptors, join results
+  gpsm::PinnedArena arena;   // pinned sources of a batch's uploads
   // hub adjacency bitmaps of later uploads: vertices of degree > the hub
   // degree, starting at hub_min_degree and doubled until they fit hub_max_bytes
   uint32_t hub_min_degree = 512;

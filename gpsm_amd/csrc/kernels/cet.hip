@@ -102,18 +102,34 @@ __global__ void __launch_bounds__(256) cet_count_kernel(const uint32_t* __restri
   }
 }
 
+// One workgroup per table scans its per-candidate counts in place, in tiles
+// of 256 x OFF_ITEMS: each thread sums OFF_ITEMS consecutive counts (loads
+// in flight together), one block scan places the thread sums, and each
+// thread writes its run.  A table of 10^5 candidates takes 49 tiles instead
+// of 391 block scans of 256.
+constexpr uint32_t OFF_ITEMS = 8;
+
 __global__ void __launch_bounds__(256) cet_offsets_kernel(const CetDesc* __restrict__ descs,
                                                           uint32_t* __restrict__ offsets) {
   __shared__ uint32_t tmp[8];
   const CetDesc d = descs[blockIdx.x];
   uint32_t* o = offsets + d.off_base;
   uint32_t carry = 0;
-  for (uint32_t t0 = 0; t0 < d.ncand; t0 += blockDim.x) {
-    const uint32_t i = t0 + threadIdx.x;
-    const uint32_t v = i < d.ncand ? o[i] : 0u;
+  for (uint32_t t0 = 0; t0 < d.ncand; t0 += blockDim.x * OFF_ITEMS) {
+    const uint32_t i0 = t0 + threadIdx.x * OFF_ITEMS;
+    uint32_t v[OFF_ITEMS], sum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < OFF_ITEMS; ++j) {
+      v[j] = i0 + j < d.ncand ? o[i0 + j] : 0u;
+      sum += v[j];
+    }
     uint32_t total;
-    const uint32_t ex = block_excl_scan(v, tmp, &total);
-    if (i < d.ncand) o[i] = min(carry + ex, d.list_cap);
+    uint32_t run = carry + block_excl_scan(sum, tmp, &total);
+#pragma unroll
+    for (uint32_t j = 0; j < OFF_ITEMS; ++j) {
+      if (i0 + j < d.ncand) o[i0 + j] = min(run, d.list_cap);
+      run += v[j];
+    }
     carry += total;
   }
   if (threadIdx.x == 0) o[d.ncand] = min(carry, d.list_cap);

@@ -24,26 +24,35 @@ namespace {
 constexpr int MAX_CAND = FCHUNK_VERTS;  // candidates of one vertex range
 static_assert(FCHUNK_VERTS <= 32 * (WAVE - 1), "a range's bitmap words fit one wave");
 
-__global__ void init_kernel(const uint32_t* __restrict__ key, uint32_t n, uint32_t W, const InitOp* __restrict__ ops,
-                            uint32_t nops, uint32_t* __restrict__ bitmaps) {
-  const uint32_t pairs = (W + 1) / 2;
-  const uint64_t items = (uint64_t)nops * pairs;
+// Start-vertex candidates of every op: lane l of a wave owns bitmap word
+// w = 64 * block + l, loads the keys of its 32 vertices once (kept in
+// registers) and builds the word of each op of its group, so the keys are
+// read once for all ops (not once per op) and every op's words leave as one
+// coalesced store per wave.
+constexpr uint32_t INIT_OPS = 16;  // ops per wave
+
+__global__ void __launch_bounds__(256) init_kernel(const uint32_t* __restrict__ key, uint32_t n, uint32_t W,
+                                                   const InitOp* __restrict__ ops, uint32_t nops,
+                                                   uint32_t* __restrict__ bitmaps) {
   const uint32_t lane = lane_id();
+  const uint32_t blocks = (W + WAVE - 1) / WAVE, groups = (nops + INIT_OPS - 1) / INIT_OPS;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / WAVE;
-  for (uint64_t it = wave; it < items; it += nwaves) {
-    const uint32_t o = (uint32_t)(it / pairs), p = (uint32_t)(it % pairs);
+  if (wave >= (uint64_t)blocks * groups) return;  // the whole wave
+  const uint32_t wb = (uint32_t)(wave % blocks), og = (uint32_t)(wave / blocks);
+  const uint32_t w = wb * WAVE + lane;
+  if (w >= W) return;
+  const uint32_t v0 = w * 32;
+  const uint32_t valid = n - v0 >= 32 ? ~0u : (1u << (n - v0)) - 1u;
+  uint32_t k[32];
+#pragma unroll
+  for (uint32_t j = 0; j < 32; ++j) k[j] = v0 + j < n ? key[v0 + j] : 0u;
+  const uint32_t o_end = min(nops, (og + 1) * INIT_OPS);
+  for (uint32_t o = og * INIT_OPS; o < o_end; ++o) {
     const InitOp op = ops[o];
-    const uint32_t v = p * 64 + lane;
-    bool pred = false;
-    if (v < n) {
-      uint32_t k = key[v];
-      pred = k >= op.lo && k <= op.hi;
-    }
-    const unsigned long long m = __ballot(pred);
-    uint32_t* bm = bitmaps + (size_t)op.slot * W;
-    if (lane == 0) bm[2 * p] = (uint32_t)m;
-    if (lane == 1 && 2 * p + 1 < W) bm[2 * p + 1] = (uint32_t)(m >> 32);
+    uint32_t word = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 32; ++j) word |= (uint32_t)(k[j] >= op.lo && k[j] <= op.hi) << j;
+    bitmaps[(size_t)op.slot * W + w] = word & valid;
   }
 }
 
@@ -363,11 +372,9 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
 
 void launch_init(const DevGraph& g, const InitOp* ops, uint32_t nops, uint32_t* bitmaps, hipStream_t s) {
   if (!nops) return;
-  const uint64_t items = (uint64_t)nops * ((g.W + 1) / 2);
-  const uint32_t block = 256;
-  uint64_t grid = (items * WAVE + block - 1) / block;
-  if (grid > 65536) grid = 65536;
-  hipLaunchKernelGGL(init_kernel, dim3((uint32_t)grid), dim3(block), 0, s, g.key, g.n, g.W, ops, nops, bitmaps);
+  const uint64_t waves = (uint64_t)((g.W + WAVE - 1) / WAVE) * ((nops + INIT_OPS - 1) / INIT_OPS);
+  hipLaunchKernelGGL(init_kernel, dim3((uint32_t)((waves + 3) / 4)), dim3(256), 0, s, g.key, g.n, g.W, ops, nops,
+                     bitmaps);
 }
 
 void launch_filter_step(const DevGraph& g, const FilterOp* ops, uint32_t nops, const uint32_t* nbr_slot,
