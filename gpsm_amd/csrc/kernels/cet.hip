@@ -226,15 +226,21 @@ __global__ void __launch_bounds__(256) cet_terms_kernel(const uint32_t* __restri
   }
 }
 
-__global__ void chunk_base_kernel(const CetDesc* __restrict__ descs, uint32_t ndesc, uint32_t* __restrict__ base) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    uint32_t acc = 0;
-    for (uint32_t d = 0; d < ndesc; ++d) {
-      base[d] = acc;
-      acc += (descs[d].ncand + WAVE - 1) / WAVE;
-    }
-    base[ndesc] = acc;
+// base[d] = chunks of 64 candidates in the tables before d (exclusive scan
+// over the tables, one workgroup of 256 threads).
+__global__ void __launch_bounds__(256) chunk_base_kernel(const CetDesc* __restrict__ descs, uint32_t ndesc,
+                                                         uint32_t* __restrict__ base) {
+  __shared__ uint32_t tmp[8];
+  uint32_t carry = 0;
+  for (uint32_t d0 = 0; d0 < ndesc; d0 += blockDim.x) {
+    const uint32_t d = d0 + threadIdx.x;
+    const uint32_t ch = d < ndesc ? (descs[d].ncand + WAVE - 1) / WAVE : 0u;
+    uint32_t total;
+    const uint32_t ex = block_excl_scan(ch, tmp, &total);
+    if (d < ndesc) base[d] = carry + ex;
+    carry += total;
   }
+  if (threadIdx.x == 0) base[ndesc] = carry;
 }
 
 }  // namespace
@@ -243,7 +249,7 @@ void launch_cet_count(const DevGraph& g, const uint32_t* bitmaps, const uint32_t
                       uint32_t ndesc, uint32_t* offsets, int per_cand, uint32_t* chunk_base, uint32_t grid,
                       hipStream_t s) {
   if (!ndesc) return;
-  hipLaunchKernelGGL(chunk_base_kernel, dim3(1), dim3(64), 0, s, descs, ndesc, chunk_base);
+  hipLaunchKernelGGL(chunk_base_kernel, dim3(1), dim3(256), 0, s, descs, ndesc, chunk_base);
   hipLaunchKernelGGL(cet_count_kernel, dim3(grid), dim3(256), 0, s, g.row, g.col, g.ckey, bitmaps, g.W, clist,
                      descs, chunk_base, ndesc, offsets, per_cand);
 }
@@ -267,7 +273,7 @@ void launch_cet_write(const DevGraph& g, const uint32_t* bitmaps, const uint32_t
                       uint32_t ndesc, const uint32_t* offsets, uint32_t* lists, uint32_t* chunk_base, uint32_t grid,
                       hipStream_t s) {
   if (!ndesc) return;
-  hipLaunchKernelGGL(chunk_base_kernel, dim3(1), dim3(64), 0, s, descs, ndesc, chunk_base);
+  hipLaunchKernelGGL(chunk_base_kernel, dim3(1), dim3(256), 0, s, descs, ndesc, chunk_base);
   hipLaunchKernelGGL(cet_write_kernel, dim3(grid), dim3(256), 0, s, g.row, g.col, g.ckey, bitmaps, g.W, clist,
                      descs, chunk_base, ndesc, offsets, lists);
 }
