@@ -57,7 +57,7 @@ def test_autotuner_keeps_results(ctx, mix):
         ctx.set_tuning(256, 1024, 4, 8)
     small = rg.match_batch(qs)
     assert [(r.count, r.hash) for r in small] == [(r.count, r.hash) for r in before]
-    ctx.set_tuning(256, 4096, 4, 8)  # the defaults, for the tests after this one
+    ctx.set_tuning(128, 4096, 8, 32)  # the defaults, for the tests after this one
 
 
 @pytest.mark.skipif(not os.path.exists(planner.WEIGHTS), reason="no trained planner weights")
