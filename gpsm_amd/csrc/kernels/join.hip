@@ -73,6 +73,26 @@ __device__ __forceinline__ bool has_edge(const uint32_t* __restrict__ row, const
   return *p == key;
 }
 
+// For K keys at once (key[k] in, index out): the largest i in [0, n) with
+// pre[i] <= key[k] (pre non-decreasing, pre[0] <= every key).  ceil(log2 n)
+// halvings with no early exit: n must be the same in every lane, so every
+// lane takes the same steps, and each step's K LDS reads are in flight
+// together (a per-key loop waits for one read at a time).
+template <int K>
+__device__ __forceinline__ void halving_search(const uint32_t* __restrict__ pre, uint32_t n, uint32_t* key) {
+  uint32_t b[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) b[k] = 0;
+  while (n > 1) {
+    const uint32_t half = n >> 1;
+#pragma unroll
+    for (int k = 0; k < K; ++k) b[k] = pre[b[k] + half] <= key[k] ? b[k] + half : b[k];
+    n -= half;
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) key[k] = b[k];
+}
+
 struct JoinShared {
   uint32_t pre[WROWS + 1];
   uint32_t delta[WROWS];  // list position of a row's candidate g is delta[row] + g
@@ -246,10 +266,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) j
             uint32_t rlo = 0, rn = 1;
             if (g0 < T) {  // uniform per wave
               const uint32_t glast = min(g0 + WAVE * EXP_UNROLL - 1, T - 1);
-              const uint32_t r = seg_search(S.pre, nw, lane == 0 ? g0 : glast);
-              rlo = __shfl(r, 0, WAVE);
-              rn = __shfl(r, 1, WAVE) - rlo + 1;
+              uint32_t r = lane == 0 ? g0 : glast;
+              halving_search<1>(S.pre, uniform(nw), &r);
+              rlo = uniform(__shfl(r, 0, WAVE));
+              rn = uniform(__shfl(r, 1, WAVE)) - rlo + 1;
             }
+            // each candidate's row inside the bracket, the EXP_UNROLL
+            // searches of a lane stepping together
+            uint32_t rb[EXP_UNROLL];
+#pragma unroll
+            for (int u = 0; u < EXP_UNROLL; ++u) rb[u] = g0 + u * WAVE + lane;
+            halving_search<EXP_UNROLL>(S.pre + rlo, rn, rb);
 #pragma unroll
             for (int u = 0; u < EXP_UNROLL; ++u) {
               const uint32_t g = g0 + u * WAVE + lane;
@@ -257,7 +284,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) j
               mvs[u] = 0;
               trm[u] = 0;
               if (g < T) {
-                const uint32_t i = rlo + seg_search(S.pre + rlo, rn, g);
+                const uint32_t i = rlo + rb[u];
                 is[u] = i;
                 const uint32_t e = S.delta[i] + g;
                 mvs[u] = lst[e];
