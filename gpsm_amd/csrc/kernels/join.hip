@@ -28,6 +28,7 @@ namespace gpsm {
 namespace {
 
 constexpr uint32_t WROWS = 256;  // rows per window = threads per workgroup
+static_assert((WROWS & (WROWS - 1)) == 0, "row indices are masked with WROWS - 1");
 constexpr int EXP_UNROLL = 4;    // candidates in flight per thread
 
 __device__ __forceinline__ uint32_t rank_of(const uint32_t* __restrict__ bm, const uint32_t* __restrict__ bp,
@@ -108,7 +109,8 @@ struct JoinShared {
 
 // BYTES: account the algorithmic bytes read and written (JoinOut::bytes);
 // the kernel without it is the one timed.  Seven waves per SIMD: the timed
-// variant fits 72 VGPRs without spills (eight would spill).
+// variant fits 72 VGPRs with two values spilled; on the benchmark that is
+// faster than six waves per SIMD with none (8.64-8.72 vs 8.75-8.78 ms).
 template <bool BYTES>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) join_kernel(JoinArgs a) {
   extern __shared__ unsigned char lds_raw[];
@@ -277,30 +279,44 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) j
 #pragma unroll
             for (int u = 0; u < EXP_UNROLL; ++u) rb[u] = g0 + u * WAVE + lane;
             halving_search<EXP_UNROLL>(S.pre + rlo, rn, rb);
+            // LDS reads of the EXP_UNROLL candidates are issued together
+            // (row offsets, then the bound vertices the checks compare
+            // with), so a lane waits once per kind rather than once per
+            // candidate
+            uint32_t es[EXP_UNROLL];
 #pragma unroll
             for (int u = 0; u < EXP_UNROLL; ++u) {
               const uint32_t g = g0 + u * WAVE + lane;
-              is[u] = WROWS;
+              const uint32_t i = rlo + rb[u];  // a row of the window even when g >= T
+              is[u] = g < T ? i : WROWS;
+              es[u] = S.delta[i] + g;
               mvs[u] = 0;
               trm[u] = 0;
-              if (g < T) {
-                const uint32_t i = rlo + rb[u];
-                is[u] = i;
-                const uint32_t e = S.delta[i] + g;
-                mvs[u] = lst[e];
+            }
+#pragma unroll
+            for (int u = 0; u < EXP_UNROLL; ++u) {
+              if (is[u] != WROWS) {
+                mvs[u] = lst[es[u]];
                 // last level: the entry's hash term, computed once per list entry
-                if constexpr (FINAL) trm[u] = trm_n[e];
+                if constexpr (FINAL) trm[u] = trm_n[es[u]];
               }
+            }
+            bool oks[EXP_UNROLL];
+#pragma unroll
+            for (int u = 0; u < EXP_UNROLL; ++u) oks[u] = is[u] != WROWS;
+            // injectivity: only the positions bound to a vertex of the same
+            // label that no edge test already separates
+            for (uint64_t m = inj; m; m &= m - 1) {
+              const uint32_t c = (uint32_t)(__ffsll((long long)m) - 1);
+#pragma unroll
+              for (int u = 0; u < EXP_UNROLL; ++u) oks[u] &= val[c * WROWS + (is[u] & (WROWS - 1))] != mvs[u];
             }
 #pragma unroll
             for (int u = 0; u < EXP_UNROLL; ++u) {
               const uint32_t i = is[u], mv = mvs[u];
-              bool ok = i != WROWS;
+              bool ok = oks[u];
               if (ok) {
                 if (BYTES) my_bytes += 4;  // the list entry (its hash term is a derived copy of it)
-                // injectivity: only the positions bound to a vertex of the
-                // same label that no edge test already separates
-                for (uint64_t m = inj; m; m &= m - 1) ok &= val[(uint32_t)(__ffsll((long long)m) - 1) * WROWS + i] != mv;
                 for (uint32_t b = bo; b < be && ok; ++b)
                   ok = has_edge<BYTES>(a.row, a.col, a.hub_of, a.hub_bits, a.W, mv, val[P.backs[b] * WROWS + i],
                                        my_bytes);
