@@ -75,16 +75,19 @@ __global__ void __launch_bounds__(256) cet_count_kernel(const uint32_t* __restri
     for (uint32_t f = 0; f < D; f += WAVE * CET_UNROLL) {
       // CET_UNROLL entries per lane: all adjacency and key loads, then the
       // probes of the entries whose key is in range
-      uint32_t cis[CET_UNROLL], mvs[CET_UNROLL], ks[CET_UNROLL];
+      uint32_t cis[CET_UNROLL], mvs[CET_UNROLL], ks[CET_UNROLL], es[CET_UNROLL];
+#pragma unroll
+      for (int u = 0; u < CET_UNROLL; ++u) cis[u] = f + u * WAVE + lane;
+      halving_search<CET_UNROLL>(L.pre, WAVE, cis);
+#pragma unroll
+      for (int u = 0; u < CET_UNROLL; ++u) es[u] = L.off[cis[u]] + (f + u * WAVE + lane - L.pre[cis[u]]);
 #pragma unroll
       for (int u = 0; u < CET_UNROLL; ++u) {
-        const uint32_t g = f + u * WAVE + lane;
-        cis[u] = WAVE;
-        if (g < D) {
-          cis[u] = seg_search(L.pre, WAVE, g);
-          const uint32_t e = L.off[cis[u]] + (g - L.pre[cis[u]]);
-          mvs[u] = col[e];
-          ks[u] = ckey[e];
+        if (f + u * WAVE + lane < D) {
+          mvs[u] = col[es[u]];
+          ks[u] = ckey[es[u]];
+        } else {
+          cis[u] = WAVE;
         }
       }
 #pragma unroll
@@ -169,18 +172,20 @@ __global__ void __launch_bounds__(256) cet_write_kernel(const uint32_t* __restri
       // (lists stay in adjacency order)
       uint32_t cis[CET_UNROLL], mvs[CET_UNROLL], ks[CET_UNROLL];
       bool hits[CET_UNROLL];
+      uint32_t es[CET_UNROLL];
+#pragma unroll
+      for (int u = 0; u < CET_UNROLL; ++u) cis[u] = f + u * WAVE + lane;
+      halving_search<CET_UNROLL>(L.pre, WAVE, cis);
+#pragma unroll
+      for (int u = 0; u < CET_UNROLL; ++u) es[u] = L.off[cis[u]] + (f + u * WAVE + lane - L.pre[cis[u]]);
 #pragma unroll
       for (int u = 0; u < CET_UNROLL; ++u) {
-        const uint32_t g = f + u * WAVE + lane;
-        cis[u] = 0;
         mvs[u] = 0;
         ks[u] = 0;
         hits[u] = false;
-        if (g < D) {
-          cis[u] = seg_search(L.pre, WAVE, g);
-          const uint32_t e = L.off[cis[u]] + (g - L.pre[cis[u]]);
-          mvs[u] = col[e];
-          ks[u] = ckey[e];
+        if (f + u * WAVE + lane < D) {
+          mvs[u] = col[es[u]];
+          ks[u] = ckey[es[u]];
         }
       }
 #pragma unroll

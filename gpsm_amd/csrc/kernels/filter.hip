@@ -275,16 +275,16 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
         wave_sync();
         // refine needs the neighbour ids, explore only their keys
         const uint32_t* __restrict__ src = type == OP_REFINE ? col : ckey;
-        uint32_t mvs[UNROLL], cis[UNROLL];
+        uint32_t mvs[UNROLL], cis[UNROLL], es[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) cis[u] = u * WAVE + lane;
+        halving_search<UNROLL>(L.pre, WAVE, cis);
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) es[u] = L.base[cis[u]] + (u * WAVE + lane - L.pre[cis[u]]);
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
-          const uint32_t g = u * WAVE + lane;
-          cis[u] = WAVE;
-          if (g < T) {
-            const uint32_t ci = seg_search(L.pre, WAVE, g);
-            cis[u] = ci;
-            mvs[u] = src[L.base[ci] + (g - L.pre[ci])];
-          }
+          if (u * WAVE + lane < T) mvs[u] = src[es[u]];
+          else cis[u] = WAVE;
         }
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
@@ -320,17 +320,19 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
         wave_sync();
         // pass 2: every matching data neighbour of a surviving prefix enters C(v_i)
         for (uint32_t f = 0; f < T; f += UNROLL * WAVE) {
-          uint32_t mvs[UNROLL], ks[UNROLL], cis[UNROLL];
+          uint32_t mvs[UNROLL], ks[UNROLL], cis[UNROLL], es[UNROLL];
+#pragma unroll
+          for (int u = 0; u < UNROLL; ++u) cis[u] = f + u * WAVE + lane;
+          halving_search<UNROLL>(L.pre, WAVE, cis);
+#pragma unroll
+          for (int u = 0; u < UNROLL; ++u) es[u] = L.base[cis[u]] + (f + u * WAVE + lane - L.pre[cis[u]]);
 #pragma unroll
           for (int u = 0; u < UNROLL; ++u) {
-            const uint32_t g = f + u * WAVE + lane;
-            cis[u] = WAVE;
-            if (g < T) {
-              const uint32_t ci = seg_search(L.pre, WAVE, g);
-              cis[u] = ci;
-              const uint32_t e = L.base[ci] + (g - L.pre[ci]);
-              mvs[u] = col[e];
-              ks[u] = ckey[e];
+            if (f + u * WAVE + lane < T) {
+              mvs[u] = col[es[u]];
+              ks[u] = ckey[es[u]];
+            } else {
+              cis[u] = WAVE;
             }
           }
 #pragma unroll

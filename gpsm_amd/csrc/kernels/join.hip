@@ -74,26 +74,6 @@ __device__ __forceinline__ bool has_edge(const uint32_t* __restrict__ row, const
   return *p == key;
 }
 
-// For K keys at once (key[k] in, index out): the largest i in [0, n) with
-// pre[i] <= key[k] (pre non-decreasing, pre[0] <= every key).  ceil(log2 n)
-// halvings with no early exit: n must be the same in every lane, so every
-// lane takes the same steps, and each step's K LDS reads are in flight
-// together (a per-key loop waits for one read at a time).
-template <int K>
-__device__ __forceinline__ void halving_search(const uint32_t* __restrict__ pre, uint32_t n, uint32_t* key) {
-  uint32_t b[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) b[k] = 0;
-  while (n > 1) {
-    const uint32_t half = n >> 1;
-#pragma unroll
-    for (int k = 0; k < K; ++k) b[k] = pre[b[k] + half] <= key[k] ? b[k] + half : b[k];
-    n -= half;
-  }
-#pragma unroll
-  for (int k = 0; k < K; ++k) key[k] = b[k];
-}
-
 struct JoinShared {
   uint32_t pre[WROWS + 1];
   uint32_t delta[WROWS];  // list position of a row's candidate g is delta[row] + g
