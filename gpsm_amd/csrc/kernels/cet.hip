@@ -90,9 +90,14 @@ __global__ void __launch_bounds__(256) cet_count_kernel(const uint32_t* __restri
           cis[u] = WAVE;
         }
       }
+      // the probes' bitmap words loaded together, then tested
+      uint32_t pw[CET_UNROLL];
 #pragma unroll
       for (int u = 0; u < CET_UNROLL; ++u)
-        if (cis[u] != WAVE && ks[u] >= wlo && ks[u] <= whi && bit_of(bm_w, mvs[u])) {
+        pw[u] = cis[u] != WAVE && ks[u] >= wlo && ks[u] <= whi ? bm_w[mvs[u] >> 5] >> (mvs[u] & 31) : 0u;
+#pragma unroll
+      for (int u = 0; u < CET_UNROLL; ++u)
+        if (pw[u] & 1u) {
           ++tot;
           if (per_cand) atomicAdd(&L.cnt[cis[u]], 1u);
         }
@@ -188,9 +193,13 @@ __global__ void __launch_bounds__(256) cet_write_kernel(const uint32_t* __restri
           ks[u] = ckey[es[u]];
         }
       }
+      // the probes' bitmap words loaded together, then tested
+      uint32_t pw[CET_UNROLL];
 #pragma unroll
       for (int u = 0; u < CET_UNROLL; ++u)
-        if (f + u * WAVE + lane < D && ks[u] >= wlo && ks[u] <= whi) hits[u] = bit_of(bm_w, mvs[u]) != 0;
+        pw[u] = f + u * WAVE + lane < D && ks[u] >= wlo && ks[u] <= whi ? bm_w[mvs[u] >> 5] >> (mvs[u] & 31) : 0u;
+#pragma unroll
+      for (int u = 0; u < CET_UNROLL; ++u) hits[u] = (pw[u] & 1u) != 0;
 #pragma unroll
       for (int u = 0; u < CET_UNROLL; ++u) {
         const uint32_t fu = f + u * WAVE, ci = cis[u];

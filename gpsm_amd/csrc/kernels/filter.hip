@@ -167,31 +167,6 @@ struct WaveLds {
   uint32_t lim[WAVE];
 };
 
-// Neighbour constraints of one adjacency entry: refine probes C(v_i) with
-// the entry's vertex mv for the constraints not yet met; explore compares
-// the entry's key (mv holds ckey[e] then) with v_i's label/degree range.
-__device__ __forceinline__ unsigned long long constraint_bits(uint32_t type, uint32_t nv, uint32_t nb, uint32_t mv,
-                                                              unsigned long long known,
-                                                              const uint32_t* __restrict__ nbr_slot,
-                                                              const uint32_t* __restrict__ nbr_lo,
-                                                              const uint32_t* __restrict__ nbr_hi,
-                                                              const uint32_t* __restrict__ bitmaps, uint32_t W,
-                                                              uint32_t& t_probe) {
-  unsigned long long bits = 0;
-  if (type == OP_REFINE) {
-    for (uint32_t i = 0; i < nv; ++i) {
-      if ((known >> i) & 1ULL) continue;
-      ++t_probe;
-      if (bit_of(bitmaps + (size_t)nbr_slot[nb + i] * W, mv)) bits |= 1ULL << i;
-    }
-  } else {
-    const uint32_t k = mv;  // read in place of the column index: no extra bytes
-    for (uint32_t i = 0; i < nv; ++i)
-      if (k >= nbr_lo[nb + i] && k <= nbr_hi[nb + i]) bits |= 1ULL << i;
-  }
-  return bits;
-}
-
 __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __restrict__ row,
                                                           const uint32_t* __restrict__ col,
                                                           const uint32_t* __restrict__ ckey, uint32_t W,
@@ -283,16 +258,48 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
         for (int u = 0; u < UNROLL; ++u) es[u] = L.base[cis[u]] + (u * WAVE + lane - L.pre[cis[u]]);
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
+          mvs[u] = 0;
           if (u * WAVE + lane < T) mvs[u] = src[es[u]];
           else cis[u] = WAVE;
+        }
+        // constraints each entry meets, the UNROLL entries of a lane side by
+        // side: their candidates' known bits read together, then per
+        // neighbour v_i one probe (refine) or key comparison (explore) each,
+        // the probes' loads in flight together
+        unsigned long long kn[UNROLL], bits[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+          kn[u] = cis[u] != WAVE ? L.hit[cis[u]] : ~0ULL;
+          bits[u] = 0;
+        }
+        if (type == OP_REFINE) {
+          for (uint32_t i = 0; i < nv; ++i) {
+            const uint32_t* bm = bitmaps + (size_t)nbr_slot[nb + i] * W;
+            uint32_t wv[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) wv[u] = (kn[u] >> i) & 1ULL ? 0u : bm[mvs[u] >> 5];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u)
+              if (!((kn[u] >> i) & 1ULL)) {
+                ++t_probe;
+                if ((wv[u] >> (mvs[u] & 31)) & 1u) bits[u] |= 1ULL << i;
+              }
+          }
+        } else {
+          // explore compares the entry's key (read in place of the column
+          // index: no extra bytes) with v_i's label/degree range
+          for (uint32_t i = 0; i < nv; ++i) {
+            const uint32_t lo = nbr_lo[nb + i], hi = nbr_hi[nb + i];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u)
+              if (mvs[u] >= lo && mvs[u] <= hi) bits[u] |= 1ULL << i;
+          }
         }
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
           if (cis[u] == WAVE) continue;
           ++t_ent;
-          const unsigned long long bits =
-              constraint_bits(type, nv, nb, mvs[u], L.hit[cis[u]], nbr_slot, nbr_lo, nbr_hi, bitmaps, W, t_probe);
-          if (bits) atomicOr(&L.hit[cis[u]], bits);
+          if (bits[u]) atomicOr(&L.hit[cis[u]], bits[u]);
         }
         wave_sync();
         // entries of this lane's candidate covered by the UNROLL*64 examined
@@ -335,17 +342,36 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
               cis[u] = WAVE;
             }
           }
+          // the UNROLL entries of a lane side by side: their prefix limits
+          // read together, then per neighbour v_i the bitmap words of the
+          // entries that enter C(v_i) loaded together, and a bit set (one
+          // atomic) only where it is still clear
+          uint32_t lms[UNROLL];
 #pragma unroll
           for (int u = 0; u < UNROLL; ++u) {
-            if (cis[u] == WAVE) continue;
-            const uint32_t k = ks[u];
-            const uint32_t lm = L.lim[cis[u]];
-            ++t_ent;
-            ++t_key;
-            for (uint32_t i = 0; i < lm; ++i)
-              if (k >= nbr_lo[nb + i] && k <= nbr_hi[nb + i]) {
-                set_bit(bitmaps + (size_t)nbr_slot[nb + i] * W, mvs[u]);
+            lms[u] = cis[u] != WAVE ? L.lim[cis[u]] : 0u;
+            if (cis[u] != WAVE) {
+              ++t_ent;
+              ++t_key;
+            }
+          }
+          for (uint32_t i = 0; i < nv; ++i) {  // lim <= nv
+            const uint32_t lo = nbr_lo[nb + i], hi = nbr_hi[nb + i];
+            uint32_t* bm = bitmaps + (size_t)nbr_slot[nb + i] * W;
+            bool want[UNROLL];
+            uint32_t cw[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+              want[u] = i < lms[u] && ks[u] >= lo && ks[u] <= hi;
+              cw[u] = want[u] ? __hip_atomic_load(bm + (mvs[u] >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                              : ~0u;
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u)
+              if (want[u]) {
                 ++t_set;
+                const uint32_t m = 1u << (mvs[u] & 31);
+                if (!(cw[u] & m)) atomicOr(bm + (mvs[u] >> 5), m);
               }
           }
         }
