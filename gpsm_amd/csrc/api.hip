@@ -198,7 +198,7 @@ extern "C" {
 
 int gpsm_profile_enable(gpsm_ctx* c, int on) {
   if (!c) return GPSM_EINVAL;
-  if (on < 0 || on > 2) return GPSM_EINVAL;
+  if (on < 0 || on > 3) return GPSM_EINVAL;
   c->prof.mode = on;
   for (auto& a : c->prof.acc) a = Prof::Acc();
   for (auto& p : c->prof.pending) {
@@ -590,20 +590,22 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   int rc;
   // per-launch timing (not inside captured graphs)
   const bool prof = c->prof.mode > 0 && !o.use_graph;
-  const bool counters = prof && c->prof.mode >= 2;
+  const bool counters = prof && c->prof.mode == 2;
   for (auto& p : c->prof.pending) {
     c->prof.pool.push_back(p.a);
     c->prof.pool.push_back(p.b);
   }
   c->prof.pending.clear();
-  auto pbegin = [&]() -> hipEvent_t {
-    if (!prof) return nullptr;
+  // mode 3 times the join alone: no events between the other launches
+  const bool join_only = c->prof.mode == 3;
+  auto pbegin = [&](int kind) -> hipEvent_t {
+    if (!prof || (join_only && kind != K_JOIN)) return nullptr;
     hipEvent_t e = c->prof.get();
     hipEventRecord(e, s);
     return e;
   };
   auto pend = [&](int kind, hipEvent_t a, double bytes, int slot) {
-    if (!prof) return;
+    if (!prof || !a) return;
     hipEvent_t b = c->prof.get();
     hipEventRecord(b, s);
     c->prof.pending.push_back({kind, a, b, bytes, slot});
@@ -679,14 +681,14 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     launch_fill_u32(dptr<uint32_t>(c, B_RFLAGS), 0u, 4ull * std::max<uint32_t>(nq, 1), s);
     if (counters) launch_fill_u32(reinterpret_cast<uint32_t*>(fstats), 0u, 2ull * fstat_step * (maxsteps + 1), s);
     {
-      hipEvent_t a = pbegin();
+      hipEvent_t a = pbegin(K_INIT);
       launch_init(dg, dptr<InitOp>(c, B_INIT), (uint32_t)init.size(), bitmaps, s);
       // the keys are read once for all ops, each op's bitmap written once
       pend(K_INIT, a, 4.0 * dg.n + (double)init.size() * 4.0 * W, -1);
     }
     for (uint32_t sidx = 0; sidx < maxsteps; ++sidx) {
       const uint32_t nops_s = step_off[sidx + 1] - step_off[sidx];
-      hipEvent_t a = pbegin();
+      hipEvent_t a = pbegin(K_FILTER);
       launch_filter_step(dg, dptr<FilterOp>(c, B_FOPS) + step_off[sidx], nops_s, dptr<uint32_t>(c, B_NBR_SLOT),
                          dptr<uint32_t>(c, B_NBR_LO), dptr<uint32_t>(c, B_NBR_HI), bitmaps, fblock,
                          65536, dptr<uint2>(c, B_FITEMS), dptr<uint32_t>(c, B_FNITEMS) + sidx,
@@ -732,7 +734,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   if ((rc = c->bufs[B_CLIST].ensure(sizeof(uint32_t) * std::max<uint64_t>(coff[nslots], 1)))) return rc;
   if ((rc = c->bufs[B_BPREFIX].ensure(sizeof(uint32_t) * (size_t)std::max<uint32_t>(nslots, 1) * NB))) return rc;
   {
-    hipEvent_t a = pbegin();
+    hipEvent_t a = pbegin(K_COMPACT);
     launch_compact_rows(dg, bitmaps, nslots, dptr<uint32_t>(c, B_SEGC), dptr<uint64_t>(c, B_COFF),
                         dptr<uint32_t>(c, B_CLIST), dptr<uint32_t>(c, B_BPREFIX), s);
     pend(K_COMPACT, a, (double)nslots * 4.0 * (W + NB) + 4.0 * coff[nslots], -1);
@@ -770,7 +772,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   if ((rc = c->bufs[B_OFFSETS].ensure(sizeof(uint32_t) * std::max<uint64_t>(fwd_words + rev_bound, 1)))) return rc;
   const uint64_t off_capacity = c->bufs[B_OFFSETS].bytes / sizeof(uint32_t);
   {
-    hipEvent_t a = pbegin();
+    hipEvent_t a = pbegin(K_CET_COUNT);
     launch_cet_count(dg, bitmaps, dptr<uint32_t>(c, B_CLIST), dptr<CetDesc>(c, B_FDESC), (uint32_t)fdesc.size(),
                      dptr<uint32_t>(c, B_OFFSETS), 1, dptr<uint32_t>(c, B_CHUNKS), cgrid, s);
     pend(K_CET_COUNT, a, 0.0, -1);
@@ -987,13 +989,13 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     if (o.use_graph && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) captured = true;
-    hipEvent_t pa = pbegin();
+    hipEvent_t pa = pbegin(K_CET_COUNT);
     if (!cdesc.empty())
       launch_cet_count(dg, bitmaps, dptr<uint32_t>(c, B_CLIST), dptr<CetDesc>(c, B_CDESC), (uint32_t)cdesc.size(),
                        dptr<uint32_t>(c, B_OFFSETS), 1, dptr<uint32_t>(c, B_CHUNKS), cgrid, s);
     pend(K_CET_COUNT, pa, 0.0, -1);
     launch_cet_offsets(dptr<CetDesc>(c, B_JDESC), (uint32_t)jdesc.size(), dptr<uint32_t>(c, B_OFFSETS), s);
-    pa = pbegin();
+    pa = pbegin(K_CET_WRITE);
     launch_cet_write(dg, bitmaps, dptr<uint32_t>(c, B_CLIST), dptr<CetDesc>(c, B_JDESC), (uint32_t)jdesc.size(),
                      dptr<uint32_t>(c, B_OFFSETS), dptr<uint32_t>(c, B_LISTS), dptr<uint32_t>(c, B_CHUNKS), cgrid, s);
     launch_cet_terms(dptr<uint32_t>(c, B_LISTS), dptr<CetDesc>(c, B_JDESC), dptr<uint32_t>(c, B_TERM_DESC),
@@ -1033,7 +1035,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     }
     ja.budget = o.work_budget;
     ja.count_bytes = counters ? 1u : 0u;
-    pa = pbegin();
+    pa = pbegin(K_JOIN);
     if (!tasks.empty()) launch_join(ja, std::min<uint32_t>(nwg, (uint32_t)tasks.size()), 256, s);
     pend(K_JOIN, pa, 0.0, -1);
     if (captured) {

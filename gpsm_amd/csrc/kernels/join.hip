@@ -291,17 +291,51 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) j
 #pragma unroll
               for (int u = 0; u < EXP_UNROLL; ++u) oks[u] &= val[c * WROWS + (is[u] & (WROWS - 1))] != mvs[u];
             }
+            if constexpr (!FINAL) {
+              // join-mode edges, then one append to the next level for the
+              // lane's EXP_UNROLL candidates together: one LDS atomic per
+              // wave and step instead of one per candidate
 #pragma unroll
-            for (int u = 0; u < EXP_UNROLL; ++u) {
-              const uint32_t i = is[u], mv = mvs[u];
-              bool ok = oks[u];
-              if (ok) {
-                if (BYTES) my_bytes += 4;  // the list entry (its hash term is a derived copy of it)
-                for (uint32_t b = bo; b < be && ok; ++b)
-                  ok = has_edge<BYTES>(a.row, a.col, a.hub_of, a.hub_bits, a.W, mv, val[P.backs[b] * WROWS + i],
-                                       my_bytes);
+              for (int u = 0; u < EXP_UNROLL; ++u) {
+                if (oks[u]) {
+                  if (BYTES) my_bytes += 4;  // the list entry
+                  for (uint32_t b = bo; b < be && oks[u]; ++b)
+                    oks[u] = has_edge<BYTES>(a.row, a.col, a.hub_of, a.hub_bits, a.W, mvs[u],
+                                             val[P.backs[b] * WROWS + is[u]], my_bytes);
+                }
               }
-              if constexpr (FINAL) {
+              unsigned long long bal[EXP_UNROLL];
+              uint32_t nbal = 0;
+#pragma unroll
+              for (int u = 0; u < EXP_UNROLL; ++u) {
+                bal[u] = __ballot(oks[u]);
+                nbal += (uint32_t)__popcll(bal[u]);
+              }
+              uint32_t wbase = 0;
+              if (lane == 0 && nbal) wbase = atomicAdd(&S.fill[nk], nbal);
+              wbase = __shfl(wbase, 0, WAVE);
+#pragma unroll
+              for (int u = 0; u < EXP_UNROLL; ++u) {
+                if (oks[u]) {
+                  const uint64_t r = wbase + __popcll(bal[u] & below);
+                  for (uint32_t c = 0; c <= k; ++c) Ln[(uint64_t)c * cap + r] = val[c * WROWS + is[u]];
+                  Ln[(uint64_t)nk * cap + r] = mvs[u];
+                  if (BYTES) my_bytes += 4 * (nk + 1);
+                }
+                wbase += (uint32_t)__popcll(bal[u]);
+              }
+            }
+            if constexpr (FINAL) {
+#pragma unroll
+              for (int u = 0; u < EXP_UNROLL; ++u) {
+                const uint32_t i = is[u], mv = mvs[u];
+                bool ok = oks[u];
+                if (ok) {
+                  if (BYTES) my_bytes += 4;  // the list entry (its hash term is a derived copy of it)
+                  for (uint32_t b = bo; b < be && ok; ++b)
+                    ok = has_edge<BYTES>(a.row, a.col, a.hub_of, a.hub_bits, a.W, mv, val[P.backs[b] * WROWS + i],
+                                         my_bytes);
+                }
                 wave_cnt += __popcll(__ballot(ok));
                 if (ok) {
                   my_hash += fmix64(S.srow[i] + trm[u]);
@@ -313,17 +347,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) j
                       t[wn] = mv;
                     }
                   }
-                }
-              } else {
-                const unsigned long long bal = __ballot(ok);
-                uint32_t wbase = 0;
-                if (lane == 0 && bal) wbase = atomicAdd(&S.fill[nk], (uint32_t)__popcll(bal));
-                wbase = __shfl(wbase, 0, WAVE);
-                if (ok) {
-                  const uint64_t r = wbase + __popcll(bal & below);
-                  for (uint32_t c = 0; c <= k; ++c) Ln[(uint64_t)c * cap + r] = val[c * WROWS + i];
-                  Ln[(uint64_t)nk * cap + r] = mv;
-                  if (BYTES) my_bytes += 4 * (nk + 1);
                 }
               }
             }

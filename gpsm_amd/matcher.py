@@ -105,7 +105,8 @@ class Context:
 
     def profile(self, mode=1):
         """0 off, 1 per-launch timing, 2 timing + the filter's work counters
-        (algorithmic bytes); setting a mode clears the statistics."""
+        (algorithmic bytes), 3 timing of the join launches only; setting a
+        mode clears the statistics."""
         check(_lib.lib().gpsm_profile_enable(self.h, int(mode)), "gpsm_profile_enable")
 
     def kernel_stats(self):

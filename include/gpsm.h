@@ -302,7 +302,8 @@ typedef struct gpsm_kstat {
  * filter and join kernels did (candidates, adjacency entries, key reads,
  * probes, list entries, searches; a few atomics per workgroup), from which
  * their algorithmic bytes are computed (the join runs a variant that counts
- * them).  Setting a mode clears the statistics.  Launches captured in a
+ * them), 3 HIP-event timing of the join launches only (no events between
+ * the other launches).  Setting a mode clears the statistics.  Launches captured in a
  * hipGraph are not timed.  Entries named "wall:<phase>" are host wall-clock
  * times of a batch's phases, synchronisations included. */
 int gpsm_profile_enable(gpsm_ctx* ctx, int mode);

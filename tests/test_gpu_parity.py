@@ -217,13 +217,13 @@ def test_explicit_orders_agree(ctx):
 def test_profile_modes_keep_results_and_count_bytes(ctx):
     """Timing-only and counting profile modes time every launch family and
     leave results unchanged; only the counting mode adds the filter's
-    measured work to its bytes."""
+    measured work to its bytes.  The join-only mode times the join alone."""
     g = G.small_graph(12, n=3000, m=12000, labels=3, kind="cl")
     qs = G.queries(g, 16, 3, 6, _lib.Q_WALK_HALF, 19, 10 ** 7)
     rg = ctx.upload(g)
     base = rg.match_batch(qs)
     stats = {}
-    for mode in (1, 2):
+    for mode in (1, 2, 3):
         ctx.profile(mode)
         res = rg.match_batch(qs)
         stats[mode] = ctx.kernel_stats()
@@ -232,6 +232,8 @@ def test_profile_modes_keep_results_and_count_bytes(ctx):
     for k in ("filter_step", "join", "cet_count", "cet_write"):
         assert stats[1][k]["launches"] == stats[2][k]["launches"] > 0 and stats[1][k]["ms"] > 0
     assert stats[2]["filter_step"]["bytes"] > stats[1]["filter_step"]["bytes"] > 0
+    timed = {k for k, v in stats[3].items() if not k.startswith("wall:") and v["launches"]}
+    assert timed == {"join"} and stats[3]["join"]["launches"] == stats[1]["join"]["launches"]
 
 
 def test_work_budget_is_exact(ctx):
