@@ -48,8 +48,9 @@ def test_compact(ctx, nbits, density):
     assert np.array_equal(ctx.compact(words, nbits), O.compact_bitmap(words, nbits))
 
 
-def test_hand_fixture(ctx):
-    fx, g, q = hand_fixture()
+@pytest.mark.parametrize("name", ["hand_aba.json", "hand_refine.json"])
+def test_hand_fixture(ctx, name):
+    fx, g, q = hand_fixture(name)
     rg = ctx.upload(g)
     bm, cnt = rg.filter(q)
     sets = bits_to_sets(bm, g.n)

@@ -113,8 +113,8 @@ def test_brute_force_tiny(case):
     assert np.array_equal(r2["tuples"], rows)
 
 
-def hand_fixture():
-    with open(os.path.join(HERE, "golden", "hand_aba.json")) as f:
+def hand_fixture(name="hand_aba.json"):
+    with open(os.path.join(HERE, "golden", name)) as f:
         fx = json.load(f)
     g = G.from_edges(fx["data"]["n"], np.array(fx["data"]["src"]), np.array(fx["data"]["dst"]),
                      np.array(fx["data"]["labels"]))
@@ -123,9 +123,12 @@ def hand_fixture():
     return fx, g, q
 
 
-def test_hand_worked_fixture():
-    """Every intermediate of the reference's steps, worked by hand."""
-    fx, g, q = hand_fixture()
+@pytest.mark.parametrize("name", ["hand_aba.json", "hand_refine.json"])
+def test_hand_worked_fixture(name):
+    """Every intermediate of the reference's steps, worked by hand: a case
+    where refine changes nothing, and one where explore and refine remove
+    candidates and the join order skips a larger first table."""
+    fx, g, q = hand_fixture(name)
     st, init, ref, fo = O.o2_filter(g, q)
     assert st == O.OK
     assert fo == fx["filter_order"]
