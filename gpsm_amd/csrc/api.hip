@@ -1189,9 +1189,11 @@ int gpsm_autotune(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_quer
   o.join_cap = 0;
   std::vector<gpsm_result> ref(nq), out(nq);
   auto trial = [&](double& ms, bool check) -> int {
-    // best of two runs; the first also warms the buffers up
+    // best of three runs (the first also warms the buffers up): batch times
+    // vary by a few percent from run to run, as much as neighbouring
+    // settings differ
     double best = 1e300;
-    for (int rep = 0; rep < 2; ++rep) {
+    for (int rep = 0; rep < 3; ++rep) {
       auto t0 = std::chrono::steady_clock::now();
       int rc = run_batch(c, g, nq, qs, &o, check ? out.data() : ref.data(), nullptr);
       if (rc) return rc;
