@@ -36,6 +36,28 @@ def _result(r, q):
                        [int(x) for x in r.level_work[:q]])
 
 
+class _BatchResult:
+    """One query's result of a batch, read from the library's result struct
+    when a field is used: a batch of hundreds of queries then converts only
+    what the caller reads (MatchResult's fields; tuples is always None)."""
+    __slots__ = ("_r", "_q")
+    tuples = None
+
+    def __init__(self, r, q):
+        self._r = r  # a view into the batch's result array, which it keeps alive
+        self._q = q
+
+    count = property(lambda s: int(s._r.count))
+    hash = property(lambda s: int(s._r.hash))
+    order = property(lambda s: list(s._r.order[:s._q]))
+    level_rows = property(lambda s: [int(x) for x in s._r.level_rows[:s._q]])
+    cand = property(lambda s: list(s._r.cand[:s._q]))
+    device_ms = property(lambda s: float(s._r.device_ms))
+    status = property(lambda s: int(s._r.status))
+    join_work = property(lambda s: int(s._r.join_work))
+    level_work = property(lambda s: [int(x) for x in s._r.level_work[:s._q]])
+
+
 def _opts(order_mode=_lib.ORDER_HEURISTIC, explicit_order=None, want_tuples=False, tuple_cap=0, use_graph=False,
           block=0, join_cap=0, work_budget=0):
     o = _lib.Opts()
@@ -205,7 +227,7 @@ class ResidentGraph:
         res = (_lib.Result * len(queries))()
         check(_lib.lib().gpsm_match_batch(self.ctx.h, self.h, len(queries), arr, C.byref(o), res),
               "gpsm_match_batch")
-        return [_result(res[i], q.n) for i, q in enumerate(queries)]
+        return [_BatchResult(res[i], q.n) for i, q in enumerate(queries)]
 
     def plan_features(self, q, cand, cet_size, order, level_rows):
         """Planner training rows of one measured run (gpsm_plan_features)."""
