@@ -1187,17 +1187,32 @@ int gpsm_autotune(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_quer
   o.want_tuples = 0;
   o.block = 0;
   o.join_cap = 0;
+  o.use_graph = 0;  // the trials time launches with HIP events, which captured graphs do not record
   std::vector<gpsm_result> ref(nq), out(nq);
+  // The join's settings are judged by the join kernel's own time (HIP
+  // events, profile mode 3) and the filter's by the batch's wall time: the
+  // other phases' run-to-run variation is as large as the difference
+  // between neighbouring join settings.  The caller's profiling state is
+  // put back afterwards.
+  const int saved_mode = c->prof.mode;
+  Prof::Acc saved_acc[sizeof(c->prof.acc) / sizeof(c->prof.acc[0])];
+  std::copy(std::begin(c->prof.acc), std::end(c->prof.acc), saved_acc);
+  auto restore = [&]() {
+    c->prof.mode = saved_mode;
+    std::copy(std::begin(saved_acc), std::end(saved_acc), std::begin(c->prof.acc));
+  };
+  bool join_param = false;
   auto trial = [&](double& ms, bool check) -> int {
-    // best of three runs (the first also warms the buffers up): batch times
-    // vary by a few percent from run to run, as much as neighbouring
-    // settings differ
+    // best of three runs (the first also warms the buffers up)
     double best = 1e300;
     for (int rep = 0; rep < 3; ++rep) {
+      c->prof.mode = 3;
+      for (auto& a : c->prof.acc) a = Prof::Acc();
       auto t0 = std::chrono::steady_clock::now();
       int rc = run_batch(c, g, nq, qs, &o, check ? out.data() : ref.data(), nullptr);
       if (rc) return rc;
-      best = std::min(best, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+      const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      best = std::min(best, join_param ? c->prof.acc[K_JOIN].ms : wall);
     }
     ms = best;
     if (check)
@@ -1207,9 +1222,20 @@ int gpsm_autotune(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_quer
   };
   double cur;
   int rc = trial(cur, false);
-  if (rc) return rc;
+  if (rc) {
+    restore();
+    return rc;
+  }
   const uint32_t values[4][4] = {{64, 128, 256, 0}, {2048, 4096, 8192, 0}, {2, 4, 8, 0}, {4, 8, 16, 32}};
   for (int param = 0; param < 4; ++param) {
+    if (param == 1) {
+      // from here on the join's time decides; the current setting first
+      join_param = true;
+      if ((rc = trial(cur, true))) {
+        restore();
+        return rc;
+      }
+    }
     Tuning best = c->tune;
     for (int i = 0; i < 4 && values[param][i]; ++i) {
       Tuning t = best;
@@ -1223,6 +1249,7 @@ int gpsm_autotune(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_quer
       rc = trial(ms, true);
       if (rc) {
         c->tune = keep;
+        restore();
         return rc;
       }
       if (ms < cur) {
@@ -1233,6 +1260,7 @@ int gpsm_autotune(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_quer
     }
     c->tune = best;
   }
+  restore();
   if (best_ms) *best_ms = cur;
   return GPSM_OK;
 }

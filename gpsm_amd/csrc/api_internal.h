@@ -119,7 +119,7 @@ namespace gpsm {
 
 // Per-launch timing on the context stream while profiling is on.
 struct Prof {
-  int mode = 0;  // 0 off, 1 HIP-event timing, 2 timing + the filter's byte counters
+  int mode = 0;  // 0 off, 1 HIP-event timing, 2 timing + the filter's byte counters, 3 the join only
   struct Pending {
     int kind;
     hipEvent_t a, b;

@@ -305,7 +305,8 @@ class ResidentGraph:
         return self.cet_edges(q)[2]
 
     def autotune(self, queries, **kw):
-        """Runs gpsm_autotune on a sample batch; returns (best ms, tuning)."""
+        """Runs gpsm_autotune on a sample batch; returns (the join kernel's ms with
+        the chosen setting, tuning)."""
         o, keep = _opts(**kw)
         arr = (_lib.QueryS * len(queries))()
         for i, q in enumerate(queries):

@@ -195,10 +195,13 @@ void gpsm_result_free(gpsm_result* r);
  * 1024/256/32, match/Match.cu:455-1618): times gpsm_match_batch on the given
  * sample for every candidate filter workgroup size (64/128/256), join level
  * buffer (2048/4096/8192 rows), persistent join workgroups per CU (2/4/8)
- * and join tasks per workgroup (4/8/16/32), one parameter at a time, and
- * keeps the fastest in the context.
- * Results of every trial are checked to agree.  *best_ms receives the time
- * of the chosen setting.  Later calls with opts->block / join_cap == 0 use it. */
+ * and join tasks per workgroup (4/8/16/32), one parameter at a time (best
+ * of three runs each; the filter workgroup size judged by the batch's wall
+ * time, the join settings by the join kernel's HIP-event time), and keeps
+ * the fastest in the context.  The caller's profiling statistics are left
+ * as they were.  Results of every trial are checked to agree.  *best_ms
+ * receives the join kernel's time (ms) with the chosen setting.  Later
+ * calls with opts->block / join_cap == 0 use it. */
 int gpsm_autotune(gpsm_ctx* ctx, const gpsm_graph* g, uint32_t nq, const gpsm_query* qs,
                   const gpsm_opts* opts, double* best_ms);
 /* The tuned values: filter workgroup size, join buffer rows, join workgroups
