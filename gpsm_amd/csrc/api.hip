@@ -1226,7 +1226,7 @@ int gpsm_autotune(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_quer
     restore();
     return rc;
   }
-  const uint32_t values[4][4] = {{64, 128, 256, 0}, {2048, 4096, 8192, 0}, {2, 4, 8, 0}, {8, 16, 32, 64}};
+  const uint32_t values[4][4] = {{64, 128, 256, 0}, {2048, 4096, 8192, 0}, {4, 8, 12, 16}, {8, 16, 32, 64}};
   for (int param = 0; param < 4; ++param) {
     if (param == 1) {
       // from here on the join's time decides; the current setting first

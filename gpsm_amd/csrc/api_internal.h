@@ -86,9 +86,9 @@ enum BufId {
 
 // Launch parameters chosen by the autotuner (defaults until it runs).
 // The defaults are the values the autotuner settles on for the 100 walk
-// queries on the 1M-vertex Chung-Lu graph on an MI355X (it picks 8 join
-// workgroups per CU and 16-32 tasks per workgroup in every run; with 4 and 8
-// the join of that batch takes 15.6 ms instead of 10).
+// queries on the 1M-vertex Chung-Lu graph on an MI355X: 8 join workgroups
+// per CU (out of 4/8/12/16) and 32 tasks per workgroup (out of 8-64) in
+// every run; with 4 and 8 the join of that batch took 15.6 ms instead of 10.
 struct Tuning {
   uint32_t filter_block = 128;   // workgroup size of the filter supersteps
   uint32_t join_cap = 4096;      // rows per join level buffer
