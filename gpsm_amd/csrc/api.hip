@@ -226,6 +226,13 @@ int gpsm_profile_read(gpsm_ctx* c, gpsm_kstat* out, uint32_t max, uint32_t* n) {
   return GPSM_OK;
 }
 
+int gpsm_graph_cache_stats(const gpsm_ctx* c, uint64_t* instantiations, uint64_t* updates) {
+  if (!c) return GPSM_EINVAL;
+  if (instantiations) *instantiations = c->graph_instantiations;
+  if (updates) *updates = c->graph_updates;
+  return GPSM_OK;
+}
+
 const char* gpsm_strerror(int code) {
   switch (code) {
     case GPSM_OK: return "ok";
@@ -282,6 +289,8 @@ void gpsm_shutdown(gpsm_ctx* c) {
   if (!c) return;
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
+  for (auto& e : c->gexec)
+    if (e) hipGraphExecDestroy(e);
   for (auto& b : c->bufs) b.release();
   for (auto& b : c->h_stage) b.release();
   c->arena.release();
@@ -517,6 +526,11 @@ namespace gpsm {
 
 namespace {
 
+// Device memory the join's level buffers may take (all workgroups together).
+// A batch of 64-vertex queries at 4096 rows per level needs 8 MB per
+// workgroup, so the bound matters only for very wide queries.
+constexpr double kLevelBytes = 16.0 * (1ull << 30);
+
 template <typename T>
 T* dptr(gpsm_ctx* c, int b) {
   return reinterpret_cast<T*>(c->bufs[b].ptr);
@@ -546,6 +560,48 @@ struct QState {
   std::vector<uint64_t> cet_size;  // forward |CET| per query edge
   uint32_t first_desc = 0;
 };
+
+// Ends a stream capture begun by the caller and launches it through the
+// context's cached executable graph of that phase: updated in place when the
+// captured topology matches, else (re)instantiated.  The captured graph is
+// destroyed on every path.
+int launch_captured(gpsm_ctx* c, int phase, hipStream_t s) {
+  hipGraph_t graph = nullptr;
+  if (hipStreamEndCapture(s, &graph) != hipSuccess || !graph) {
+    (void)hipGetLastError();
+    if (graph) hipGraphDestroy(graph);
+    return GPSM_EDEVICE;
+  }
+  hipGraphExec_t& exec = c->gexec[phase];
+  bool updated = false;
+  if (exec) {
+    hipGraphNode_t err_node = nullptr;
+    hipGraphExecUpdateResult why;
+    updated = hipGraphExecUpdate(exec, graph, &err_node, &why) == hipSuccess && why == hipGraphExecUpdateSuccess;
+    if (!updated) {
+      (void)hipGetLastError();
+      hipGraphExecDestroy(exec);
+      exec = nullptr;
+    }
+  }
+  if (!exec) {
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGetLastError();
+      exec = nullptr;
+      hipGraphDestroy(graph);
+      return GPSM_EDEVICE;
+    }
+    ++c->graph_instantiations;
+  } else {
+    ++c->graph_updates;
+  }
+  hipGraphDestroy(graph);
+  if (hipGraphLaunch(exec, s) != hipSuccess) {
+    (void)hipGetLastError();
+    return GPSM_EDEVICE;
+  }
+  return GPSM_OK;
+}
 
 int predict_with_ctx(void* user, const float* x, uint32_t B, float* y) {
   return gpsm_mlp_forward(reinterpret_cast<gpsm_mlp*>(user), x, B, y);
@@ -673,8 +729,6 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   unsigned long long* fstats = dptr<unsigned long long>(c, B_FSTATS);
   {
     bool captured = false;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
     if (o.use_graph && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) captured = true;
     launch_fill_u32(bitmaps, 0u, (uint64_t)nslots * W, s);
     if (maxsteps) launch_fill_u32(dptr<uint32_t>(c, B_FNITEMS), 0u, maxsteps, s);
@@ -697,12 +751,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       pend(K_FILTER, a, 4.0 * W * nops_s, counters ? (int)sidx : -1);
     }
     launch_row_popcount(dg, bitmaps, nslots, dptr<uint32_t>(c, B_SEGC), dptr<uint32_t>(c, B_COUNTS), s);
-    if (captured) {
-      if (hipStreamEndCapture(s, &graph) != hipSuccess || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) !=
-                                                              hipSuccess)
-        return GPSM_EDEVICE;
-      HIPCHK(hipGraphLaunch(exec, s));
-    }
+    if (captured && (rc = launch_captured(c, 0, s))) return rc;
     HIPCHK(hipGetLastError());
     std::vector<uint32_t>& counts = c->h_counts;
     counts.assign(std::max<uint32_t>(nslots, 1), 0);
@@ -710,8 +759,6 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
                           hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     std::memcpy(counts.data(), c->h_stage[0].ptr, sizeof(uint32_t) * nslots);
-    if (exec) hipGraphExecDestroy(exec);
-    if (graph) hipGraphDestroy(graph);
   }
   wall(K_WALL_FILTER);
   const std::vector<uint32_t>& counts = c->h_counts;
@@ -933,11 +980,15 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     plans.push_back(jp);
   }
   // tasks: interleaved row sets of the first tables.  A query gets tasks in
-  // proportion to its predicted join work (about 8 tasks per workgroup over
-  // the batch), and at least one per tsize first-table rows, so a query
-  // whose few first rows fan out to most of the batch's work does not leave
-  // a handful of long tasks running after the rest of the grid is idle.
-  const uint32_t nwg = std::max<uint32_t>(1, c->ncu * wg_per_cu);
+  // proportion to its predicted join work (tasks_per_wg tasks per workgroup
+  // over the batch, 32 by default), and at least one per tsize first-table
+  // rows, so a query whose few first rows fan out to most of the batch's
+  // work does not leave a handful of long tasks running after the rest of
+  // the grid is idle.  The persistent grid holds at most as many workgroups
+  // as are resident at once: a workgroup beyond that would start only once
+  // the task counter has run out.
+  const uint32_t resident = join_resident_per_cu(qmax);
+  uint32_t nwg = std::max<uint32_t>(1, c->ncu * std::min(wg_per_cu, resident));
   const uint64_t tpw = std::max<uint32_t>(1, c->tune.tasks_per_wg);
   uint64_t rows_total = 0;
   double est_total = 0;
@@ -977,8 +1028,14 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       (rc = c->h_stage[2].ensure(sizeof(JoinOut) * std::max<uint32_t>(nplans, 1))))
     return rc;
   if ((rc = c->bufs[B_CTR].ensure(256))) return rc;
-  const uint64_t lw = join_level_words(qmax, cap);
-  if ((rc = c->bufs[B_LEVELS].ensure(sizeof(uint32_t) * std::max<uint64_t>(lw * nwg, 1)))) return rc;
+  // Level buffers of the workgroups launched, within kLevelBytes: rows per
+  // level halve (to 2048) and then workgroups are dropped until they fit.
+  const uint32_t nlaunch = std::max<uint32_t>(1, std::min<uint32_t>(nwg, (uint32_t)tasks.size()));
+  uint32_t lcap = cap, lwg = nlaunch;
+  while (lcap > 2048 && (double)join_level_words(qmax, lcap) * 4.0 * lwg > kLevelBytes) lcap /= 2;
+  const uint64_t lw = join_level_words(qmax, lcap);
+  if (lw && (double)lw * 4.0 * lwg > kLevelBytes) lwg = std::max<uint32_t>(1, (uint32_t)(kLevelBytes / (4.0 * lw)));
+  if ((rc = c->bufs[B_LEVELS].ensure(sizeof(uint32_t) * std::max<uint64_t>(lw * lwg, 1)))) return rc;
   uint64_t emit_cap = 0;
   if (o.want_tuples && nq == 1) {
     emit_cap = o.tuple_cap;
@@ -986,8 +1043,6 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   }
   {
     bool captured = false;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
     if (o.use_graph && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) captured = true;
     hipEvent_t pa = pbegin(K_CET_COUNT);
     if (!cdesc.empty())
@@ -1022,7 +1077,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     ja.ntasks = (uint32_t)tasks.size();
     ja.W = W;
     ja.NB = NB;
-    ja.cap = cap;
+    ja.cap = lcap;
     ja.qmax = qmax;
     ja.levels = dptr<uint32_t>(c, B_LEVELS);
     ja.level_stride = lw;
@@ -1036,14 +1091,9 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     ja.budget = o.work_budget;
     ja.count_bytes = counters ? 1u : 0u;
     pa = pbegin(K_JOIN);
-    if (!tasks.empty()) launch_join(ja, std::min<uint32_t>(nwg, (uint32_t)tasks.size()), 256, s);
+    if (!tasks.empty()) launch_join(ja, lwg, 256, s);
     pend(K_JOIN, pa, 0.0, -1);
-    if (captured) {
-      if (hipStreamEndCapture(s, &graph) != hipSuccess || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) !=
-                                                              hipSuccess)
-        return GPSM_EDEVICE;
-      HIPCHK(hipGraphLaunch(exec, s));
-    }
+    if (captured && (rc = launch_captured(c, 1, s))) return rc;
     HIPCHK(hipGetLastError());
     std::vector<JoinOut> jo(std::max<uint32_t>(nplans, 1));
     HIPCHK(hipMemcpyAsync(c->h_stage[2].ptr, dptr<JoinOut>(c, B_JOUT), sizeof(JoinOut) * nplans,
@@ -1055,8 +1105,6 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     HIPCHK(hipStreamSynchronize(s));
     if (nplans) std::memcpy(jo.data(), c->h_stage[2].ptr, sizeof(JoinOut) * nplans);
     wall(K_WALL_JOIN);
-    if (exec) hipGraphExecDestroy(exec);
-    if (graph) hipGraphDestroy(graph);
     float ms = 0;
     hipEventElapsedTime(&ms, c->ev0, c->ev1);
     if (prof) {
@@ -1226,7 +1274,9 @@ int gpsm_autotune(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_quer
     restore();
     return rc;
   }
-  const uint32_t values[4][4] = {{64, 128, 256, 0}, {2048, 4096, 8192, 0}, {4, 8, 12, 16}, {8, 16, 32, 64}};
+  // join workgroups per CU above the number resident at once change nothing
+  // (run_batch caps them), so the search stops at 8
+  const uint32_t values[4][4] = {{64, 128, 256, 0}, {2048, 4096, 8192, 0}, {2, 4, 6, 8}, {8, 16, 32, 64}};
   for (int param = 0; param < 4; ++param) {
     if (param == 1) {
       // from here on the join's time decides; the current setting first

@@ -170,6 +170,12 @@ struct gpsm_ctx {
   std::vector<uint32_t> h_counts;
   gpsm::HostBuf h_stage[3];  // pinned readbacks of a batch: counts, |CET| descriptors, join results
   gpsm::PinnedArena arena;   // pinned sources of a batch's uploads
+  // Instantiated hipGraphs of a batch's two captured phases (filter; tables +
+  // join), kept across batches: a new capture updates them in place when its
+  // topology is the same (hipGraphExecUpdate) and is instantiated afresh
+  // only when it is not.  graph_instantiations counts the latter.
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  uint64_t graph_instantiations = 0, graph_updates = 0;
   // hub adjacency bitmaps of later uploads: vertices of degree > the hub
   // degree, starting at hub_min_degree and doubled until they fit hub_max_bytes
   uint32_t hub_min_degree = 512;

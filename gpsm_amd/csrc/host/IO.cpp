@@ -108,6 +108,19 @@ bool IO::output(const unsigned* final_result, unsigned rows, unsigned cols, cons
   return true;
 }
 
+bool IO::output() {
+  if (!ofp) return false;
+  std::fprintf(ofp, "\n\n\n");
+  return true;
+}
+
+bool IO::output(int* m, int size) {
+  if (!ofp || (size > 0 && !m)) return false;
+  for (int i = 0; i < size; ++i) std::fprintf(ofp, "(%d, %d) ", i, m[i]);
+  std::fprintf(ofp, "\n");
+  return true;
+}
+
 void IO::flush() {
   if (ofp) std::fflush(ofp);
 }

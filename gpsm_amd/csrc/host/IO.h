@@ -29,6 +29,11 @@ class IO {
   // a data graph read by other means (a binary CSR cache) counts as the next
   void next_data() { ++data_id; }
   bool output(int qid);
+  // separator after a data graph's results: three newlines (io/IO.cpp:138-142)
+  bool output();
+  // one embedding given as m[i] = image of query vertex i: "(i, m[i]) " pairs
+  // and a newline (io/IO.cpp:223-232)
+  bool output(int* m, int size);
   // one line per embedding: "(query id, data id)" pairs in query-id order
   bool output(const unsigned* final_result, unsigned rows, unsigned cols, const int* id_map, Graph* query,
               Graph* data);

@@ -35,17 +35,18 @@ struct QueryArrays {
 
 }  // namespace
 
-bool Match::initGPU(int dev) {
-  if (g_ctx) return true;
+void Match::initGPU(int dev) {
+  if (g_ctx) return;
   int rc = gpsm_init(dev, &g_ctx);
   if (rc != GPSM_OK) {
     std::fprintf(stderr, "initGPU: %s\n", gpsm_strerror(rc));
     g_ctx = nullptr;
-    return false;
+    return;
   }
   std::printf("Using device %d: %s\n", dev, gpsm_device_name(g_ctx));
-  return true;
 }
+
+bool Match::gpuReady() { return g_ctx != nullptr; }
 
 void Match::releaseGPU() {
   for (auto& kv : g_resident) gpsm_graph_free(kv.second);
@@ -71,8 +72,8 @@ gpsm_graph* Match::resident() {
   auto it = g_resident.find(data);
   if (it != g_resident.end()) return it->second;
   gpsm_graph* g = nullptr;
-  status = gpsm_graph_upload(g_ctx, data->vertex_num, data->undirected_row_offset.data(),
-                             data->undirected_column_index.data(), data->vertex_value.data(), &g);
+  status = gpsm_graph_upload(g_ctx, data->vertex_num, data->undirected_row_offset,
+                             data->undirected_column_index, data->vertex_value, &g);
   if (status != GPSM_OK) return nullptr;
   g_resident[data] = g;
   return g;

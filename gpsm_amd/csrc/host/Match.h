@@ -30,8 +30,10 @@ class Match {
   // IO::output's embedding lines formatted on the device into fd
   // (gpsm_write_tuples); an IO::RowWriter.
   static int write_rows(int fd, const unsigned* rows, unsigned n, unsigned cols);
-  // Selects the device; returns false (never exits) when none is usable.
-  static bool initGPU(int dev);
+  // Selects the device (match/Match.h:87).  The reference exit()s when none is
+  // usable; here the failure is printed and gpuReady() returns false.
+  static void initGPU(int dev);
+  static bool gpuReady();
   static void releaseGPU();
   // Frees the device copy of a data graph (call before deleting the Graph:
   // resident copies are keyed by its address).

@@ -48,10 +48,10 @@ static int write_cache(const std::string& data, const std::string& out) {
     std::fprintf(stderr, "data file: malformed or missing terminator\n");
     return 1;
   }
-  const int rc = gpsm_io_write_csr(out.c_str(), g->vertex_num, g->undirected_row_offset.data(),
-                                   g->undirected_column_index.data(), g->vertex_value.data());
-  std::fprintf(stderr, "%s: %u vertices, %zu adjacency entries -> %s (%s)\n", data.c_str(), g->vertex_num,
-               g->undirected_column_index.size(), out.c_str(), gpsm_strerror(rc));
+  const int rc = gpsm_io_write_csr(out.c_str(), g->vertex_num, g->undirected_row_offset,
+                                   g->undirected_column_index, g->vertex_value);
+  std::fprintf(stderr, "%s: %u vertices, %u adjacency entries -> %s (%s)\n", data.c_str(), g->vertex_num,
+               g->eSize(), out.c_str(), gpsm_strerror(rc));
   delete g;
   return rc == GPSM_OK ? 0 : 1;
 }
@@ -79,7 +79,8 @@ int main(int argc, const char* argv[]) {
     std::fprintf(stderr, "--directed needs the text data format and tuples (not --count)\n");
     return 2;
   }
-  if (!Match::initGPU(dev)) return 1;
+  Match::initGPU(dev);
+  if (!Match::gpuReady()) return 1;
   // a cache is read below; the text reader then only serves the queries
   IO io(args[1], cache ? args[1] : args[0], output);
   if (!io.ok()) return 1;

@@ -18,6 +18,7 @@
 //    shorter adjacency) and injectivity is checked in the same pass;
 //  * the last level is never written: each embedding adds its tuple hash
 //    (additive over vertices, one fmix64 per emit) and 1 to the counters.
+#include <algorithm>
 #include <type_traits>
 
 #include "launch.h"
@@ -432,9 +433,29 @@ uint64_t join_level_words(uint32_t qmax, uint32_t cap) {
   return (uint64_t)cap * (k * (k + 1) / 2 - 3);
 }
 
+namespace {
+size_t join_lds_bytes(uint32_t qmax) {
+  return ((sizeof(JoinShared) + 15) & ~(size_t)15) + sizeof(uint32_t) * WROWS * qmax;
+}
+}  // namespace
+
+uint32_t join_resident_per_cu(uint32_t qmax) {
+  const size_t lds = join_lds_bytes(std::max<uint32_t>(qmax, 2));
+  if (lds > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(join_kernel<false>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, join_kernel<false>, (int)WROWS, lds) != hipSuccess ||
+      per_cu < 1) {
+    (void)hipGetLastError();
+    per_cu = 1;
+  }
+  return (uint32_t)per_cu;
+}
+
 void launch_join(const JoinArgs& a, uint32_t nwg, uint32_t block, hipStream_t s) {
   (void)block;
-  const size_t lds = ((sizeof(JoinShared) + 15) & ~(size_t)15) + sizeof(uint32_t) * WROWS * a.qmax;
+  const size_t lds = join_lds_bytes(a.qmax);
   if (lds > 65536) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(join_kernel<true>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -164,5 +164,9 @@ struct JoinArgs {
 };
 void launch_join(const JoinArgs& a, uint32_t nwg, uint32_t block, hipStream_t s);
 uint64_t join_level_words(uint32_t qmax, uint32_t cap);
+// Join workgroups (of the timed variant) that are resident on one CU at once
+// for a batch whose widest query has qmax vertices (LDS and registers); a
+// persistent grid gains nothing from more.
+uint32_t join_resident_per_cu(uint32_t qmax);
 
 }  // namespace gpsm

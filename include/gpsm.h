@@ -210,8 +210,10 @@ int gpsm_tuning(const gpsm_ctx* ctx, uint32_t* filter_block, uint32_t* join_cap,
                 uint32_t* tasks_per_wg);
 /* Sets the same values, e.g. a tuning saved from an earlier gpsm_autotune
  * (filter_block 64/128/256, join_cap 2048..2^20, join_wg_per_cu 1..16,
- * tasks_per_wg 1..256; GPSM_EINVAL otherwise).  No counterpart in the
- * reference, whose launch shapes are constants. */
+ * tasks_per_wg 1..256; GPSM_EINVAL otherwise).  A batch launches at most as
+ * many join workgroups per CU as are resident at once (about 7 for queries
+ * of up to 16 vertices), whatever join_wg_per_cu asks.  No counterpart in
+ * the reference, whose launch shapes are constants. */
 int gpsm_set_tuning(gpsm_ctx* ctx, uint32_t filter_block, uint32_t join_cap, uint32_t join_wg_per_cu,
                     uint32_t tasks_per_wg);
 
@@ -312,6 +314,12 @@ typedef struct gpsm_kstat {
 int gpsm_profile_enable(gpsm_ctx* ctx, int mode);
 /* Copies up to max kernel-family statistics; *n receives the number. */
 int gpsm_profile_read(gpsm_ctx* ctx, gpsm_kstat* out, uint32_t max, uint32_t* n);
+/* With opts.use_graph, a batch's two captured phases run through executable
+ * hipGraphs that the context keeps: a later batch's capture updates them in
+ * place when its topology is the same and instantiates them again when it is
+ * not.  These counters (since gpsm_init) say which happened.  No counterpart
+ * in the reference (no graphs). */
+int gpsm_graph_cache_stats(const gpsm_ctx* ctx, uint64_t* instantiations, uint64_t* updates);
 
 /* --- set hash ------------------------------------------------------------- */
 

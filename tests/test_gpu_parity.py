@@ -176,6 +176,30 @@ def test_batch_equals_single_and_oracle(ctx):
         assert b.level_work == s.level_work == gr.level_work
 
 
+def test_cached_graphs_replay_and_update(ctx):
+    """The captured phases run through the context's cached executable
+    graphs: the same batch again updates them in place (no new
+    instantiation), eager batches and synchronisations in between change
+    nothing, and a batch of another shape re-instantiates them; every replay
+    equals the eager result."""
+    g = G.small_graph(21, n=4000, m=20000, labels=3, kind="cl")
+    rg = ctx.upload(g)
+    qa = G.queries(g, 16, 3, 7, _lib.Q_WALK_HALF, 31, 10 ** 7)
+    qb = G.queries(g, 9, 4, 9, _lib.Q_WALK_INDUCED, 32, 10 ** 7)
+    want_a = [(r.count, r.hash, r.join_work) for r in rg.match_batch(qa)]
+    want_b = [(r.count, r.hash, r.join_work) for r in rg.match_batch(qb)]
+    i0, u0 = ctx.graph_cache_stats()
+    for rep in range(4):
+        got = [(r.count, r.hash, r.join_work) for r in rg.match_batch(qa, use_graph=True)]
+        assert got == want_a, rep
+        assert [(r.count, r.hash, r.join_work) for r in rg.match_batch(qa)] == want_a  # eager between replays
+    i1, u1 = ctx.graph_cache_stats()
+    assert i1 - i0 <= 2 and u1 - u0 >= 6  # two phases: instantiated once, then updated
+    for rep in range(2):
+        assert [(r.count, r.hash, r.join_work) for r in rg.match_batch(qb, use_graph=True)] == want_b
+    assert [(r.count, r.hash, r.join_work) for r in rg.match_batch(qa, use_graph=True)] == want_a
+
+
 def test_small_join_buffers_force_depth_first_spills(ctx):
     """A join buffer of 512 rows forces many partial windows and re-entries."""
     g = G.small_graph(10, n=2000, m=16000, labels=1, kind="er")

@@ -46,6 +46,7 @@ def test_autotuner_keeps_results(ctx, mix):
     before = rg.match_batch(qs)
     ms, tune = rg.autotune(qs[:32])
     assert ms > 0 and tune["filter_block"] in (64, 128, 256) and tune["tasks_per_wg"] in (8, 16, 32, 64)
+    assert tune["join_wg_per_cu"] in (2, 4, 6, 8) and tune["join_cap"] in (2048, 4096, 8192)
     after = rg.match_batch(qs)
     assert [(r.count, r.hash) for r in before] == [(r.count, r.hash) for r in after]
     # a saved tuning is set back exactly; out-of-range values are refused
