@@ -722,8 +722,12 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       (rc = c->bufs[B_FNITEMS].ensure(sizeof(uint32_t) * std::max<uint32_t>(maxsteps, 1))) ||
       (rc = c->bufs[B_RFLAGS].ensure(sizeof(uint32_t) * 4 * (uint64_t)std::max<uint32_t>(nq, 1))))
     return rc;
+  // work counters (profile mode 2): one slot per filter superstep, then the
+  // candidate-edge count of phase B, the count of phase C and the write
   const size_t fstat_step = (size_t)STAT_COPIES * STAT_STRIDE;
-  if ((rc = c->bufs[B_FSTATS].ensure(sizeof(unsigned long long) * fstat_step * (maxsteps + 1)))) return rc;
+  const uint32_t nstat_slots = maxsteps + 4;
+  const int slot_cet_b = (int)maxsteps + 1, slot_cet_c = (int)maxsteps + 2, slot_cet_w = (int)maxsteps + 3;
+  if ((rc = c->bufs[B_FSTATS].ensure(sizeof(unsigned long long) * fstat_step * nstat_slots))) return rc;
   if ((rc = c->h_stage[0].ensure(sizeof(uint32_t) * std::max<uint32_t>(nslots, 1)))) return rc;
   uint32_t* bitmaps = dptr<uint32_t>(c, B_BITMAPS);
   unsigned long long* fstats = dptr<unsigned long long>(c, B_FSTATS);
@@ -733,7 +737,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     launch_fill_u32(bitmaps, 0u, (uint64_t)nslots * W, s);
     if (maxsteps) launch_fill_u32(dptr<uint32_t>(c, B_FNITEMS), 0u, maxsteps, s);
     launch_fill_u32(dptr<uint32_t>(c, B_RFLAGS), 0u, 4ull * std::max<uint32_t>(nq, 1), s);
-    if (counters) launch_fill_u32(reinterpret_cast<uint32_t*>(fstats), 0u, 2ull * fstat_step * (maxsteps + 1), s);
+    if (counters) launch_fill_u32(reinterpret_cast<uint32_t*>(fstats), 0u, 2ull * fstat_step * nstat_slots, s);
     {
       hipEvent_t a = pbegin(K_INIT);
       launch_init(dg, dptr<InitOp>(c, B_INIT), (uint32_t)init.size(), bitmaps, s);
@@ -821,8 +825,9 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   {
     hipEvent_t a = pbegin(K_CET_COUNT);
     launch_cet_count(dg, bitmaps, dptr<uint32_t>(c, B_CLIST), dptr<CetDesc>(c, B_FDESC), (uint32_t)fdesc.size(),
-                     dptr<uint32_t>(c, B_OFFSETS), 1, dptr<uint32_t>(c, B_CHUNKS), cgrid, s);
-    pend(K_CET_COUNT, a, 0.0, -1);
+                     dptr<uint32_t>(c, B_OFFSETS), 1, dptr<uint32_t>(c, B_CHUNKS), cgrid, s,
+                     counters ? fstats + fstat_step * slot_cet_b : nullptr);
+    pend(K_CET_COUNT, a, 0.0, counters ? slot_cet_b : -1);
   }
   HIPCHK(hipGetLastError());
   if ((rc = c->h_stage[1].ensure(sizeof(CetDesc) * std::max<size_t>(fdesc.size(), 1)))) return rc;
@@ -1047,16 +1052,18 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     hipEvent_t pa = pbegin(K_CET_COUNT);
     if (!cdesc.empty())
       launch_cet_count(dg, bitmaps, dptr<uint32_t>(c, B_CLIST), dptr<CetDesc>(c, B_CDESC), (uint32_t)cdesc.size(),
-                       dptr<uint32_t>(c, B_OFFSETS), 1, dptr<uint32_t>(c, B_CHUNKS), cgrid, s);
-    pend(K_CET_COUNT, pa, 0.0, -1);
+                       dptr<uint32_t>(c, B_OFFSETS), 1, dptr<uint32_t>(c, B_CHUNKS), cgrid, s,
+                       counters ? fstats + fstat_step * slot_cet_c : nullptr);
+    pend(K_CET_COUNT, pa, 0.0, counters ? slot_cet_c : -1);
     launch_cet_offsets(dptr<CetDesc>(c, B_JDESC), (uint32_t)jdesc.size(), dptr<uint32_t>(c, B_OFFSETS), s);
     pa = pbegin(K_CET_WRITE);
     launch_cet_write(dg, bitmaps, dptr<uint32_t>(c, B_CLIST), dptr<CetDesc>(c, B_JDESC), (uint32_t)jdesc.size(),
-                     dptr<uint32_t>(c, B_OFFSETS), dptr<uint32_t>(c, B_LISTS), dptr<uint32_t>(c, B_CHUNKS), cgrid, s);
+                     dptr<uint32_t>(c, B_OFFSETS), dptr<uint32_t>(c, B_LISTS), dptr<uint32_t>(c, B_CHUNKS), cgrid, s,
+                     counters ? fstats + fstat_step * slot_cet_w : nullptr);
     launch_cet_terms(dptr<uint32_t>(c, B_LISTS), dptr<CetDesc>(c, B_JDESC), dptr<uint32_t>(c, B_TERM_DESC),
                      dptr<uint64_t>(c, B_TERM_START), (uint32_t)term_desc.size(), term_words,
                      dptr<unsigned long long>(c, B_TERMS), s);
-    pend(K_CET_WRITE, pa, 0.0, -1);
+    pend(K_CET_WRITE, pa, 0.0, counters ? slot_cet_w : -1);
     // ---------------- phase D: join ----------------
     launch_fill_u32(dptr<uint32_t>(c, B_JOUT), 0u, sizeof(JoinOut) / 4 * (uint64_t)std::max<uint32_t>(nplans, 1), s);
     launch_fill_u32(dptr<uint32_t>(c, B_CTR), 0u, 64, s);
@@ -1108,7 +1115,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     float ms = 0;
     hipEventElapsedTime(&ms, c->ev0, c->ev1);
     if (prof) {
-      std::vector<unsigned long long> fs(fstat_step * (maxsteps + 1), 0);
+      std::vector<unsigned long long> fs(fstat_step * nstat_slots, 0);
       if (counters)
         HIPCHK(hipMemcpy(fs.data(), fstats, sizeof(unsigned long long) * fs.size(), hipMemcpyDeviceToHost));
       double join_bytes = 0;
@@ -1123,10 +1130,19 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
           unsigned long long f[5] = {0, 0, 0, 0, 0};
           for (uint32_t cp = 0; cp < STAT_COPIES; ++cp)
             for (int i = 0; i < 5; ++i) f[i] += fs[fstat_step * pnd.stats_slot + cp * STAT_STRIDE + i];
-          // candidates: two row offsets; entries: column index or neighbour
-          // key; neighbour keys read beside the column index (explore's
-          // second pass); bitmap probes and bit writes count one bit each
-          bytes += 8.0 * f[0] + 4.0 * f[1] + 4.0 * f[2] + (f[3] + f[4]) / 8.0;
+          if (pnd.kind == K_FILTER) {
+            // candidates: two row offsets; entries: column index or neighbour
+            // key; neighbour keys read beside the column index (explore's
+            // second pass); bitmap probes and bit writes count one bit each
+            bytes += 8.0 * f[0] + 4.0 * f[1] + 4.0 * f[2] + (f[3] + f[4]) / 8.0;
+          } else {
+            // candidate-edge count / write: per candidate its id and two row
+            // offsets, plus its count written (count) or its list offset read
+            // (write); per adjacency entry the column index and the
+            // neighbour's key; one bit per probe of C(w); 4 per list entry
+            // written
+            bytes += 16.0 * f[0] + 8.0 * f[1] + f[2] / 8.0 + 4.0 * f[3];
+          }
         }
         Prof::Acc& acc = c->prof.acc[pnd.kind];
         acc.launches += 1;

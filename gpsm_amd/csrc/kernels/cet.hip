@@ -29,6 +29,27 @@ struct CetLds {
   uint32_t cnt[WAVE];
 };
 
+// Optional work counters (profiling): per lane, candidates loaded, adjacency
+// entries examined, bitmap probes and list entries written; one atomic per
+// counter per workgroup into copy blockIdx % STAT_COPIES of stats.
+struct CetTally {
+  uint32_t cand = 0, ent = 0, probe = 0, wrote = 0;
+};
+
+__device__ __forceinline__ void flush_tally(const CetTally& t, unsigned long long* stats) {
+  __shared__ unsigned long long red[4][4];
+  const unsigned long long v[4] = {wave_sum64(t.cand), wave_sum64(t.ent), wave_sum64(t.probe), wave_sum64(t.wrote)};
+  const uint32_t w = threadIdx.x / WAVE;
+  if (lane_id() == 0)
+    for (int i = 0; i < 4; ++i) red[w][i] = v[i];
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    unsigned long long tot = 0;
+    for (uint32_t x = 0; x < blockDim.x / WAVE; ++x) tot += red[x][threadIdx.x];
+    if (tot) atomicAdd(stats + (blockIdx.x % STAT_COPIES) * STAT_STRIDE + threadIdx.x, tot);
+  }
+}
+
 // Lane l of chunk j of a table with nch chunks holds candidate j + l * nch.
 __device__ __forceinline__ uint32_t cand_index(uint32_t j, uint32_t nch, uint32_t lane) { return j + lane * nch; }
 
@@ -55,8 +76,10 @@ __global__ void __launch_bounds__(256) cet_count_kernel(const uint32_t* __restri
                                                         const uint32_t* __restrict__ bitmaps, uint32_t W,
                                                         const uint32_t* __restrict__ clist, CetDesc* descs,
                                                         const uint32_t* __restrict__ chunk_base, uint32_t ndesc,
-                                                        uint32_t* __restrict__ offsets, int per_cand) {
+                                                        uint32_t* __restrict__ offsets, int per_cand,
+                                                        unsigned long long* __restrict__ stats) {
   __shared__ CetLds lds[4];
+  CetTally tl;
   CetLds& L = lds[threadIdx.x / WAVE];
   const uint32_t lane = lane_id();
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
@@ -69,6 +92,7 @@ __global__ void __launch_bounds__(256) cet_count_kernel(const uint32_t* __restri
     bool mine;
     uint32_t D;
     load_chunk(row, clist, d, j, nch, L, mine, D);
+    tl.cand += mine;
     const uint32_t* bm_w = bitmaps + (size_t)d.w_slot * W;
     const uint32_t wlo = d.w_hi ? d.w_lo : 0u, whi = d.w_hi ? d.w_hi : ~0u;
     uint32_t tot = 0;
@@ -95,6 +119,13 @@ __global__ void __launch_bounds__(256) cet_count_kernel(const uint32_t* __restri
 #pragma unroll
       for (int u = 0; u < CET_UNROLL; ++u)
         pw[u] = cis[u] != WAVE && ks[u] >= wlo && ks[u] <= whi ? bm_w[mvs[u] >> 5] >> (mvs[u] & 31) : 0u;
+      if (stats) {
+#pragma unroll
+        for (int u = 0; u < CET_UNROLL; ++u) {
+          tl.ent += cis[u] != WAVE;
+          tl.probe += cis[u] != WAVE && ks[u] >= wlo && ks[u] <= whi;
+        }
+      }
 #pragma unroll
       for (int u = 0; u < CET_UNROLL; ++u)
         if (pw[u] & 1u) {
@@ -108,6 +139,7 @@ __global__ void __launch_bounds__(256) cet_count_kernel(const uint32_t* __restri
     if (lane == 0 && tot) atomicAdd(reinterpret_cast<unsigned long long*>(&descs[di].total), (unsigned long long)tot);
     wave_sync();
   }
+  if (stats) flush_tally(tl, stats);
 }
 
 // One workgroup per table scans its per-candidate counts in place, in tiles
@@ -151,8 +183,10 @@ __global__ void __launch_bounds__(256) cet_write_kernel(const uint32_t* __restri
                                                         const CetDesc* __restrict__ descs,
                                                         const uint32_t* __restrict__ chunk_base, uint32_t ndesc,
                                                         const uint32_t* __restrict__ offsets,
-                                                        uint32_t* __restrict__ lists) {
+                                                        uint32_t* __restrict__ lists,
+                                                        unsigned long long* __restrict__ stats) {
   __shared__ CetLds lds[4];
+  CetTally tl;
   CetLds& L = lds[threadIdx.x / WAVE];
   const uint32_t lane = lane_id();
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
@@ -165,6 +199,7 @@ __global__ void __launch_bounds__(256) cet_write_kernel(const uint32_t* __restri
     bool mine;
     uint32_t D;
     load_chunk(row, clist, d, j, nch, L, mine, D);
+    tl.cand += mine;
     // each lane's list starts at its candidate's offset; cnt advances it
     L.cnt[lane_id()] = mine ? offsets[d.off_base + cand_index(j, nch, lane_id())] : 0u;
     wave_sync();
@@ -200,6 +235,13 @@ __global__ void __launch_bounds__(256) cet_write_kernel(const uint32_t* __restri
         pw[u] = f + u * WAVE + lane < D && ks[u] >= wlo && ks[u] <= whi ? bm_w[mvs[u] >> 5] >> (mvs[u] & 31) : 0u;
 #pragma unroll
       for (int u = 0; u < CET_UNROLL; ++u) hits[u] = (pw[u] & 1u) != 0;
+      if (stats) {
+#pragma unroll
+        for (int u = 0; u < CET_UNROLL; ++u) {
+          tl.ent += f + u * WAVE + lane < D;
+          tl.probe += f + u * WAVE + lane < D && ks[u] >= wlo && ks[u] <= whi;
+        }
+      }
 #pragma unroll
       for (int u = 0; u < CET_UNROLL; ++u) {
         const uint32_t fu = f + u * WAVE, ci = cis[u];
@@ -210,7 +252,10 @@ __global__ void __launch_bounds__(256) cet_write_kernel(const uint32_t* __restri
           const unsigned long long below = (lane ? (~0ULL >> (WAVE - lane)) : 0ULL) & (~0ULL << seg_lo);
           const uint32_t rank = __popcll(hm & below);
           const uint32_t pos = L.cnt[ci] + rank;
-          if (pos < d.list_cap) out[pos] = mvs[u];
+          if (pos < d.list_cap) {
+            out[pos] = mvs[u];
+            ++tl.wrote;
+          }
         }
         wave_sync();
         if (hits[u]) atomicAdd(&L.cnt[ci], 1u);
@@ -218,6 +263,7 @@ __global__ void __launch_bounds__(256) cet_write_kernel(const uint32_t* __restri
       }
     }
   }
+  if (stats) flush_tally(tl, stats);
 }
 
 // One thread per term: the table by a binary search over the term ranges.
@@ -261,11 +307,11 @@ __global__ void __launch_bounds__(256) chunk_base_kernel(const CetDesc* __restri
 
 void launch_cet_count(const DevGraph& g, const uint32_t* bitmaps, const uint32_t* clist, CetDesc* descs,
                       uint32_t ndesc, uint32_t* offsets, int per_cand, uint32_t* chunk_base, uint32_t grid,
-                      hipStream_t s) {
+                      hipStream_t s, unsigned long long* stats) {
   if (!ndesc) return;
   hipLaunchKernelGGL(chunk_base_kernel, dim3(1), dim3(256), 0, s, descs, ndesc, chunk_base);
   hipLaunchKernelGGL(cet_count_kernel, dim3(grid), dim3(256), 0, s, g.row, g.col, g.ckey, bitmaps, g.W, clist,
-                     descs, chunk_base, ndesc, offsets, per_cand);
+                     descs, chunk_base, ndesc, offsets, per_cand, stats);
 }
 
 void launch_cet_offsets(CetDesc* descs, uint32_t ndesc, uint32_t* offsets, hipStream_t s) {
@@ -285,11 +331,11 @@ void launch_cet_terms(const uint32_t* lists, const CetDesc* descs, const uint32_
 
 void launch_cet_write(const DevGraph& g, const uint32_t* bitmaps, const uint32_t* clist, const CetDesc* descs,
                       uint32_t ndesc, const uint32_t* offsets, uint32_t* lists, uint32_t* chunk_base, uint32_t grid,
-                      hipStream_t s) {
+                      hipStream_t s, unsigned long long* stats) {
   if (!ndesc) return;
   hipLaunchKernelGGL(chunk_base_kernel, dim3(1), dim3(256), 0, s, descs, ndesc, chunk_base);
   hipLaunchKernelGGL(cet_write_kernel, dim3(grid), dim3(256), 0, s, g.row, g.col, g.ckey, bitmaps, g.W, clist,
-                     descs, chunk_base, ndesc, offsets, lists);
+                     descs, chunk_base, ndesc, offsets, lists, stats);
 }
 
 }  // namespace gpsm

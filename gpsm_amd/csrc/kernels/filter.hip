@@ -24,11 +24,12 @@ namespace {
 constexpr int MAX_CAND = FCHUNK_VERTS;  // candidates of one vertex range
 static_assert(FCHUNK_VERTS <= 32 * (WAVE - 1), "a range's bitmap words fit one wave");
 
-// Start-vertex candidates of every op: lane l of a wave owns bitmap word
-// w = 64 * block + l, loads the keys of its 32 vertices once (kept in
-// registers) and builds the word of each op of its group, so the keys are
-// read once for all ops (not once per op) and every op's words leave as one
-// coalesced store per wave.
+// Start-vertex candidates of every op.  A wave owns 64 consecutive bitmap
+// words (2048 vertices); lane l loads the keys of vertices base + l + 64 j,
+// j < 32 (each load instruction reads 256 contiguous bytes), keeps them in
+// registers, and for each op of its group one ballot per j gives the two
+// words 2j and 2j + 1, which lanes 2j and 2j + 1 keep.  The keys are read
+// once for all ops, and each op's 64 words leave as one coalesced store.
 constexpr uint32_t INIT_OPS = 16;  // ops per wave
 
 __global__ void __launch_bounds__(256) init_kernel(const uint32_t* __restrict__ key, uint32_t n, uint32_t W,
@@ -39,20 +40,25 @@ __global__ void __launch_bounds__(256) init_kernel(const uint32_t* __restrict__ 
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
   if (wave >= (uint64_t)blocks * groups) return;  // the whole wave
   const uint32_t wb = (uint32_t)(wave % blocks), og = (uint32_t)(wave / blocks);
-  const uint32_t w = wb * WAVE + lane;
-  if (w >= W) return;
-  const uint32_t v0 = w * 32;
-  const uint32_t valid = n - v0 >= 32 ? ~0u : (1u << (n - v0)) - 1u;
+  const uint32_t v0 = wb * WAVE * 32;
   uint32_t k[32];
 #pragma unroll
-  for (uint32_t j = 0; j < 32; ++j) k[j] = v0 + j < n ? key[v0 + j] : 0u;
+  for (uint32_t j = 0; j < 32; ++j) {
+    const uint32_t v = v0 + j * WAVE + lane;
+    k[j] = v < n ? key[v] : 0u;  // key 0 matches no op (dense labels start at 1)
+  }
+  const uint32_t w = wb * WAVE + lane;
   const uint32_t o_end = min(nops, (og + 1) * INIT_OPS);
   for (uint32_t o = og * INIT_OPS; o < o_end; ++o) {
-    const InitOp op = ops[o];
+    const uint32_t lo = uniform(ops[o].lo), hi = uniform(ops[o].hi), slot = uniform(ops[o].slot);
     uint32_t word = 0;
 #pragma unroll
-    for (uint32_t j = 0; j < 32; ++j) word |= (uint32_t)(k[j] >= op.lo && k[j] <= op.hi) << j;
-    bitmaps[(size_t)op.slot * W + w] = word & valid;
+    for (uint32_t j = 0; j < 32; ++j) {
+      const unsigned long long m = __ballot(k[j] >= lo && k[j] <= hi);
+      if (lane == 2 * j) word = (uint32_t)m;
+      if (lane == 2 * j + 1) word = (uint32_t)(m >> 32);
+    }
+    if (w < W) bitmaps[(size_t)slot * W + w] = word;
   }
 }
 

@@ -115,9 +115,12 @@ void launch_text_write(const uint32_t* tuples, uint64_t rows, uint32_t q, const 
 // Per CetDesc: |CET| into desc.total (atomic); per-candidate counts into
 // offsets (when per_cand != 0) at desc.off_base.
 // chunk_base: device scratch of ndesc+1 words; grid: workgroups (persistent).
+// stats (optional, STAT_COPIES x STAT_STRIDE u64, zeroed by the caller):
+// entries 0..3 = candidates, adjacency entries examined, bitmap probes, list
+// entries written (cet_write only), summed over the copies.
 void launch_cet_count(const DevGraph& g, const uint32_t* bitmaps, const uint32_t* clist, CetDesc* descs,
                       uint32_t ndesc, uint32_t* offsets, int per_cand, uint32_t* chunk_base, uint32_t grid,
-                      hipStream_t s);
+                      hipStream_t s, unsigned long long* stats = nullptr);
 // Segmented exclusive scan of the per-candidate counts of every desc.
 void launch_cet_offsets(CetDesc* descs, uint32_t ndesc, uint32_t* offsets, hipStream_t s);
 // Hash terms of the last join levels' lists: for table term_desc[i] (terms
@@ -128,7 +131,7 @@ void launch_cet_terms(const uint32_t* lists, const CetDesc* descs, const uint32_
 // Writes the lists.
 void launch_cet_write(const DevGraph& g, const uint32_t* bitmaps, const uint32_t* clist, const CetDesc* descs,
                       uint32_t ndesc, const uint32_t* offsets, uint32_t* lists, uint32_t* chunk_base, uint32_t grid,
-                      hipStream_t s);
+                      hipStream_t s, unsigned long long* stats = nullptr);
 
 // join.hip ---------------------------------------------------------------------
 struct JoinArgs {

@@ -48,7 +48,7 @@ def test_compact(ctx, nbits, density):
     assert np.array_equal(ctx.compact(words, nbits), O.compact_bitmap(words, nbits))
 
 
-@pytest.mark.parametrize("name", ["hand_aba.json", "hand_refine.json"])
+@pytest.mark.parametrize("name", ["hand_aba.json", "hand_refine.json", "hand_explore_prefix.json"])
 def test_hand_fixture(ctx, name):
     fx, g, q = hand_fixture(name)
     rg = ctx.upload(g)

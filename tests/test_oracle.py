@@ -123,7 +123,7 @@ def hand_fixture(name="hand_aba.json"):
     return fx, g, q
 
 
-@pytest.mark.parametrize("name", ["hand_aba.json", "hand_refine.json"])
+@pytest.mark.parametrize("name", ["hand_aba.json", "hand_refine.json", "hand_explore_prefix.json"])
 def test_hand_worked_fixture(name):
     """Every intermediate of the reference's steps, worked by hand: a case
     where refine changes nothing, and one where explore and refine remove
