@@ -1,18 +1,25 @@
 """Benchmark of the GpSM filter-and-join matcher on one MI355X.
 
-Workload: Chung-Lu power-law data graph, 1,000,000 vertices and 8,000,000
-undirected edges (gamma 2.1), 16 labels with Zipf(1.0) frequencies; 100
-random-walk queries drawn with 8-16 vertices, half tree-like and half with
-all induced edges, under the feasibility rule of
-gpsm_gen_queries (embedding-count bound 1e8; slots shrink until feasible).
-All inputs are synthetic and seeded.
+Workload: the batched / learned-planner configuration on a Chung-Lu
+power-law data graph (1,000,000 vertices, 8,000,000 undirected edges, gamma
+2.1, 16 labels with Zipf(1.0) frequencies): 256 random-walk query slots
+(half tree-like, half with every induced edge), each draw taking a size
+uniform in 4-32, kept by the device-exact feasibility rule (the exact join
+with the reference's heuristic order finishes within 1e10 examined
+candidates; resample only, sizes never shrunk).  The set is generated once
+by gpsm_amd.workloads (--generate) and read from gpsm_amd/data/.  The
+100-slot 8-16-vertex configuration on the same graph keeps no query under
+the same rule; its record (draws, lower bounds on the rejected draws'
+embedding counts) is reported in the JSON line.
 
 A step is one pass of the matcher over the whole query set (every query's
 exact embedding count and set hash), with the data graph resident in HBM.
 The headline path is the batched launch (filter supersteps for all queries,
 one persistent join grid) with the learned join order when a trained planner
-is present, else the reference heuristic order; the plain path (one query at
-a time, heuristic order, the reference's structure) is timed beside it.
+is present, else the reference heuristic order.  Beside it: the heuristic
+order on the same path (alternating batches), the same path replayed through
+the cached hipGraphs, and the plain path (one query at a time, heuristic
+order, the reference's structure).
 
 Prints one JSON line.
 """
@@ -27,7 +34,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic__chunglu1M_100q_walk8_16.json")
+WORKLOAD = "chunglu1m_mix4_32"
+STATED = "chunglu1m_walk8_16"
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic__chunglu1M_mix256.json")
+LIMITER_JSON = os.path.join(ROOT, "profiles", "valu__join_chunglu1M_mix256.json")
+SAMPLE_MAX_WORK = 5e8  # heaviest query of the CPU sample (examined candidates)
 
 
 def cpu_model():
@@ -43,11 +54,9 @@ def cpu_model():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--budget", type=float, default=1e8, help="feasibility bound on embeddings per query")
-    ap.add_argument("--queries", type=int, default=100)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline sample length")
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline sample length (per mode)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
@@ -73,13 +82,14 @@ def main():
         sys.exit(2)
 
     from gpsm_amd import _lib
-    from gpsm_amd import graphs as G
     from gpsm_amd import planner
+    from gpsm_amd import workloads as WL
     from gpsm_amd.matcher import Context
 
     t0 = time.time()
-    g = G.chunglu_1m()
-    qs = G.queries(g, args.queries, 8, 16, _lib.Q_WALK_HALF, 2, int(args.budget))
+    qs, rec = WL.load_set(WORKLOAD)
+    _, stated = WL.load_set(STATED)
+    g = WL.chunglu1m()
     t_gen = time.time() - t0
     sizes = [q.n for q in qs]
 
@@ -93,9 +103,10 @@ def main():
 
     tuning = None
     if not args.no_autotune:
-        # filter block, join buffer rows and join workgroups per CU, chosen on
-        # this query set before the timed region (results checked equal)
-        _, tuning = rg.autotune(qs, order_mode=mode)
+        # filter block, join buffer rows, join workgroups per CU and tasks per
+        # workgroup, chosen on the first 64 queries before the timed region
+        # (results checked equal)
+        _, tuning = rg.autotune(qs[:64], order_mode=mode)
     for _ in range(args.warmup):
         rg.match_batch(qs, order_mode=mode)
     # algorithmic bytes per launch: one untimed step with the device work
@@ -103,8 +114,8 @@ def main():
     ctx.profile(2)
     rg.match_batch(qs, order_mode=mode)
     counted = ctx.kernel_stats()
-    # timed steps: HIP events around every join launch (profile mode 3: no
-    # events between the other launches), no counters
+    # timed steps: HIP events around every join launch on the matcher's
+    # stream (profile mode 3: no events between the other launches), no counters
     ctx.profile(3)
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -126,13 +137,14 @@ def main():
     ms_per_step = 1e3 * elapsed / args.steps
     emb = sum(r.count for r in res)
     qps = len(qs) * args.steps / elapsed
+    assert all(r.status == 0 for r in res), "a query of the stored set failed"
+    want = [(q["count"], int(q["hash"], 16)) for q in rec["queries"]]
+    assert [(r.count, r.hash) for r in res] == want, "results differ from the stored set's"
 
-    # the heuristic order in the same batched launch, and the plain path
     extra = {}
+    # learned vs heuristic order on the same batched path: the two alternate
+    # batch by batch, so that drift of the device's clocks affects both alike
     if planner_used:
-        # learned vs heuristic order on the same batched path: the two
-        # alternate batch by batch, so that drift of the device's clocks
-        # affects both alike; same timing mode as the timed steps
         cmp = {"learned": [0.0, 0.0], "heuristic": [0.0, 0.0]}
         for _ in range(args.steps):
             for name, m in (("learned", mode), ("heuristic", _lib.ORDER_HEURISTIC)):
@@ -149,10 +161,25 @@ def main():
         extra["alternating_qps"] = {k: round(len(qs) * args.steps / v[0], 3) for k, v in cmp.items()}
         extra["join_ms_per_step"] = {k: round(v[1] / args.steps, 3) for k, v in cmp.items()}
         assert [(r.count, r.hash) for r in rh] == [(r.count, r.hash) for r in res]
-        # join work (first-table rows + candidates examined) of both orders
         extra["join_work_learned"] = sum(r.join_work for r in res)
         extra["join_work_heuristic"] = sum(r.join_work for r in rh)
         extra["plans_changed"] = sum(a.order != b.order for a, b in zip(res, rh))
+    # the same batched path replayed through the context's cached hipGraphs
+    # (captured phases updated in place), alternating with plain launches
+    gcmp = {"graph": 0.0, "launches": 0.0}
+    i0, u0 = ctx.graph_cache_stats()
+    for _ in range(args.steps):
+        for name, ug in (("graph", True), ("launches", False)):
+            t0 = time.perf_counter()
+            r = rg.match_batch(qs, order_mode=mode, use_graph=ug)
+            gcmp[name] += time.perf_counter() - t0
+            if ug:
+                assert [(x.count, x.hash) for x in r] == [(x.count, x.hash) for x in res]
+    i1, u1 = ctx.graph_cache_stats()
+    extra["graph_replay_qps"] = len(qs) * args.steps / gcmp["graph"]
+    extra["graph_over_launches"] = gcmp["launches"] / gcmp["graph"]
+    extra["graph_instantiations"] = i1 - i0
+    extra["graph_updates"] = u1 - u0
     if not args.no_plain:
         rg.match(qs[0])
         t0 = time.perf_counter()
@@ -182,35 +209,62 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": round(bytes_per_launch),
                 "launches": st["launches"]}
+    if os.path.exists(LIMITER_JSON):
+        # what limits the kernel instead, from a rocprofv3 counter run of it
+        lim = json.load(open(LIMITER_JSON)).get(name + "_kernel")
+        if lim:
+            roofline["limiter"] = lim
     kernels = {k: {"ms_per_step": round(v["ms"] / args.steps, 3), "launches_per_step": v["launches"] / args.steps,
-                   "GBps": round(v["bytes"] / (v["ms"] / 1e3) / 1e9, 1) if v["bytes"] and v["ms"] else None}
+                   "GBps": round(v["bytes"] / (v["ms"] / 1e3) / 1e9, 1) if v["bytes"] and v["ms"] else None,
+                   "frac_of_hbm_peak": round(v["bytes"] / (v["ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+                   if v["bytes"] and v["ms"] else None}
                for k, v in stats.items()}
 
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as O
-        # all cores: O2 across the whole query set (OpenMP, one query per thread)
+        # a bounded sample of the same workload: queries in slot order whose
+        # device-measured join work is at most SAMPLE_MAX_WORK (a heavier one
+        # alone would run for minutes on one core); the rates are scaled to
+        # the whole set by join work (the CPU's time is dominated by its join)
         threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        t0 = time.perf_counter()
-        many = O.o2_match_many(g, qs, threads=threads)
-        dt_all = time.perf_counter() - t0
-        assert many == [(r.count, r.hash) for r in res], "CPU oracle disagrees with the device"
-        # one thread: the first queries until the sample length has elapsed
-        done, t0 = 0, time.perf_counter()
-        for q, r in zip(qs, res):
-            ro = O.o2_match(g, q, mode=0)
-            assert (ro["count"], ro["hash"]) == (r.count, r.hash), "CPU oracle disagrees with the device"
+        work = [r.join_work for r in res]
+        total_work = sum(work)
+        light = [i for i, w in enumerate(work) if w <= SAMPLE_MAX_WORK]
+        # one thread: sample queries until the sample length has elapsed
+        done, done_work, t0 = 0, 0, time.perf_counter()
+        for i in light:
+            ro = O.o2_match(g, qs[i], mode=0)
+            assert (ro["count"], ro["hash"]) == (res[i].count, res[i].hash), "CPU oracle disagrees with the device"
             done += 1
+            done_work += work[i]
             if time.perf_counter() - t0 > args.cpu_seconds:
                 break
-        dt = time.perf_counter() - t0
-        cpu = {"value": round(len(qs) / dt_all, 4), "unit": "queries/s", "cores": threads, "kind": "port",
-               "sample": "O2 host restatement (g++ -O3) over all %d queries, OpenMP across queries on %d threads, "
-                         "%.1f s; every count and hash equal to the device's" % (len(qs), threads, dt_all),
-               "single_thread": {"value": round(done / dt, 4), "cores": 1,
-                                 "sample": "first %d of the %d queries in %.1f s" % (done, len(qs), dt)},
-               "embeddings_per_sec": round(emb / dt_all), "cpu": cpu_model(), "nproc": os.cpu_count()}
+        dt1 = time.perf_counter() - t0
+        rate1 = done_work / dt1  # examined candidates per second, one thread
+        # all cores: sample queries up to about the same length
+        target = rate1 * threads * args.cpu_seconds
+        pick, acc = [], 0
+        for i in light:
+            if acc >= target:
+                break
+            pick.append(i)
+            acc += work[i]
+        t0 = time.perf_counter()
+        many = O.o2_match_many(g, [qs[i] for i in pick], threads=threads)
+        dt_all = time.perf_counter() - t0
+        assert many == [(res[i].count, res[i].hash) for i in pick], "CPU oracle disagrees with the device"
+        full_s = dt_all * total_work / max(acc, 1)
+        cpu = {"value": round(len(qs) / full_s, 4), "unit": "queries/s", "cores": threads, "kind": "port",
+               "sample": "O2 host restatement (g++ -O3), OpenMP across queries on %d threads, over %d of the %d "
+                         "queries (slot order, join work <= %.0e each; %.3g of %.3g examined candidates) in %.1f s, "
+                         "scaled to the whole set by join work; every count and hash equal to the device's" % (
+                             threads, len(pick), len(qs), SAMPLE_MAX_WORK, acc, total_work, dt_all),
+               "single_thread": {"value": round(len(qs) / (dt1 * total_work / max(done_work, 1)), 4), "cores": 1,
+                                 "sample": "%d queries (%.3g examined candidates) in %.1f s, scaled by join work"
+                                           % (done, done_work, dt1)},
+               "embeddings_per_sec": round(emb / full_s), "cpu": cpu_model(), "nproc": os.cpu_count()}
 
     out = {
         "metric": "queries/sec + embeddings/sec, learned vs plain join order, same GPU, exact",
@@ -225,21 +279,32 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": "chunglu-1M-8M-zipf16/%d-walk-queries" % len(qs), "n": g.n, "undirected_edges": g.adj // 2,
-                   "queries": len(qs), "query_sizes": {str(k): sizes.count(k) for k in sorted(set(sizes))},
-                   "feasibility_budget": args.budget, "order": "learned" if planner_used else "heuristic",
-                   "tuning": tuning,
+        "config": {"workload": "chunglu-1M-8M-zipf16/256-walk-queries-4to32-feasible", "n": g.n,
+                   "undirected_edges": g.adj // 2, "queries": len(qs),
+                   "query_sizes": {str(k): sizes.count(k) for k in sorted(set(sizes))},
+                   "query_edges_total": sum(q.m for q in qs),
+                   "feasibility": {"rule": "exact device join with the heuristic order within the work budget; "
+                                           "fresh size per draw, never shrunk",
+                                   "work_budget": rec["rule"]["work_budget"], "slots": rec["rule"]["slots"],
+                                   "draws": rec["draws"], "dropped_slots": rec["dropped_slots"]},
+                   "stated_8_16": {"slots": stated["rule"]["slots"], "draws": stated["draws"],
+                                   "kept": stated["kept"],
+                                   "rejected_count_lower_bound_quantiles":
+                                       stated["rejected_count_lower_bound_quantiles"]},
+                   "order": "learned" if planner_used else "heuristic", "tuning": tuning,
                    "device": ctx.device_name},
         "embeddings_per_step": emb,
         "embeddings_per_sec": round(emb * args.steps / elapsed),
+        "join_work_per_step": sum(r.join_work for r in res),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
         "phase_ms_per_step": phases,
-        "workload_gen_s": round(t_gen, 1),
+        "workload_load_s": round(t_gen, 1),
     }
     out.update({k: (round(v, 3) if isinstance(v, float) else v) for k, v in extra.items()})
     print(json.dumps(out))
+    ctx.close()
 
 
 if __name__ == "__main__":

@@ -25,8 +25,11 @@ ERRORS = {
     0: "ok", -1: "invalid argument", -2: "out of memory", -3: "HIP runtime error", -4: "query too large",
     -5: "tuple capacity exceeded", -6: "no HIP device", -7: "query disconnected or without edges",
     -8: "malformed or unterminated input", -9: "join work budget exceeded",
+    -10: "input larger than the fixed scratch of an allocation-free entry point",
 }
 EBUDGET = -9
+ESCRATCH = -10
+SCAN_NOALLOC_MAX = 32767 * 4096  # GPSM_SCAN_NOALLOC_MAX
 
 
 class GpsmError(RuntimeError):
@@ -125,6 +128,7 @@ SIGNATURES = {
     "gpsm_set_tuning": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "gpsm_profile_read": (C.c_int, [C.c_void_p, C.POINTER(KStat), C.c_uint32, C.POINTER(C.c_uint32)]),
     "gpsm_graph_cache_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "gpsm_scan_u32_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     # generators and text I/O (include/gpsm_gen.h)
     "gpsm_gen_csr_from_edges": (C.c_int, [C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.POINTER(CSR)]),

@@ -39,6 +39,12 @@ using namespace gpsm;
     }                                                   \
   } while (0)
 
+// Scratch of the allocation-free scan (gpsm_scan_u32_noalloc): the scan's
+// total (2 words) and its block sums.  In host code the array's name is not
+// a device address: gpsm_init takes the address of this device's copy with
+// hipGetSymbolAddress and every use goes through that pointer.
+static __device__ __align__(256) uint32_t scratch[GPSM_SCAN_NOALLOC_SCRATCH_BYTES / sizeof(uint32_t)];
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -245,6 +251,7 @@ const char* gpsm_strerror(int code) {
     case GPSM_EDISCONN: return "query not connected or without edges";
     case GPSM_EFORMAT: return "malformed or unterminated input";
     case GPSM_EBUDGET: return "join work budget exceeded";
+    case GPSM_ESCRATCH: return "input larger than the fixed scratch of an allocation-free entry point";
     default: return "unknown error";
   }
 }
@@ -281,6 +288,17 @@ int gpsm_init(int device, gpsm_ctx** out) {
     delete c;
     return GPSM_ENOMEM;
   }
+  // this device's copy of the allocation-free scan's scratch array
+  if (hipGetSymbolAddress(reinterpret_cast<void**>(&c->noalloc_scratch), HIP_SYMBOL(scratch)) != hipSuccess ||
+      !c->noalloc_scratch) {
+    set_error("hipGetSymbolAddress(scratch)", hipGetLastError());
+    hipEventDestroy(c->ev0);
+    hipEventDestroy(c->ev1);
+    hipStreamDestroy(c->stream);
+    c->bufs[B_SCRATCH].release();
+    delete c;
+    return GPSM_EDEVICE;
+  }
   *out = c;
   return GPSM_OK;
 }
@@ -301,6 +319,27 @@ void gpsm_shutdown(gpsm_ctx* c) {
 }
 
 const char* gpsm_device_name(const gpsm_ctx* c) { return c ? c->name : ""; }
+
+int gpsm_scan_u32_noalloc(gpsm_ctx* c, const uint32_t* d_in, uint32_t* d_out, uint64_t n, uint64_t* total) {
+  if (!c || (n && (!d_in || !d_out))) return GPSM_EINVAL;
+  if (total) *total = 0;
+  if (n == 0) return GPSM_OK;
+  if (n > GPSM_SCAN_NOALLOC_MAX) return GPSM_ESCRATCH;
+  if (hipSetDevice(c->device) != hipSuccess) return GPSM_EDEVICE;
+  hipStream_t s = c->stream;
+  // words 0-1: the total; from word 2: two words per block sum
+  uint32_t* scr = c->noalloc_scratch;
+  unsigned long long* dtot = reinterpret_cast<unsigned long long*>(scr);
+  const uint64_t words = GPSM_SCAN_NOALLOC_SCRATCH_BYTES / sizeof(uint32_t);
+  launch_exclusive_scan(d_in, d_out, n, dtot, scr + 2, words - 2, s);
+  unsigned long long t = 0;
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(&t, dtot, sizeof(t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return GPSM_EDEVICE;
+  if (total) *total = t;
+  return GPSM_OK;
+}
 
 }  // extern "C"
 

@@ -175,6 +175,9 @@ struct gpsm_ctx {
   // topology is the same (hipGraphExecUpdate) and is instantiated afresh
   // only when it is not.  graph_instantiations counts the latter.
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  // this device's copy of the allocation-free scan's static scratch array
+  // (hipGetSymbolAddress at gpsm_init)
+  uint32_t* noalloc_scratch = nullptr;
   uint64_t graph_instantiations = 0, graph_updates = 0;
   // hub adjacency bitmaps of later uploads: vertices of degree > the hub
   // degree, starting at hub_min_degree and doubled until they fit hub_max_bytes

@@ -191,6 +191,9 @@ struct Feasibility {
   }
 };
 
+// Draws per slot before gpsm_gen_queries gives up (GPSM_EINVAL).
+constexpr int kMaxDraws = 3200;
+
 struct QueryOut {
   std::vector<uint32_t> labels;
   std::vector<std::pair<uint32_t, uint32_t>> edges;
@@ -460,15 +463,16 @@ int gpsm_gen_queries(const gpsm_csr* g, uint32_t nq, uint32_t kmin, uint32_t kma
 #pragma omp parallel for schedule(dynamic, 1)
   for (int64_t slot = 0; slot < (int64_t)nq; ++slot) {
     SplitMix64 rng(seed, 1000 + (uint64_t)slot);
-    uint32_t k = kmin + (uint32_t)rng.below(kmax - kmin + 1);
     int qkind = kind == GPSM_Q_WALK_HALF ? ((slot & 1) ? GPSM_Q_WALK_INDUCED : GPSM_Q_WALK_TREE) : kind;
     Feasibility fz;
     fz.g = g;
     fz.by_label = &by_label;
     bool done = false;
-    // after every 50 rejected draws the slot's size shrinks by one vertex
-    for (int attempt = 0; attempt < 50 * 64 && !done; ++attempt) {
-      if (attempt && attempt % 50 == 0 && k > std::min<uint32_t>(kmin, 3)) --k;
+    // every draw takes a fresh size, uniform in [kmin, kmax]: a slot's query
+    // follows the size distribution conditioned on passing the rule, and no
+    // size is ever shrunk below kmin
+    for (int attempt = 0; attempt < kMaxDraws && !done; ++attempt) {
+      const uint32_t k = kmin + (uint32_t)rng.below(kmax - kmin + 1);
       QueryOut q;
       bool ok;
       if (qkind == GPSM_Q_CYCLE) ok = cycle_query(g, rng, k, q);

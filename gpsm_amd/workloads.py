@@ -2,71 +2,109 @@
 
 * er10k_walk6: ER 10k vertices / 40k edges, 8 uniform labels, one 6-vertex
   walk query (the size the CPU enumerator finishes in milliseconds).
-* chunglu1m_walk8_16: Chung-Lu 1M/8M (gamma 2.1), 16 Zipf(1.0) labels, 100
-  walk queries drawn with 8-16 vertices, half tree-like, half induced.
-* chunglu1m_mix4_32: the same graph with 256 walk queries drawn with 4-32
-  vertices (matched with the heuristic or the learned join order).
-* rmat22_64m + rmat_shape_queries_device: R-MAT scale 22 (a, b, c, d =
-  .57, .19, .19, .05), 64M undirected edges, 32 Zipf(0.8) labels; cycles
-  C4-C8, cliques K4-K8 and induced walks of 8-50 vertices.
+* chunglu1m_mix4_32: Chung-Lu 1M/8M (gamma 2.1), 16 Zipf(1.0) labels, 256
+  walk queries (half tree-like, half induced), each drawn with a size uniform
+  in 4-32 and kept by the device-exact feasibility rule below.
+* chunglu1m_walk8_16: the same graph with 100 walk query slots of 8-16
+  vertices under the same rule (on this graph no draw of 7 or more vertices
+  passes it, so every slot is dropped; generate() records the evidence).
+* rmat22_64m + rmat22_shapes: R-MAT scale 22 (a, b, c, d = .57, .19, .19,
+  .05), 64M undirected edges, 32 Zipf(0.8) labels; cycles and cliques of
+  4-8 vertices and induced walks of 8-50 vertices under the same rule.
 
-The Chung-Lu queries pass the feasibility rule of gpsm_gen_queries (an exact
-upper bound on the embedding count, `budget`; slots shrink until it holds).
-The R-MAT queries pass the device-exact rule of feasible_queries (the join
-finishes within `budget` examined candidates).  Realised sizes are reported
-with every result.
+Feasibility rule (device-exact, resample only).  Every candidate query is
+drawn with a fresh size uniform in the slot's range (never shrunk) and is
+kept if the device's exact join of it, with the reference's heuristic join
+order, finishes within `work_budget` examined candidates (first-table rows
+plus candidates examined at every level, JoinOut::work); a query over the
+budget is abandoned by the join, so a rejected draw costs at most the
+budget.  A slot takes the first kept candidate of its draws, in draw order;
+a slot with none after `rounds` rounds of `per_round` draws is dropped and
+counted.  Every decision is an exact property of the query, so the set does
+not depend on the device or the thread count.  The kept queries therefore
+follow the drawn size distribution conditioned on feasibility.
+
+Generating a set runs many budgeted joins, so the sets the benchmark and the
+size-scale tests use are generated once (``python -m gpsm_amd.workloads
+--generate gpsm_amd/data``) and stored as JSON under gpsm_amd/data/, with the
+rule's parameters and the outcome of every slot.
 """
+import json
+import os
+
+import numpy as np
+
 from . import _lib
 from . import graphs as G
 
 EMBEDDING_BUDGET = 10 ** 8
+WORK_BUDGET = 10 ** 10  # examined candidates per query (about 20 ms of join on one MI355X)
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+MATCH_CHUNK = 128  # candidates per batch of the rule
 
 
-def feasible_queries(rg, nq, kmin, kmax, kinds, seed, work_budget, per_slot=6, rounds=4, log=None):
-    """Queries that pass an exact device-measured feasibility rule.
+def _match_split(rg, qs, work_budget, order_mode):
+    """match_batch, halving the batch while the device runs out of memory
+    (the candidate-edge tables of a batch of large draws can be tens of GB)."""
+    try:
+        return rg.match_batch(qs, work_budget=work_budget, order_mode=order_mode)
+    except _lib.GpsmError as e:
+        if e.code != -2 or len(qs) == 1:
+            raise
+    h = len(qs) // 2
+    return _match_split(rg, qs[:h], work_budget, order_mode) + _match_split(rg, qs[h:], work_budget, order_mode)
 
-    For output slot i, per_slot candidates of kind kinds[i % len(kinds)] are
-    drawn (sizes uniform in the round's range, no host-side rule), matched in
-    one batch with a join work budget, and the first candidate whose join
-    finishes within the budget is kept.  Slots left empty are drawn again in
-    the next round with the upper size bound halved towards kmin; after
-    `rounds` rounds a slot that is still empty is dropped.  Every decision is
-    an exact property of the query, so the set does not depend on the device
-    or thread count.  Returns (queries, number of dropped slots)."""
+
+def feasible_queries(rg, nq, kmin, kmax, kinds, seed, work_budget, per_round=8, rounds=8, log=None,
+                     order_mode=_lib.ORDER_HEURISTIC):
+    """Queries kept by the device-exact rule (module docstring).
+
+    Slot i draws candidates of kind kinds[i % len(kinds)], each with a fresh
+    size uniform in [kmin, kmax]; per round, every open slot draws per_round
+    candidates, all candidates of a round are matched in one batch with the
+    work budget, and a slot keeps its first candidate within the budget.
+    Returns (queries in slot order, dropped slot count, stats) where stats
+    holds the draws made, the kept draws' (slot, round, index) and, for the
+    rejected draws, the embeddings already found when the join was
+    abandoned (each a lower bound on that draw's embedding count)."""
     picks = [None] * nq
-    hi = kmax
+    where = [None] * nq
+    draws = 0
+    lower_bounds = []
     for r in range(rounds):
         for ki, kind in enumerate(kinds):
             slots = [s for s in range(ki, nq, len(kinds)) if picks[s] is None]
             if not slots:
                 continue
-            cands = G.queries(rg.g, len(slots) * per_slot, kmin, hi, kind, seed + 7919 * ki + 104729 * r, 0)
-            res = rg.match_batch(cands, work_budget=work_budget)
+            cands = G.queries(rg.g, len(slots) * per_round, kmin, kmax, kind, seed + 7919 * ki + 104729 * r, 0)
+            draws += len(cands)
+            res = []
+            for c0 in range(0, len(cands), MATCH_CHUNK):  # bounded candidate-edge tables per batch
+                res += _match_split(rg, cands[c0:c0 + MATCH_CHUNK], work_budget, order_mode)
             for si, slot in enumerate(slots):
-                for j in range(per_slot):
-                    if res[si * per_slot + j].status == 0:
-                        picks[slot] = cands[si * per_slot + j]
-                        break
+                for j in range(per_round):
+                    c = si * per_round + j
+                    if res[c].status == 0:
+                        if picks[slot] is None:
+                            picks[slot] = cands[c]
+                            where[slot] = (slot, r, j)
+                    elif res[c].status == _lib.EBUDGET:
+                        lower_bounds.append(res[c].level_rows[cands[c].n - 1])
             if log:
-                log("  feasibility round %d, kind %d, sizes %d-%d: %d of %d slots filled"
-                    % (r, kind, kmin, hi, sum(picks[s] is not None for s in slots), len(slots)))
-        hi = max(kmin, (kmin + hi) // 2)
+                log("  feasibility round %d, kind %d, sizes %d-%d: %d of %d open slots filled"
+                    % (r, kind, kmin, kmax, sum(picks[s] is not None for s in slots), len(slots)))
     out = [q for q in picks if q is not None]
-    return out, nq - len(out)
+    stats = {"draws": draws, "kept": [w for w in where if w is not None],
+             "rejected_count_lower_bounds": lower_bounds}
+    return out, nq - len(out), stats
 
 
 def er10k_walk6(seed=1):
     return G.er10k_walk6(seed)
 
 
-def chunglu1m_walk8_16(seed=2, budget=EMBEDDING_BUDGET, nq=100):
-    g = G.chunglu_1m(seed)
-    return g, G.queries(g, nq, 8, 16, _lib.Q_WALK_HALF, seed, budget)
-
-
-def chunglu1m_mix4_32(seed=2, budget=EMBEDDING_BUDGET, nq=256, g=None):
-    g = g if g is not None else G.chunglu_1m(seed)
-    return g, G.queries(g, nq, 4, 32, _lib.Q_WALK_HALF, seed + 1000, budget)
+def chunglu1m(seed=2):
+    return G.chunglu_1m(seed)
 
 
 def rmat22_64m(seed=5, scale=22, m=64_000_000):
@@ -74,42 +112,86 @@ def rmat22_64m(seed=5, scale=22, m=64_000_000):
     return G.set_labels(g, 32, 0.8, seed)
 
 
-def rmat_shape_queries_device(rg, seed=5, work_budget=EMBEDDING_BUDGET, per_kind=8, per_slot=6, log=None):
-    """Cycles C4-C8, cliques K4-K8 and induced walks of 8-50 vertices drawn at
-    their sizes and kept by the device-exact rule (feasible_queries).  On the
-    R-MAT graph the spanning-tree bound of the host rule passes almost no
-    cycle or clique of 4 or more vertices (paths through 10^5-degree hubs),
-    and evaluating it for one 8-50-vertex walk takes minutes of host time.
-    Returns (queries, dropped slots)."""
-    qs, dropped = [], 0
-    for kind, kmin, kmax, s in ((_lib.Q_CYCLE, 4, 8, seed), (_lib.Q_CLIQUE, 4, 8, seed + 1),
-                                (_lib.Q_WALK_INDUCED, 8, 50, seed + 2)):
-        q, d = feasible_queries(rg, per_kind, kmin, kmax, [kind], s, work_budget, per_slot, log=log)
-        qs += q
-        dropped += d
-    return qs, dropped
+# The stored query sets: (graph builder, slots, size range, kinds, seed,
+# draws per round, rounds).
+SETS = {
+    "chunglu1m_mix4_32": ("chunglu1m", 256, 4, 32, [_lib.Q_WALK_TREE, _lib.Q_WALK_INDUCED], 1002, 8, 10),
+    "chunglu1m_walk8_16": ("chunglu1m", 100, 8, 16, [_lib.Q_WALK_TREE, _lib.Q_WALK_INDUCED], 2, 8, 1),
+    "rmat22_cycles4_8": ("rmat22_64m", 16, 4, 8, [_lib.Q_CYCLE], 5, 8, 4),
+    "rmat22_cliques4_8": ("rmat22_64m", 16, 4, 8, [_lib.Q_CLIQUE], 6, 8, 4),
+    "rmat22_walks8_50": ("rmat22_64m", 16, 8, 50, [_lib.Q_WALK_INDUCED], 7, 8, 1),
+}
+GRAPHS = {"chunglu1m": chunglu1m, "rmat22_64m": rmat22_64m}
 
 
-def rmat_shape_queries(g, seed=5, budget=EMBEDDING_BUDGET, per_kind=8):
-    """The same shapes under the host rule (sizes shrink until the bound holds)."""
-    qs = []
-    qs += G.queries(g, per_kind, 4, 8, _lib.Q_CYCLE, seed, budget)
-    qs += G.queries(g, per_kind, 4, 8, _lib.Q_CLIQUE, seed + 1, budget)
-    qs += G.queries(g, per_kind, 8, 50, _lib.Q_WALK_INDUCED, seed + 2, budget)
-    return qs
+def set_path(name, data_dir=DATA_DIR):
+    return os.path.join(data_dir, name + ".json")
+
+
+def load_set(name, data_dir=DATA_DIR):
+    """(queries, record) of a stored query set."""
+    with open(set_path(name, data_dir)) as f:
+        rec = json.load(f)
+    qs = [G.Query(np.array(q["labels"], dtype=np.uint32), np.array(q["src"], dtype=np.uint32),
+                  np.array(q["dst"], dtype=np.uint32)) for q in rec["queries"]]
+    return qs, rec
+
+
+def generate(names, data_dir=DATA_DIR, log=print, work_budget=WORK_BUDGET):
+    """Runs the device-exact rule for the named sets and stores them."""
+    from .matcher import Context
+
+    os.makedirs(data_dir, exist_ok=True)
+    with Context(0) as ctx:
+        by_graph = {}
+        for name in names:
+            by_graph.setdefault(SETS[name][0], []).append(name)
+        for gname, group in by_graph.items():
+            g = GRAPHS[gname]()
+            rg = ctx.upload(g)
+            log("%s: %d vertices, %d undirected edges resident" % (gname, g.n, g.adj // 2))
+            for name in group:
+                _, nq, kmin, kmax, kinds, seed, per_round, rounds = SETS[name]
+                qs, dropped, st = feasible_queries(rg, nq, kmin, kmax, kinds, seed, work_budget, per_round, rounds,
+                                                   log=log)
+                res = rg.match_batch(qs) if qs else []
+                lb = sorted(st["rejected_count_lower_bounds"])
+                rec = {
+                    "workload": name, "graph": gname, "n": g.n, "undirected_edges": g.adj // 2,
+                    "rule": {"work_budget": work_budget, "slots": nq, "sizes": [kmin, kmax], "kinds": kinds,
+                             "seed": seed, "draws_per_round": per_round, "rounds": rounds,
+                             "order": "heuristic"},
+                    "draws": st["draws"], "kept": len(qs), "dropped_slots": dropped,
+                    "rejected_draws": len(lb),
+                    "rejected_count_lower_bound_quantiles": (
+                        [lb[int(p * (len(lb) - 1))] for p in (0.0, 0.25, 0.5, 0.75, 1.0)] if lb else None),
+                    "query_sizes": {str(k): [q.n for q in qs].count(k) for k in sorted({q.n for q in qs})},
+                    "queries": [{"slot": w[0], "round": w[1], "draw": w[2], "labels": q.labels.tolist(),
+                                 "src": q.src.tolist(), "dst": q.dst.tolist(), "count": r.count,
+                                 "hash": "%016x" % r.hash, "work": r.join_work}
+                                for q, r, w in zip(qs, res, st["kept"])],
+                }
+                with open(set_path(name, data_dir), "w") as f:
+                    json.dump(rec, f, separators=(",", ":"))
+                log("%s: %d draws, %d kept (sizes %s), %d slots dropped" % (
+                    name, st["draws"], len(qs), rec["query_sizes"], dropped))
+            rg.free()
 
 
 def build(name, seed=None, budget=EMBEDDING_BUDGET):
-    """(graph, queries) of a named workload."""
+    """(graph, queries) of a named workload; stored sets are read from
+    gpsm_amd/data (queries None for a graph without a stored set)."""
     if name == "er10k_walk6":
         return er10k_walk6(*(() if seed is None else (seed,)))
-    if name == "chunglu1m_walk8_16":
-        return chunglu1m_walk8_16(2 if seed is None else seed, budget)
-    if name == "chunglu1m_mix4_32":
-        return chunglu1m_mix4_32(2 if seed is None else seed, budget)
-    if name == "rmat22_64m":
-        g = rmat22_64m(5 if seed is None else seed)
-        return g, None  # queries need the device (rmat_shape_queries_device)
+    if name in SETS:
+        qs, _ = load_set(name)
+        return GRAPHS[SETS[name][0]](), qs
+    if name == "rmat22_shapes":
+        g = rmat22_64m()
+        qs = []
+        for part in ("rmat22_cycles4_8", "rmat22_cliques4_8", "rmat22_walks8_50"):
+            qs += load_set(part)[0]
+        return g, qs
     raise ValueError("unknown workload %r" % name)
 
 
@@ -130,8 +212,6 @@ def run(name, steps=5, autotune=False, plain=True, seed=None, budget=EMBEDDING_B
         rg = ctx.upload(g)
         log("graph resident on %s" % ctx.device_name)
         dropped = None
-        if qs is None:
-            qs, dropped = rmat_shape_queries_device(rg, 5 if seed is None else seed, budget, log=log)
         t_gen = time.time() - t0
         log("%s: n=%d, %d undirected edges, %d queries (sizes %s), generated in %.1f s"
             % (name, g.n, g.adj // 2, len(qs), [q.n for q in qs], t_gen))
@@ -237,8 +317,12 @@ def main():
     import argparse
     import json
 
-    ap = argparse.ArgumentParser(description="Times a benchmark workload on the device")
-    ap.add_argument("workload", choices=["er10k_walk6", "chunglu1m_walk8_16", "chunglu1m_mix4_32", "rmat22_64m"])
+    ap = argparse.ArgumentParser(description="Times a benchmark workload on the device, or generates the "
+                                             "stored query sets")
+    ap.add_argument("workload", nargs="?", choices=["er10k_walk6", "chunglu1m_mix4_32", "rmat22_shapes"])
+    ap.add_argument("--generate", metavar="DIR", default=None,
+                    help="run the feasibility rule for the stored query sets (all, or --sets) into DIR")
+    ap.add_argument("--sets", default=",".join(SETS), help="comma-separated sets for --generate")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -248,6 +332,9 @@ def main():
                     help="time text parsing, the binary cache and the device CSR build instead")
     ap.add_argument("--no-text", action="store_true", help="with --loaders: skip the text format")
     args = ap.parse_args()
+    if args.generate:
+        generate(args.sets.split(","), args.generate, log=lambda s: print(s, flush=True))
+        return
     if args.loaders:
         out = loaders(args.workload, args.loaders, not args.no_text, log=lambda s: print(s, flush=True))
     else:

@@ -42,6 +42,7 @@ extern "C" {
 #define GPSM_EDISCONN (-7)    /* query not connected or without edges */
 #define GPSM_EFORMAT (-8)     /* text reader: malformed or unterminated input */
 #define GPSM_EBUDGET (-9)     /* the join exceeded gpsm_opts.work_budget (result incomplete) */
+#define GPSM_ESCRATCH (-10)   /* input larger than the fixed scratch of an allocation-free entry point */
 
 typedef struct gpsm_ctx gpsm_ctx;
 typedef struct gpsm_graph gpsm_graph;
@@ -240,6 +241,22 @@ int gpsm_candidate_edges(gpsm_ctx* ctx, const gpsm_graph* g, const gpsm_query* q
  * (match/Match.cu:504-508) and gpsm::scan::prefixSum (util/prefix_scan.cu:410-427):
  * out[i] = sum(in[0..i)), returns the total in *total. Host buffers. */
 int gpsm_scan_u32(gpsm_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* total);
+
+/* The scan of gpsm_scan_u32 on device buffers without allocating anything:
+ * no counterpart in the reference; for callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  d_in and d_out are device pointers of n u32 values
+ * (d_out may alias d_in), written by work the caller has completed; *total
+ * (host) receives the u64 sum.  The block sums live in a fixed device array
+ * of GPSM_SCAN_NOALLOC_SCRATCH_BYTES owned by the library (one per device,
+ * held until the process ends), which bounds n at GPSM_SCAN_NOALLOC_MAX
+ * (GPSM_ESCRATCH beyond it).  Runs on the context's stream and waits for it.
+ * Calls that overlap on one device share that array and race. */
+#define GPSM_SCAN_NOALLOC_SCRATCH_BYTES (256u * 1024u)
+/* 2 words hold the total, 2 words per block of 4096 values hold its sum:
+ * (65536 - 2) / 2 = 32767 blocks */
+#define GPSM_SCAN_NOALLOC_MAX (32767ull * 4096ull)
+int gpsm_scan_u32_noalloc(gpsm_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t n, uint64_t* total);
 
 /* kernel_collect (match/Match.cu:488-526) / gpsm::scan::compact
  * (util/compact_scan.cu:52-70) on a bitmap: ids of set bits, ascending. */

@@ -70,10 +70,11 @@ int gpsm_gen_labels(gpsm_csr* g, uint32_t L, double zipf_s, uint64_t seed);
  * first; the decision is an exact upper bound on the embedding count (the
  * label- and degree-preserving homomorphisms of the pattern's BFS spanning
  * tree) against node_budget.  A rejected query makes the slot draw again,
- * shrinking k by one after every 50 rejections, down to 3 vertices (below
- * kmin when kmin > 3); node_budget 0 disables the rule.  Slots are
- * independent streams (seed, slot), so the set is the same for any thread
- * count. */
+ * with a fresh size uniform in [kmin, kmax] (sizes are never shrunk, so a
+ * slot's query follows the size distribution conditioned on the rule);
+ * after 3200 rejected draws of one slot the call fails (GPSM_EINVAL).
+ * node_budget 0 disables the rule.  Slots are independent streams (seed,
+ * slot), so the set is the same for any thread count. */
 int gpsm_gen_queries(const gpsm_csr* g, uint32_t nq, uint32_t kmin, uint32_t kmax, int kind,
                      uint64_t seed, uint64_t node_budget, gpsm_qset* out);
 

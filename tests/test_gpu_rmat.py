@@ -15,7 +15,7 @@ import pyoracle as O
 from gpsm_amd import _lib
 from gpsm_amd import graphs as G
 from gpsm_amd import planner
-from gpsm_amd.workloads import rmat_shape_queries_device
+from gpsm_amd.workloads import feasible_queries
 
 pytestmark = pytest.mark.gpu
 
@@ -24,7 +24,11 @@ def test_rmat_shapes_match_o2(ctx):
     g = G.rmat(16, 1_000_000, 0.57, 0.19, 0.19, 5)
     G.set_labels(g, 32, 0.8, 5)
     rg = ctx.upload(g)
-    qs, dropped = rmat_shape_queries_device(rg, seed=5, work_budget=2 * 10 ** 6, per_kind=4)
+    qs, dropped = [], 0
+    for kind, kmin, kmax, seed in ((_lib.Q_CYCLE, 4, 8, 5), (_lib.Q_CLIQUE, 4, 8, 6), (_lib.Q_WALK_INDUCED, 8, 50, 7)):
+        q, d, _ = feasible_queries(rg, 4, kmin, kmax, [kind], seed, 2 * 10 ** 6, per_round=6, rounds=3)
+        qs += q
+        dropped += d
     kinds = sorted({q.m > q.n - 1 for q in qs})
     print("%d queries (sizes %s), %d slots dropped" % (len(qs), [q.n for q in qs], dropped))
     assert len(qs) >= 6 and kinds == [True]  # cycles, cliques and induced walks all have cycles
