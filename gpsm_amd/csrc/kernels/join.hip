@@ -31,6 +31,11 @@ namespace {
 constexpr uint32_t WROWS = 256;  // rows per window = threads per workgroup
 static_assert((WROWS & (WROWS - 1)) == 0, "row indices are masked with WROWS - 1");
 constexpr int EXP_UNROLL = 4;    // candidates in flight per thread
+// At the last level a row with at least this many candidates is expanded by
+// one wave on its own (no row search per candidate, the row's values
+// uniform); shorter rows share lanes through the flattened expansion.
+constexpr uint32_t LONG_ROW = 128;
+static_assert((uint64_t)WROWS * LONG_ROW < (1u << 23) && WROWS < 512, "packed scan of short spans and long flags");
 
 __device__ __forceinline__ uint32_t rank_of(const uint32_t* __restrict__ bm, const uint32_t* __restrict__ bp,
                                             uint32_t x) {
@@ -85,6 +90,10 @@ struct JoinShared {
   unsigned long long lwork[MAXQ];
   uint32_t task, k, abort;
   unsigned long long red[4];
+  // last level: the window's long rows (row, list start, candidates) and
+  // the candidates they hold
+  uint32_t lrow[WROWS], lst[WROWS], lub[WROWS];
+  uint32_t long_cands;
   // followed by the row window: val[c * WROWS + r], c < qmax
 };
 
@@ -215,11 +224,30 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) j
             st += S.cur_cand[k];
           }
         }
-        uint32_t T;
-        const uint32_t ex = block_excl_scan(ub, S.tmp, &T);
-        S.pre[tid] = tid < nw ? ex : T;
-        S.delta[tid] = st - ex;  // wraps; delta + g is in range for g in the row's span
-        if (tid == 0 && !final) S.fill[nk] = 0;
+        uint32_t T, nlong = 0;
+        if (!final) {
+          const uint32_t ex = block_excl_scan(ub, S.tmp, &T);
+          S.pre[tid] = tid < nw ? ex : T;
+          S.delta[tid] = st - ex;  // wraps; delta + g is in range for g in the row's span
+          if (tid == 0) S.fill[nk] = 0;
+        } else {
+          // last level: long rows are listed for the wave-per-row pass, the
+          // others get spans of the flattened pass (T = their candidates);
+          // one scan of (short candidates << 9 | long flag)
+          const bool lng = tid < nw && ub >= LONG_ROW;
+          uint32_t tot;
+          const uint32_t ex2 = block_excl_scan(((lng ? 0u : ub) << 9) | (lng ? 1u : 0u), S.tmp, &tot);
+          T = tot >> 9;
+          nlong = tot & 511u;
+          S.pre[tid] = tid < nw ? ex2 >> 9 : T;
+          S.delta[tid] = st - (ex2 >> 9);
+          if (lng) {
+            S.lrow[ex2 & 511u] = tid;
+            S.lst[ex2 & 511u] = st;
+            S.lub[ex2 & 511u] = ub;
+          }
+          if (tid == 0) S.long_cands = 0;
+        }
         __syncthreads();
 
         // expand: EXP_UNROLL candidates per thread per step, list loads in
@@ -355,13 +383,68 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) j
           }
           return f;
         };
+        if (final && nlong) {
+          // wave-per-row pass over the window's long rows: lanes take
+          // consecutive list entries of one row, whose values are uniform
+          const unsigned long long* trm_n = a.terms + dn.term_base;
+          for (uint32_t j = tid / WAVE; j < nlong; j += WROWS / WAVE) {
+            const uint32_t i = uniform(S.lrow[j]);
+            const uint32_t st_r = uniform(S.lst[j]), n_r = uniform(S.lub[j]);
+            const unsigned long long srow = S.srow[i];
+            if (lane == 0) atomicAdd(&S.long_cands, n_r);
+            for (uint32_t g0 = 0; g0 < n_r; g0 += WAVE * EXP_UNROLL) {
+              uint32_t mvs[EXP_UNROLL];
+              unsigned long long trm[EXP_UNROLL];
+              bool oks[EXP_UNROLL];
+#pragma unroll
+              for (int u = 0; u < EXP_UNROLL; ++u) {
+                const uint32_t g = g0 + u * WAVE + lane;
+                oks[u] = g < n_r;
+                mvs[u] = 0;
+                trm[u] = 0;
+                if (oks[u]) {
+                  mvs[u] = lst[st_r + g];
+                  trm[u] = trm_n[st_r + g];
+                }
+              }
+              for (uint64_t m = inj; m; m &= m - 1) {
+                const uint32_t vc = val[(uint32_t)(__ffsll((long long)m) - 1) * WROWS + i];
+#pragma unroll
+                for (int u = 0; u < EXP_UNROLL; ++u) oks[u] &= mvs[u] != vc;
+              }
+#pragma unroll
+              for (int u = 0; u < EXP_UNROLL; ++u) {
+                bool ok = oks[u];
+                if (ok) {
+                  if (BYTES) my_bytes += 4;  // the list entry
+                  for (uint32_t b = bo; b < be && ok; ++b)
+                    ok = has_edge<BYTES>(a.row, a.col, a.hub_of, a.hub_bits, a.W, mvs[u], val[P.backs[b] * WROWS + i],
+                                         my_bytes);
+                }
+                wave_cnt += __popcll(__ballot(ok));
+                if (ok) {
+                  my_hash += fmix64(srow + trm[u]);
+                  if (a.emit) {
+                    const unsigned long long pos = atomicAdd(a.emit_ctr, 1ULL);
+                    if (pos < a.emit_cap) {
+                      uint32_t* t = a.emit + pos * q;
+                      for (uint32_t c = 0; c <= k; ++c) t[P.order[c]] = val[c * WROWS + i];
+                      t[wn] = mvs[u];
+                    }
+                  }
+                }
+              }
+            }
+          }
+        }
         const uint32_t f = final ? expand(std::true_type{}) : expand(std::false_type{});
+        if (final) __syncthreads();  // long_cands complete
         if (!final && tid == 0) S.lrows[nk] += S.fill[nk];
         // advance level k's cursor past what was expanded
         if (tid == 0) {
           // join work: candidates examined (a window cut short by a full
           // buffer is reloaded later from the cursor), and the budget on it
-          const unsigned long long done = min(f, T);
+          const unsigned long long done = (unsigned long long)min(f, T) + (final ? S.long_cands : 0u);
           const unsigned long long before = atomicAdd(&a.out[task.query].work, done);
           S.lwork[nk] += done;
           if (a.budget && before + done > a.budget) {
