@@ -551,9 +551,9 @@ int gpsm_io_write_end(const char* path) {
 }
 
 uint64_t gpsm_tuple_hash(const uint32_t* t, uint32_t q) {
-  uint64_t s = 0;
-  for (uint32_t j = 0; j < q; ++j) s += gpsm::tuple_term(j, t[j]);
-  return gpsm::fmix64(s);
+  uint64_t h = 1;
+  for (uint32_t j = 0; j < q; ++j) h *= gpsm::tuple_term(j, t[j]);
+  return h;
 }
 
 }  // extern "C"

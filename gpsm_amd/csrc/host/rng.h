@@ -17,7 +17,7 @@ inline uint64_t fmix64(uint64_t k) {
 
 // Per-vertex term of the tuple hash: a bijection of (query id, data id).
 inline uint64_t tuple_term(uint32_t qid, uint32_t v) {
-  return fmix64((((uint64_t)qid << 32) | v) ^ 0x9E3779B97F4A7C15ULL);
+  return fmix64((((uint64_t)qid << 32) | v) ^ 0x9E3779B97F4A7C15ULL) | 1ULL;
 }
 
 struct SplitMix64 {

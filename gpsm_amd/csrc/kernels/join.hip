@@ -17,7 +17,11 @@
 //  * join-mode edges become edge-membership checks (binary search in the
 //    shorter adjacency) and injectivity is checked in the same pass;
 //  * the last level is never written: each embedding adds its tuple hash
-//    (additive over vertices, one fmix64 per emit) and 1 to the counters.
+//    and 1 to the counters.  The tuple hash is a product of one factor per
+//    (query vertex, image) pair, so a row carries the product of its
+//    factors and an embedding costs one multiply by its last factor
+//    (precomputed per list entry); a wave expanding one long row sums the
+//    factors of its valid candidates and multiplies once per row.
 #include <algorithm>
 #include <type_traits>
 
@@ -83,7 +87,7 @@ __device__ __forceinline__ bool has_edge(const uint32_t* __restrict__ row, const
 struct JoinShared {
   uint32_t pre[WROWS + 1];
   uint32_t delta[WROWS];  // list position of a row's candidate g is delta[row] + g
-  unsigned long long srow[WROWS];
+  unsigned long long srow[WROWS];  // product of a row's hash factors
   uint32_t tmp[8];
   uint32_t fill[MAXQ], cur_row[MAXQ], cur_cand[MAXQ];
   unsigned long long lrows[MAXQ];
@@ -161,7 +165,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) j
         const uint64_t e = task.first + i * task.stride;
         const uint32_t j = seg_search(off0, d0.ncand + 1, (uint32_t)e);
         const uint32_t mu = cl0[j], mv = ls0[e];
-        my_hash += fmix64(tuple_term(P.order[0], mu) + tuple_term(P.order[1], mv));
+        my_hash += tuple_term(P.order[0], mu) * tuple_term(P.order[1], mv);
         ++my_cnt;
         if (a.emit) {
           const unsigned long long pos = atomicAdd(a.emit_ctr, 1ULL);
@@ -211,8 +215,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) j
           } else {
             for (uint32_t c = 0; c <= k; ++c) val[c * WROWS + tid] = Lk[(uint64_t)c * cap + r];
           }
-          unsigned long long s = 0;
-          for (uint32_t c = 0; c <= k; ++c) s += tuple_term(P.order[c], val[c * WROWS + tid]);
+          unsigned long long s = 1;
+          for (uint32_t c = 0; c <= k; ++c) s *= tuple_term(P.order[c], val[c * WROWS + tid]);
           S.srow[tid] = s;
           // row values, rank index (prefix + words) and the offset pair
           if (BYTES) my_bytes += (k == 1 ? 8 : 4 * (k + 1)) + 4 + 4 * (((val[p * WROWS + tid] >> 5) & 7) + 1) + 8;
@@ -367,7 +371,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) j
                 }
                 wave_cnt += __popcll(__ballot(ok));
                 if (ok) {
-                  my_hash += fmix64(S.srow[i] + trm[u]);
+                  my_hash += S.srow[i] * trm[u];
                   if (a.emit) {
                     const unsigned long long pos = atomicAdd(a.emit_ctr, 1ULL);
                     if (pos < a.emit_cap) {
@@ -391,6 +395,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) j
             const uint32_t i = uniform(S.lrow[j]);
             const uint32_t st_r = uniform(S.lst[j]), n_r = uniform(S.lub[j]);
             const unsigned long long srow = S.srow[i];
+            unsigned long long rsum = 0;  // this lane's factors of the row's valid candidates
             if (lane == 0) atomicAdd(&S.long_cands, n_r);
             for (uint32_t g0 = 0; g0 < n_r; g0 += WAVE * EXP_UNROLL) {
               uint32_t mvs[EXP_UNROLL];
@@ -423,7 +428,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) j
                 }
                 wave_cnt += __popcll(__ballot(ok));
                 if (ok) {
-                  my_hash += fmix64(srow + trm[u]);
+                  rsum += trm[u];
                   if (a.emit) {
                     const unsigned long long pos = atomicAdd(a.emit_ctr, 1ULL);
                     if (pos < a.emit_cap) {
@@ -435,6 +440,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) j
                 }
               }
             }
+            my_hash += srow * rsum;
           }
         }
         const uint32_t f = final ? expand(std::true_type{}) : expand(std::false_type{});

@@ -94,8 +94,11 @@ __device__ __forceinline__ uint64_t fmix64(uint64_t k) {
   return k;
 }
 
+// One factor of a tuple's hash (odd): the tuple hash is the product of the
+// factors of its (query vertex, image) pairs mod 2^64, so it does not depend
+// on the order in which the join binds the vertices.
 __device__ __forceinline__ uint64_t tuple_term(uint32_t qid, uint32_t v) {
-  return fmix64((((uint64_t)qid << 32) | v) ^ 0x9E3779B97F4A7C15ULL);
+  return fmix64((((uint64_t)qid << 32) | v) ^ 0x9E3779B97F4A7C15ULL) | 1ULL;
 }
 
 }  // namespace gpsm

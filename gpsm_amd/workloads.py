@@ -178,6 +178,33 @@ def generate(names, data_dir=DATA_DIR, log=print, work_budget=WORK_BUDGET):
             rg.free()
 
 
+def refresh(names, data_dir=DATA_DIR, out_dir=None, log=print):
+    """Recomputes the stored sets' per-query count, hash and join work on
+    the device (heuristic order) without running the rule again, e.g. after
+    a change of the hash definition; the rule's record is kept."""
+    from .matcher import Context
+
+    out_dir = out_dir or data_dir
+    os.makedirs(out_dir, exist_ok=True)
+    with Context(0) as ctx:
+        by_graph = {}
+        for name in names:
+            by_graph.setdefault(SETS[name][0], []).append(name)
+        for gname, group in by_graph.items():
+            g = GRAPHS[gname]()
+            rg = ctx.upload(g)
+            for name in group:
+                qs, rec = load_set(name, data_dir)
+                res = rg.match_batch(qs) if qs else []
+                for q, r in zip(rec["queries"], res):
+                    assert r.status == 0
+                    q["count"], q["hash"], q["work"] = r.count, "%016x" % r.hash, r.join_work
+                with open(set_path(name, out_dir), "w") as f:
+                    json.dump(rec, f, separators=(",", ":"))
+                log("%s: %d queries refreshed" % (name, len(qs)))
+            rg.free()
+
+
 def build(name, seed=None, budget=EMBEDDING_BUDGET):
     """(graph, queries) of a named workload; stored sets are read from
     gpsm_amd/data (queries None for a graph without a stored set)."""
@@ -322,7 +349,9 @@ def main():
     ap.add_argument("workload", nargs="?", choices=["er10k_walk6", "chunglu1m_mix4_32", "rmat22_shapes"])
     ap.add_argument("--generate", metavar="DIR", default=None,
                     help="run the feasibility rule for the stored query sets (all, or --sets) into DIR")
-    ap.add_argument("--sets", default=",".join(SETS), help="comma-separated sets for --generate")
+    ap.add_argument("--refresh", metavar="DIR", default=None,
+                    help="recompute the stored sets' counts, hashes and join work into DIR")
+    ap.add_argument("--sets", default=",".join(SETS), help="comma-separated sets for --generate / --refresh")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -334,6 +363,9 @@ def main():
     args = ap.parse_args()
     if args.generate:
         generate(args.sets.split(","), args.generate, log=lambda s: print(s, flush=True))
+        return
+    if args.refresh:
+        refresh(args.sets.split(","), out_dir=args.refresh, log=lambda s: print(s, flush=True))
         return
     if args.loaders:
         out = loaders(args.workload, args.loaders, not args.no_text, log=lambda s: print(s, flush=True))

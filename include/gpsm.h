@@ -340,10 +340,12 @@ int gpsm_graph_cache_stats(const gpsm_ctx* ctx, uint64_t* instantiations, uint64
 
 /* --- set hash ------------------------------------------------------------- */
 
-/* Hash of one tuple t[0..q) in query-id order:
- * fmix64( sum_j fmix64(((uint64)j << 32 | t[j]) ^ 0x9E3779B97F4A7C15) ),
- * fmix64 = MurmurHash3's 64-bit finaliser.  The set hash is the sum of tuple
- * hashes mod 2^64, so it is independent of enumeration order. */
+/* Hash of one tuple t[0..q) in query-id order: the product mod 2^64 over j of
+ * ( fmix64(((uint64)j << 32 | t[j]) ^ 0x9E3779B97F4A7C15) | 1 ),
+ * fmix64 = MurmurHash3's 64-bit finaliser (every factor is odd, so the
+ * product is an odd 64-bit value).  The set hash is the sum of tuple hashes
+ * mod 2^64, so it is independent of enumeration order; the product is
+ * independent of the order in which a join binds the vertices. */
 uint64_t gpsm_tuple_hash(const uint32_t* t, uint32_t q);
 
 #ifdef __cplusplus

@@ -23,7 +23,9 @@ uint64_t fmix64(uint64_t k) {
   return k;
 }
 
-uint64_t term(uint32_t j, uint32_t v) { return fmix64((((uint64_t)j << 32) | v) ^ 0x9E3779B97F4A7C15ULL); }
+// Hash of one tuple: the product mod 2^64 of one odd 64-bit mix per
+// (query vertex, image) pair; the set hash is the sum of tuple hashes.
+uint64_t term(uint32_t j, uint32_t v) { return fmix64((((uint64_t)j << 32) | v) ^ 0x9E3779B97F4A7C15ULL) | 1ULL; }
 
 // ---------------------------------------------------------------------------
 // Query in the reference's representation: per-vertex in/out lists in file
@@ -297,9 +299,9 @@ struct Joiner {
       std::sort(s.begin(), s.end());
       if (std::adjacent_find(s.begin(), s.end()) != s.end()) return;
     }
-    uint64_t s = 0;
-    for (uint32_t j = 0; j < q; ++j) s += term(j, t[j]);
-    hash += fmix64(s);
+    uint64_t h = 1;
+    for (uint32_t j = 0; j < q; ++j) h *= term(j, t[j]);
+    hash += h;
     ++count;
     if (tuples && n_tuples < cap) tuples->insert(tuples->end(), t.begin(), t.end());
     ++n_tuples;
@@ -342,9 +344,9 @@ struct Joiner {
 extern "C" {
 
 uint64_t orc_tuple_hash(const uint32_t* t, uint32_t q) {
-  uint64_t s = 0;
-  for (uint32_t j = 0; j < q; ++j) s += term(j, t[j]);
-  return fmix64(s);
+  uint64_t h = 1;
+  for (uint32_t j = 0; j < q; ++j) h *= term(j, t[j]);
+  return h;
 }
 
 int orc_o2_filter(const orc_graph* g, const orc_query* q, uint8_t* init, uint8_t* refined,
@@ -649,8 +651,7 @@ int orc_o1_enumerate(const orc_graph* g, const orc_query* q, int threads, uint64
       if (g->labels[r] != lab0) continue;
       img[0] = (uint32_t)r;
       if (k == 1) {
-        uint64_t s = term(ord[0], img[0]);
-        hs += fmix64(s);
+        hs += term(ord[0], img[0]);
         ++tot;
         continue;
       }
@@ -677,9 +678,9 @@ int orc_o1_enumerate(const orc_graph* g, const orc_query* q, int threads, uint64
         if (!ok) continue;
         img[d] = c;
         if (d == (int)k - 1) {
-          uint64_t s = 0;
-          for (uint32_t p = 0; p < k; ++p) s += term(ord[p], img[p]);
-          hs += fmix64(s);
+          uint64_t h = 1;
+          for (uint32_t p = 0; p < k; ++p) h *= term(ord[p], img[p]);
+          hs += h;
           ++tot;
           if (tuples) {
             size_t base = mine.size();

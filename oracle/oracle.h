@@ -107,7 +107,8 @@ int64_t orc_o2_cet(const orc_graph* g, const orc_query* q, uint32_t* buf, uint64
 int orc_o1_enumerate(const orc_graph* g, const orc_query* q, int threads, uint64_t* count,
                      uint64_t* hash, uint32_t* tuples, uint64_t cap, uint64_t* n_tuples);
 
-/* Set-hash definition: fmix64(sum_j fmix64((j<<32|t[j]) ^ 0x9E3779B97F4A7C15)). */
+/* Tuple hash: prod_j (fmix64((j<<32|t[j]) ^ 0x9E3779B97F4A7C15) | 1) mod 2^64;
+ * the set hash is the sum of tuple hashes mod 2^64. */
 uint64_t orc_tuple_hash(const uint32_t* t, uint32_t q);
 
 /* Exclusive scan (thrust::exclusive_scan semantics); returns the total. */
