@@ -6,11 +6,13 @@ level with b join-mode (back) edges costs b edge-membership searches more,
 and every materialised row costs a window load (its columns, a rank-index
 lookup and an offset pair).  This script measures the weights on the device.
 
-Batches of the benchmark's walk queries on the 1M-vertex Chung-Lu graph are
-matched with explicit orders, every query taking one of its candidate orders
-(the heuristic order and the greedy continuations the planner ranks) at
-random.  Per batch the join kernel's time (HIP events, best of 3) and, from
-the device's per-level counters,
+Batches of cycles and cliques of 4-6 vertices (every vertex after the first
+edge has back edges) on an R-MAT graph of 2^18 vertices (a, b, c = .57,
+.19, .19, average degree 32, 32 Zipf(0.8) labels), kept by the device-exact
+feasibility rule at a work budget of 5e8, are matched with explicit orders,
+every query taking one of the orders the planner ranks (the heuristic order
+and candidate_list's) at random.  Per batch the join kernel's time (HIP
+events, best of 3) and, from the device's per-level counters,
 
     W1  first-table rows
     W   candidates examined at levels >= 2
@@ -36,7 +38,8 @@ sys.path.insert(0, os.path.dirname(HERE))
 from gpsm_amd import _lib  # noqa: E402
 from gpsm_amd import graphs as G  # noqa: E402
 from gpsm_amd.matcher import Context  # noqa: E402
-from gpsm_amd.train_planner import continuations  # noqa: E402
+from gpsm_amd.train_planner import candidate_list  # noqa: E402
+from gpsm_amd.workloads import feasible_queries  # noqa: E402
 
 
 def backs_per_level(q, order, ef, et, size):
@@ -76,9 +79,9 @@ def nnls(A, y):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--batches", type=int, default=40)
-    ap.add_argument("--queries", type=int, default=100)
-    ap.add_argument("--budget", type=float, default=1e8)
+    ap.add_argument("--batches", type=int, default=24)
+    ap.add_argument("--queries", type=int, default=32)
+    ap.add_argument("--budget", type=float, default=5e8, help="work budget of the feasibility rule")
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
@@ -87,16 +90,20 @@ def main():
     def log(s):
         print("[%6.1fs] %s" % (time.time() - t0, s), flush=True)
 
-    g = G.chunglu_1m()
-    qs = G.queries(g, args.queries, 8, 16, _lib.Q_WALK_HALF, 2, int(args.budget))
+    g = G.rmat(18, 16 << 18, 0.57, 0.19, 0.19, 31)
+    G.set_labels(g, 32, 0.8, 31)
     ctx = Context(0)
     rg = ctx.upload(g)
+    qs = []
+    for kind, seed in ((_lib.Q_CYCLE, 41), (_lib.Q_CLIQUE, 42)):
+        q, _, _ = feasible_queries(rg, args.queries // 2, 4, 6, [kind], seed, int(args.budget), per_round=6, rounds=3)
+        qs += q
     base = rg.match_batch(qs)
     tabs = [rg.cet_edges(q) for q in qs]
     cands = []
     for q, r, (ef, et, size) in zip(qs, base, tabs):
         c = [list(r.order)]
-        c += [od for od in continuations(q, ef, et, size) if od not in c]
+        c += [od for od in candidate_list(q, ef, et, size) if od not in c]
         cands.append(c)
     log("%d queries, %.1f candidate orders each" % (len(qs), np.mean([len(c) for c in cands])))
     rng = np.random.default_rng(args.seed)

@@ -505,6 +505,41 @@ void continue_order(const HostQuery& q, const std::vector<uint64_t>& cs, bool co
 
 constexpr uint32_t kFirst = 4;
 
+// Every vertex order whose prefixes are connected, in lexicographic order
+// (depth-first, smallest vertex first); false (out incomplete) once there
+// are more than cap.
+bool all_connected_orders(const HostQuery& q, size_t cap, std::vector<std::vector<uint32_t>>& out) {
+  std::vector<std::vector<uint8_t>> adj(q.n, std::vector<uint8_t>(q.n, 0));
+  for (size_t i = 0; i < q.ef.size(); ++i) adj[q.ef[i]][q.et[i]] = adj[q.et[i]][q.ef[i]] = 1;
+  std::vector<uint32_t> od;
+  std::vector<uint8_t> placed(q.n, 0);
+  bool ok = true;
+  auto rec = [&](auto&& self) -> void {
+    if (!ok) return;
+    if (od.size() == q.n) {
+      if (out.size() == cap) {
+        ok = false;
+        return;
+      }
+      out.push_back(od);
+      return;
+    }
+    for (uint32_t w = 0; w < q.n && ok; ++w) {
+      if (placed[w]) continue;
+      bool touches = od.empty();
+      for (uint32_t v : od) touches = touches || adj[w][v];
+      if (!touches) continue;
+      placed[w] = 1;
+      od.push_back(w);
+      self(self);
+      od.pop_back();
+      placed[w] = 0;
+    }
+  };
+  rec(rec);
+  return ok;
+}
+
 }  // namespace
 
 void candidate_orders(const HostQuery& q, const std::vector<uint64_t>& cet_size, uint64_t big,
@@ -513,6 +548,13 @@ void candidate_orders(const HostQuery& q, const std::vector<uint64_t>& cet_size,
   OrderPlan h;
   heuristic_order(q, cet_size, big, h);
   out.push_back(h.order);
+  // a small query: every connected order is a candidate
+  std::vector<std::vector<uint32_t>> all;
+  if (all_connected_orders(q, kAllOrders, all)) {
+    for (auto& od : all)
+      if (od != h.order) out.push_back(std::move(od));
+    return;
+  }
   const size_t E = q.ef.size();
   std::vector<uint32_t> edges(E);
   for (uint32_t i = 0; i < E; ++i) edges[i] = i;

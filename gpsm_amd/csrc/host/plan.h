@@ -114,10 +114,13 @@ void learned_consume(LearnedState& s, const float* y);
 bool learned_finish(const LearnedState& s, OrderPlan& out);
 
 // Candidate vertex orders for the learned cost model to choose among:
-// the heuristic order first, then, from each of the kFirst smallest-|CET|
-// first edges in both orientations, a min-|CET| continuation and a
-// most-constrained continuation (most bound neighbours, then min |CET|).
-// Duplicates are dropped.
+// the heuristic order first, then, for a query with at most kAllOrders
+// connected vertex orders (every prefix connected), all of them in
+// lexicographic order; for a larger one, from each of the kFirst
+// smallest-|CET| first edges in both orientations, a min-|CET| continuation
+// and a most-constrained continuation (most bound neighbours, then min
+// |CET|).  Duplicates are dropped.
+constexpr size_t kAllOrders = 64;
 void candidate_orders(const HostQuery& q, const std::vector<uint64_t>& cet_size, uint64_t big,
                       std::vector<std::vector<uint32_t>>& out);
 

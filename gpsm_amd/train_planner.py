@@ -3,10 +3,14 @@
 Samples are cardinalities the matcher itself measures: seeded random-walk
 queries (4-24 vertices; half-induced, induced and tree walks) on Chung-Lu
 graphs of 100K, 300K and 1M vertices (gamma 2.1, average degree 16, 16
-Zipf(1.0) labels; other seeds than the benchmark graph) are matched with
-their heuristic order, with the greedy continuations the planner ranks and
-with random connected vertex orders; every join level of every run gives one
-(features, log1p(rows)) pair through gpsm_plan_features.  The network is
+Zipf(1.0) labels), and cycles, cliques and induced walks of 4-8 vertices on
+R-MAT graphs of 2^16 and 2^17 vertices (average degree 32, 32 Zipf(0.8)
+labels), all with other seeds than the benchmark graphs, are matched with
+their heuristic order, with every order the planner ranks (all connected
+orders of a small query, else the greedy continuations) and with random
+connected vertex orders; every join level of every completed run (within
+the work budget) gives one (features, log1p(rows)) pair through
+gpsm_plan_features.  The network is
 trained by the device's own BF16-MFMA forward/backward kernels with Adam
 (gpsm_mlp_train_step); the FP32 feature standardisation is saved beside the
 weights and applied inside the kernels.
@@ -47,6 +51,47 @@ def random_order(q, rng):
     return order
 
 
+ALL_ORDERS = 64  # kAllOrders in plan.h
+
+
+def connected_orders(n, ef, et, cap=ALL_ORDERS):
+    """Every vertex order with connected prefixes, lexicographic; None if
+    there are more than cap (plan.cpp all_connected_orders)."""
+    adj = [set() for _ in range(n)]
+    for a, b in zip(ef, et):
+        adj[int(a)].add(int(b))
+        adj[int(b)].add(int(a))
+    out, od, placed = [], [], [False] * n
+
+    def rec():
+        if len(od) == n:
+            if len(out) == cap:
+                return False
+            out.append(list(od))
+            return True
+        for w in range(n):
+            if placed[w] or (od and not any(v in adj[w] for v in od)):
+                continue
+            placed[w] = True
+            od.append(w)
+            ok = rec()
+            od.pop()
+            placed[w] = False
+            if not ok:
+                return False
+        return True
+
+    return out if rec() else None
+
+
+def candidate_list(q, ef, et, size):
+    """The orders the device planner ranks besides the heuristic one
+    (plan.cpp candidate_orders): all connected orders of a small query, else
+    the greedy continuations."""
+    allo = connected_orders(q.n, ef, et)
+    return allo if allo is not None else continuations(q, ef, et, size)
+
+
 def continuations(q, ef, et, size, first=4):
     """The candidate orders the device planner ranks (plan.cpp
     candidate_orders): from each of the `first` smallest-|CET| edges in both
@@ -75,29 +120,51 @@ def continuations(q, ef, et, size, first=4):
     return out
 
 
-# graph sizes and query kinds cycled over the training graphs
-SIZES = (100_000, 300_000, 1_000_000)
+# The training graphs, cycled over: Chung-Lu (gamma 2.1, average degree
+# 16, 16 Zipf(1.0) labels) of three sizes with walk queries, and R-MAT
+# (a, b, c = .57, .19, .19, average degree 32, 32 Zipf(0.8) labels) of two
+# scales with cycles, cliques and induced walks; other seeds than the
+# benchmark graphs.
+SPECS = (("chunglu", 100_000), ("rmat", 16), ("chunglu", 300_000), ("rmat", 17), ("chunglu", 1_000_000))
 KINDS = (_lib.Q_WALK_HALF, _lib.Q_WALK_HALF, _lib.Q_WALK_INDUCED, _lib.Q_WALK_TREE)
+RMAT_KINDS = ((_lib.Q_CYCLE, 4, 6), (_lib.Q_CLIQUE, 4, 6), (_lib.Q_WALK_INDUCED, 4, 8))
+
+
+def training_queries(gi, seed, queries_per_graph, budget, kmax):
+    """(graph, queries) of training graph gi."""
+    kind, size = SPECS[gi % len(SPECS)]
+    if kind == "chunglu":
+        g = G.chung_lu(size, 8 * size, 2.1, seed)
+        G.set_labels(g, 16, 1.0, seed)
+        return g, G.queries(g, queries_per_graph, 4, kmax, KINDS[(gi // len(SPECS)) % len(KINDS)], seed, budget)
+    g = G.rmat(size, 16 << size, 0.57, 0.19, 0.19, seed)
+    G.set_labels(g, 32, 0.8, seed)
+    qs = []
+    for j, (qk, lo, hi) in enumerate(RMAT_KINDS):
+        qs += G.queries(g, queries_per_graph // len(RMAT_KINDS), lo, hi, qk, seed + 7 * j, 0)
+    return g, qs
 
 
 def collect(ctx, target_rows, seeds, queries_per_graph, budget, orders_per_query, work_budget, log, kmax=24):
     X, Y = [], []
     rng = np.random.default_rng(12345)
     for gi, seed in enumerate(seeds):
-        n = SIZES[gi % len(SIZES)]
-        g = G.chung_lu(n, 8 * n, 2.1, seed)
-        G.set_labels(g, 16, 1.0, seed)
-        kind = KINDS[gi % len(KINDS)]
-        qs = G.queries(g, queries_per_graph, 4, kmax, kind, seed, budget)
+        g, qs = training_queries(gi, seed, queries_per_graph, budget, kmax)
+        n = g.n
+        kind = SPECS[gi % len(SPECS)][0]
         rg = ctx.upload(g)
-        base = rg.match_batch(qs)
+        # queries whose heuristic join exceeds the work budget are left out
+        base = rg.match_batch(qs, work_budget=work_budget)
+        keep = [i for i, r in enumerate(base) if r.status == 0]
+        qs = [qs[i] for i in keep]
+        base = [base[i] for i in keep]
         tabs = [rg.cet_edges(q) for q in qs]
         # every (query, order) pair is one entry of a batch
         pairs = [(i, list(r.order)) for i, r in enumerate(base)]
         for i, q in enumerate(qs):
             if q.m == 0:
                 continue
-            pairs += [(i, od) for od in continuations(q, *tabs[i]) if od != list(base[i].order)]
+            pairs += [(i, od) for od in candidate_list(q, *tabs[i]) if od != list(base[i].order)]
             pairs += [(i, random_order(q, rng)) for _ in range(orders_per_query)]
         used = 0
         for c0 in range(0, len(pairs), 1024):
@@ -117,8 +184,8 @@ def collect(ctx, target_rows, seeds, queries_per_graph, budget, orders_per_query
                 used += 1
         rg.free()
         rows = sum(len(y) for y in Y)
-        log("graph seed %d (n=%d, kind %d): %d queries, %d runs, %d rows so far"
-            % (seed, n, kind, len(qs), used, rows))
+        log("graph seed %d (%s, n=%d): %d queries, %d runs, %d rows so far"
+            % (seed, kind, n, len(qs), used, rows))
         if rows >= target_rows:
             break
     return np.concatenate(X), np.concatenate(Y)

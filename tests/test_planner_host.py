@@ -12,7 +12,7 @@ import types
 import numpy as np
 import pytest
 
-from gpsm_amd.train_planner import continuations
+from gpsm_amd.train_planner import candidate_list
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 MARGIN = 0.9  # kLearnedMargin in plan.h
@@ -125,7 +125,7 @@ def test_plan_choice(driver, seed, mode):
     # ... and exactly the continuations the training data covers
     ef = np.array([a for a, _, _ in E]), np.array([b for _, b, _ in E]), np.array([s for _, _, s in E])
     mirror = [out["heur"]]
-    for od in continuations(types.SimpleNamespace(n=n), *ef):
+    for od in candidate_list(types.SimpleNamespace(n=n), *ef):
         if od not in mirror:
             mirror.append(od)
     assert mirror == cands
