@@ -275,6 +275,7 @@ int gpsm_init(int device, gpsm_ctx** out) {
     else
       std::snprintf(c->name, sizeof(c->name), "%s", prop.gcnArchName);
     c->ncu = prop.multiProcessorCount;
+    c->hub_max_bytes = std::min<uint64_t>(48ull << 30, (uint64_t)prop.totalGlobalMem / 4);
   } else {
     c->ncu = 256;
   }

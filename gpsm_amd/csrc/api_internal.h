@@ -180,8 +180,10 @@ struct gpsm_ctx {
   uint32_t* noalloc_scratch = nullptr;
   uint64_t graph_instantiations = 0, graph_updates = 0;
   // hub adjacency bitmaps of later uploads: vertices of degree > the hub
-  // degree, starting at hub_min_degree and doubled until they fit hub_max_bytes
-  uint32_t hub_min_degree = 512;
+  // degree, starting at hub_min_degree and doubled until they fit
+  // hub_max_bytes (gpsm_init: 48 GiB or a quarter of the device's memory,
+  // whichever is less)
+  uint32_t hub_min_degree = 64;
   uint64_t hub_max_bytes = 8ull << 30;
 };
 

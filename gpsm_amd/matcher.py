@@ -138,7 +138,7 @@ class Context:
         return {arr[i].name.decode(): {"launches": int(arr[i].launches), "ms": float(arr[i].total_ms),
                                        "bytes": float(arr[i].bytes)} for i in range(n.value)}
 
-    def set_hub_policy(self, min_degree=512, max_bytes=8 << 30):
+    def set_hub_policy(self, min_degree=64, max_bytes=48 << 30):
         """Hub adjacency bitmaps for graphs uploaded afterwards (vertices of
         degree > D, D doubled from min_degree until they fit max_bytes)."""
         check(_lib.lib().gpsm_set_hub_policy(self.h, min_degree, max_bytes), "gpsm_set_hub_policy")

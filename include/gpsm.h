@@ -133,8 +133,9 @@ uint64_t gpsm_graph_device_bytes(const gpsm_graph* g);
  * in the reference, whose join scans candidate lists linearly): every vertex
  * of degree > D gets an n-bit adjacency bitmap in device memory, so a
  * join-mode edge test touching it is one bit instead of a binary search.
- * D starts at min_degree (default 512) and doubles until the bitmaps fit
- * max_bytes (default 8 GiB; 0 = no hubs).  Results do not depend on it. */
+ * D starts at min_degree (default 64) and doubles until the bitmaps fit
+ * max_bytes (default: 48 GiB or a quarter of the device's memory, whichever
+ * is less; 0 = no hubs).  Results do not depend on it. */
 int gpsm_set_hub_policy(gpsm_ctx* ctx, uint32_t min_degree, uint64_t max_bytes);
 /* The hubs of an uploaded graph: their number and the degree D above which
  * a vertex is one (0 when there are none). */
