@@ -885,7 +885,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   // ---------------- host: join orders ----------------
   std::vector<CetDesc> jdesc;
   std::vector<JoinPlan> plans;
-  std::vector<JoinTask> tasks;
+  std::vector<JoinTask> tasks, tasks_cyc;
   std::vector<uint32_t> plan_of_q(nq, ~0u);
   uint64_t off_words = fwd_words, list_words = 0, term_words = 0;  // reversed tables' regions follow the forward ones
   std::vector<uint32_t> recount;  // tables of reversed plan edges (count again)
@@ -1032,8 +1032,8 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   // the grid is idle.  The persistent grid holds at most as many workgroups
   // as are resident at once: a workgroup beyond that would start only once
   // the task counter has run out.
-  const uint32_t resident = join_resident_per_cu(qmax);
-  uint32_t nwg = std::max<uint32_t>(1, c->ncu * std::min(wg_per_cu, resident));
+  const uint32_t nwg = std::max<uint32_t>(1, c->ncu * std::min(wg_per_cu, join_resident_per_cu(qmax, false)));
+  const uint32_t nwg_cyc = std::max<uint32_t>(1, c->ncu * std::min(wg_per_cu, join_resident_per_cu(qmax, true)));
   const uint64_t tpw = std::max<uint32_t>(1, c->tune.tasks_per_wg);
   uint64_t rows_total = 0;
   double est_total = 0;
@@ -1051,8 +1051,12 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     uint64_t T = (R1 + tsize - 1) / tsize;
     if (target > 0 && est[b] > target) T = std::max<uint64_t>(T, (uint64_t)std::min(est[b] / target + 1.0, 1e9));
     T = std::min<uint64_t>(std::max<uint64_t>(T, 1), R1);
-    for (uint64_t t = 0; t < T; ++t) tasks.push_back({plan_of_q[b], (uint32_t)T, t, (R1 - t + T - 1) / T});
+    // tree plans' tasks first, the cyclic plans' after them (two launches)
+    std::vector<JoinTask>& dst = plans[plan_of_q[b]].nback_total ? tasks_cyc : tasks;
+    for (uint64_t t = 0; t < T; ++t) dst.push_back({plan_of_q[b], (uint32_t)T, t, (R1 - t + T - 1) / T});
   }
+  const uint32_t ntree = (uint32_t)tasks.size(), ncyc = (uint32_t)tasks_cyc.size();
+  tasks.insert(tasks.end(), tasks_cyc.begin(), tasks_cyc.end());
 
   // ---------------- phase C: candidate-edge tables ----------------
   wall(K_WALL_PLAN);
@@ -1075,7 +1079,8 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   if ((rc = c->bufs[B_CTR].ensure(256))) return rc;
   // Level buffers of the workgroups launched, within kLevelBytes: rows per
   // level halve (to 2048) and then workgroups are dropped until they fit.
-  const uint32_t nlaunch = std::max<uint32_t>(1, std::min<uint32_t>(nwg, (uint32_t)tasks.size()));
+  const uint32_t nlaunch = std::max<uint32_t>(
+      1, std::max(std::min<uint32_t>(nwg, ntree), std::min<uint32_t>(nwg_cyc, ncyc)));
   uint32_t lcap = cap, lwg = nlaunch;
   while (lcap > 2048 && (double)join_level_words(qmax, lcap) * 4.0 * lwg > kLevelBytes) lcap /= 2;
   const uint64_t lw = join_level_words(qmax, lcap);
@@ -1137,8 +1142,19 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     }
     ja.budget = o.work_budget;
     ja.count_bytes = counters ? 1u : 0u;
+    // tree plans, then cyclic plans: one launch each, its own task counter;
+    // timed together as one join
     pa = pbegin(K_JOIN);
-    if (!tasks.empty()) launch_join(ja, lwg, 256, s);
+    if (ntree) {
+      ja.ntasks = ntree;
+      launch_join(ja, std::min(lwg, std::min(nwg, ntree)), false, s);
+    }
+    if (ncyc) {
+      ja.tasks = dptr<JoinTask>(c, B_TASKS) + ntree;
+      ja.ntasks = ncyc;
+      ja.task_counter = dptr<unsigned int>(c, B_CTR) + 1;
+      launch_join(ja, std::min(lwg, std::min(nwg_cyc, ncyc)), true, s);
+    }
     pend(K_JOIN, pa, 0.0, -1);
     if (captured && (rc = launch_captured(c, 1, s))) return rc;
     HIPCHK(hipGetLastError());

@@ -84,6 +84,40 @@ __device__ __forceinline__ bool has_edge(const uint32_t* __restrict__ row, const
   return *p == key;
 }
 
+// The back-edge tests of a lane's EXP_UNROLL candidates of one row (row i
+// of the window, the wave-per-row pass) against the bound vertices of
+// positions P.backs[bo..be): those vertices are the same for the whole
+// wave, so each one's hub index is read once, and when it is a hub the
+// candidates' tests are bits of its bitmap, loaded together (one load per
+// candidate and edge).  Otherwise the candidates' own hub indices are read
+// together and a pair with no hub falls back to has_edge's binary search.
+// A candidate whose test fails skips the later ones.  oks[u] in/out.
+template <bool BYTES>
+__device__ __forceinline__ void back_edges_of_row(const JoinArgs& a, const JoinPlan& P,
+                                                  const uint32_t* __restrict__ val, uint32_t bo, uint32_t be,
+                                                  uint32_t i, const uint32_t (&mvs)[EXP_UNROLL],
+                                                  bool (&oks)[EXP_UNROLL], unsigned long long& bytes) {
+  for (uint32_t b = bo; b < be; ++b) {
+    const uint32_t bv = uniform(val[P.backs[b] * WROWS + i]);
+    const uint32_t hb = uniform(a.hub_of[bv]);
+    if (hb != ~0u) {
+      const uint32_t* bm = a.hub_bits + (size_t)hb * a.W;
+      uint32_t w[EXP_UNROLL];
+#pragma unroll
+      for (int u = 0; u < EXP_UNROLL; ++u) w[u] = oks[u] ? bm[mvs[u] >> 5] >> (mvs[u] & 31) : 0u;
+#pragma unroll
+      for (int u = 0; u < EXP_UNROLL; ++u) {
+        if (BYTES && oks[u]) bytes += 12;  // two hub indices, one bitmap word
+        oks[u] = oks[u] && (w[u] & 1u);
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < EXP_UNROLL; ++u)
+        if (oks[u]) oks[u] = has_edge<BYTES>(a.row, a.col, a.hub_of, a.hub_bits, a.W, mvs[u], bv, bytes);
+    }
+  }
+}
+
 struct JoinShared {
   uint32_t pre[WROWS + 1];
   uint32_t delta[WROWS];  // list position of a row's candidate g is delta[row] + g
@@ -102,11 +136,15 @@ struct JoinShared {
 };
 
 // BYTES: account the algorithmic bytes read and written (JoinOut::bytes);
-// the kernel without it is the one timed.  Seven waves per SIMD: the timed
-// variant fits 72 VGPRs with two values spilled; on the benchmark that is
-// faster than six waves per SIMD with none (8.64-8.72 vs 8.75-8.78 ms).
-template <bool BYTES>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) join_kernel(JoinArgs a) {
+// the kernel without it is the one timed.  CYCLIC: the plans of the launch
+// have back edges (join-mode edges); a tree plan has none, and its kernel
+// holds no back-edge code.  The two are separate kernels with their own
+// register budgets (join_tree_kernel, join_cyclic_kernel below): the
+// cyclic one's batched back-edge tests need more registers, and sharing
+// them made the tree plans' join 1.2-1.6x slower (spills at seven waves
+// per SIMD, or fewer waves).
+template <bool BYTES, bool CYCLIC>
+__device__ __forceinline__ void join_body(const JoinArgs& a) {
   extern __shared__ unsigned char lds_raw[];
   JoinShared& S = *reinterpret_cast<JoinShared*>(lds_raw);
   uint32_t* val = reinterpret_cast<uint32_t*>(lds_raw + ((sizeof(JoinShared) + 15) & ~(size_t)15));
@@ -195,7 +233,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) j
         const uint32_t* bm_x = a.bitmaps + (size_t)dn.x_slot * a.W;
         const uint32_t* bp_x = a.bprefix + (size_t)dn.x_slot * a.NB;
         const uint32_t* offn = a.offsets + dn.off_base;
-        const uint32_t bo = P.back_off[nk], be = P.back_off[nk + 1];
+        const uint32_t bo = P.back_off[nk], be = CYCLIC ? P.back_off[nk + 1] : bo;  // a tree plan has none
         const uint64_t inj = P.inj[nk];
         const uint32_t wn = P.order[nk];
         const uint32_t cap = a.cap;
@@ -409,7 +447,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) j
                 trm[u] = 0;
                 if (oks[u]) {
                   mvs[u] = lst[st_r + g];
-                  trm[u] = trm_n[st_r + g];
+                  // without back edges the hash factors load beside the entries
+                  if (bo == be) trm[u] = trm_n[st_r + g];
                 }
               }
               for (uint64_t m = inj; m; m &= m - 1) {
@@ -418,14 +457,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) j
                 for (int u = 0; u < EXP_UNROLL; ++u) oks[u] &= mvs[u] != vc;
               }
 #pragma unroll
+              for (int u = 0; u < EXP_UNROLL; ++u)
+                if (BYTES && oks[u]) my_bytes += 4;  // the list entry
+              if (CYCLIC && bo != be) {
+                // with back edges: the tests first, then the factors of the
+                // candidates that passed them
+                back_edges_of_row<BYTES>(a, P, val, bo, be, i, mvs, oks, my_bytes);
+#pragma unroll
+                for (int u = 0; u < EXP_UNROLL; ++u)
+                  if (oks[u]) trm[u] = trm_n[st_r + g0 + u * WAVE + lane];
+              }
+#pragma unroll
               for (int u = 0; u < EXP_UNROLL; ++u) {
-                bool ok = oks[u];
-                if (ok) {
-                  if (BYTES) my_bytes += 4;  // the list entry
-                  for (uint32_t b = bo; b < be && ok; ++b)
-                    ok = has_edge<BYTES>(a.row, a.col, a.hub_of, a.hub_bits, a.W, mvs[u], val[P.backs[b] * WROWS + i],
-                                         my_bytes);
-                }
+                const bool ok = oks[u];
                 wave_cnt += __popcll(__ballot(ok));
                 if (ok) {
                   rsum += trm[u];
@@ -513,6 +557,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) j
   }
 }
 
+// Seven waves per SIMD for tree plans: 72 VGPRs (on the earlier benchmark
+// faster than six waves with no spill); five for cyclic plans, whose
+// batched back-edge tests spill heavily at seven (R-MAT cycles and cliques
+// 1.7x faster than with the per-candidate tests).
+template <bool BYTES>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) join_tree_kernel(JoinArgs a) {
+  join_body<BYTES, false>(a);
+}
+
+template <bool BYTES>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) join_cyclic_kernel(JoinArgs a) {
+  join_body<BYTES, true>(a);
+}
+
+template <bool BYTES>
+const void* join_fn(bool cyclic) {
+  return cyclic ? reinterpret_cast<const void*>(join_cyclic_kernel<BYTES>)
+                : reinterpret_cast<const void*>(join_tree_kernel<BYTES>);
+}
+
 }  // namespace
 
 uint64_t join_level_words(uint32_t qmax, uint32_t cap) {
@@ -528,33 +592,29 @@ size_t join_lds_bytes(uint32_t qmax) {
 }
 }  // namespace
 
-uint32_t join_resident_per_cu(uint32_t qmax) {
+uint32_t join_resident_per_cu(uint32_t qmax, bool cyclic) {
   const size_t lds = join_lds_bytes(std::max<uint32_t>(qmax, 2));
-  if (lds > 65536)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(join_kernel<false>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const void* fn = join_fn<false>(cyclic);
+  if (lds > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, join_kernel<false>, (int)WROWS, lds) != hipSuccess ||
-      per_cu < 1) {
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)WROWS, lds) != hipSuccess || per_cu < 1) {
     (void)hipGetLastError();
     per_cu = 1;
   }
   return (uint32_t)per_cu;
 }
 
-void launch_join(const JoinArgs& a, uint32_t nwg, uint32_t block, hipStream_t s) {
-  (void)block;
+void launch_join(const JoinArgs& a, uint32_t nwg, bool cyclic, hipStream_t s) {
   const size_t lds = join_lds_bytes(a.qmax);
-  if (lds > 65536) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(join_kernel<true>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(join_kernel<false>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const void* fn = a.count_bytes ? join_fn<true>(cyclic) : join_fn<false>(cyclic);
+  if (lds > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (a.count_bytes) {
+    if (cyclic) hipLaunchKernelGGL(join_cyclic_kernel<true>, dim3(nwg), dim3(WROWS), lds, s, a);
+    else hipLaunchKernelGGL(join_tree_kernel<true>, dim3(nwg), dim3(WROWS), lds, s, a);
+  } else {
+    if (cyclic) hipLaunchKernelGGL(join_cyclic_kernel<false>, dim3(nwg), dim3(WROWS), lds, s, a);
+    else hipLaunchKernelGGL(join_tree_kernel<false>, dim3(nwg), dim3(WROWS), lds, s, a);
   }
-  if (a.count_bytes)
-    hipLaunchKernelGGL(join_kernel<true>, dim3(nwg), dim3(WROWS), lds, s, a);
-  else
-    hipLaunchKernelGGL(join_kernel<false>, dim3(nwg), dim3(WROWS), lds, s, a);
 }
 
 }  // namespace gpsm

@@ -165,11 +165,14 @@ struct JoinArgs {
   // nonzero: account algorithmic bytes in JoinOut::bytes (a slower variant)
   uint32_t count_bytes;
 };
-void launch_join(const JoinArgs& a, uint32_t nwg, uint32_t block, hipStream_t s);
+// One persistent join launch over tasks [0, ntasks) of a.tasks; cyclic: the
+// plans of those tasks have back edges (join_cyclic_kernel), else none
+// (join_tree_kernel, which holds no back-edge tests).
+void launch_join(const JoinArgs& a, uint32_t nwg, bool cyclic, hipStream_t s);
 uint64_t join_level_words(uint32_t qmax, uint32_t cap);
-// Join workgroups (of the timed variant) that are resident on one CU at once
-// for a batch whose widest query has qmax vertices (LDS and registers); a
-// persistent grid gains nothing from more.
-uint32_t join_resident_per_cu(uint32_t qmax);
+// Join workgroups (of the timed variant of the tree or cyclic kernel) that
+// are resident on one CU at once for a batch whose widest query has qmax
+// vertices (LDS and registers); a persistent grid gains nothing from more.
+uint32_t join_resident_per_cu(uint32_t qmax, bool cyclic);
 
 }  // namespace gpsm
