@@ -50,6 +50,19 @@ def short(name):
     return m.group(1) + ("" if not m.group(2) else ("_counting" if m.group(3) == "true" else ""))
 
 
+def add_join_total(per_kernel, key, count_key):
+    """'join_kernel': the tree-plan and cyclic-plan launches of a batch
+    together (one of each per batch), per batch."""
+    parts = [per_kernel[k] for k in ("join_tree_kernel", "join_cyclic_kernel") if k in per_kernel]
+    if not parts:
+        return
+    batches = max(p[count_key] for p in parts)
+    tot = {count_key: batches}
+    for f in key:
+        tot[f] = round(sum(p[f] * p[count_key] for p in parts) / batches)
+    per_kernel["join_kernel"] = tot
+
+
 def find(d, suffix):
     for f in os.listdir(d):
         if f.endswith(suffix):
@@ -88,6 +101,8 @@ def main():
         wb = sum(w) / len(w) * 1024 * wscale
         traffic[short(name)] = {"dispatches": len(f), "fetch_bytes_per_dispatch": round(fb),
                                 "write_bytes_per_dispatch": round(wb), "hbm_bytes_per_dispatch": round(fb + wb)}
+    add_join_total(traffic, ("fetch_bytes_per_dispatch", "write_bytes_per_dispatch", "hbm_bytes_per_dispatch"),
+                   "dispatches")
     bl = bench_line(os.path.join(P, "trace.log"))
     with open(os.path.join(args.out, "traffic__%s.json" % args.tag), "w") as fh:
         json.dump({"about": "HBM bytes per dispatch from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of "
@@ -98,6 +113,15 @@ def main():
     # SQ counters of the join (timed variant)
     sq = counters(find(os.path.join(P, "sq"), "counter_collection.csv"))
     lim = {}
+    joined = collections.defaultdict(list)
+    for name, cs in sq.items():
+        if short(name) in ("join_tree_kernel", "join_cyclic_kernel"):
+            for c, v in cs.items():
+                joined[c].append(v)
+    if joined:
+        # a batch's two join launches together: per-batch sums of the counters
+        n = max(len(v) for v in joined["SQ_INSTS_VALU"])
+        sq["join_kernel"] = {c: [sum(sum(x) for x in vs) / n] * n for c, vs in joined.items()}
     for name, cs in sq.items():
         k = short(name)
         if k not in ("join_kernel", "filter_step_kernel", "cet_count_kernel", "cet_write_kernel"):
