@@ -8,21 +8,17 @@
 // (row, segment), so that a batch of a few hundred rows still fills 256 CUs:
 // a first pass popcounts every segment (and sums a row's segments into its
 // candidate count), the second gives a segment its row-relative base from
-// the counts of the segments before it and walks it in tiles of 256 words:
-// each thread popcounts one word, a wave64 shuffle scan plus a per-wave LDS
-// carry gives every word its output position, ids are staged in LDS and
-// written back coalesced.  The same pass writes the rank index (set bits
-// before each 256-bit block, row-relative) the join uses to find a vertex's
-// candidate-edge list.
+// the counts of the segments before it and compacts it in one pass (each
+// thread owns eight consecutive words, one block scan places them), ids
+// staged in LDS and written back coalesced.  The same pass writes the rank
+// index (set bits before each 256-bit block, row-relative) the join uses to
+// find a vertex's candidate-edge list.
 #include "launch.h"
 #include "wave.h"
 
 namespace gpsm {
 
 namespace {
-
-constexpr uint32_t TILE_WORDS = 256;  // one word per thread
-static_assert(SEG_WORDS % TILE_WORDS == 0 && TILE_WORDS % 8 == 0, "segments are whole tiles and rank blocks");
 
 __global__ void __launch_bounds__(256) seg_popcount_kernel(const uint32_t* __restrict__ bitmaps, uint32_t W,
                                                            uint32_t nseg, uint32_t* __restrict__ segc) {
@@ -51,13 +47,23 @@ __global__ void row_totals_kernel(const uint32_t* __restrict__ segc, uint32_t ns
   counts[r] = t;
 }
 
+// A workgroup compacts one (row, segment) in one pass: thread t owns the
+// eight consecutive words 8t..8t+7 of the segment (eight loads in flight),
+// one block scan of the threads' popcounts gives every thread its output
+// position and its 256-bit block's rank-index entry, and the ids go through
+// LDS when the segment's fit (written back coalesced), else straight from
+// the threads.
+constexpr uint32_t CW = 8;             // words per thread (one rank-index block)
+constexpr uint32_t STAGE_IDS = 8192;   // ids staged in LDS per segment
+static_assert(SEG_WORDS == 256 * CW, "a segment is one word block per thread");
+
 __global__ void __launch_bounds__(256) compact_rows_kernel(const uint32_t* __restrict__ bitmaps, uint32_t W,
                                                            uint32_t NB, uint32_t nseg,
                                                            const uint32_t* __restrict__ segc,
                                                            const uint64_t* __restrict__ coff,
                                                            uint32_t* __restrict__ clist,
                                                            uint32_t* __restrict__ bprefix) {
-  __shared__ uint32_t stage[TILE_WORDS * 32];
+  __shared__ uint32_t stage[STAGE_IDS];
   __shared__ uint32_t tmp[8];
   __shared__ uint32_t rel;
   const uint32_t r = blockIdx.x / nseg, sg = blockIdx.x % nseg;
@@ -70,27 +76,31 @@ __global__ void __launch_bounds__(256) compact_rows_kernel(const uint32_t* __res
     x = wave_sum(x);
     if (threadIdx.x == 0) rel = x;
   }
-  __syncthreads();
-  uint32_t carry = rel;
-  const uint32_t end = min(W, (sg + 1) * SEG_WORDS);
-  for (uint32_t t0 = sg * SEG_WORDS; t0 < end; t0 += TILE_WORDS) {
-    const uint32_t wi = t0 + threadIdx.x;
-    const uint32_t w = wi < end ? bm[wi] : 0u;
-    uint32_t total;
-    const uint32_t ex = block_excl_scan(__popc(w), tmp, &total);
-    if ((wi & 7) == 0 && wi < end) bp[wi >> 3] = carry + ex;
-    uint32_t pos = ex, x = w;
-    while (x) {
-      const uint32_t b = __ffs(x) - 1;
-      x &= x - 1;
-      stage[pos++] = wi * 32 + b;
-    }
+  const uint32_t w0 = sg * SEG_WORDS + threadIdx.x * CW;
+  uint32_t w[CW], c = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < CW; ++j) {
+    w[j] = w0 + j < W ? bm[w0 + j] : 0u;
+    c += __popc(w[j]);
+  }
+  uint32_t total;
+  const uint32_t ex = block_excl_scan(c, tmp, &total);  // its barriers also publish rel
+  const uint32_t carry = rel;
+  if (w0 < W) bp[w0 >> 3] = carry + ex;
+  if (total <= STAGE_IDS) {
+    uint32_t pos = ex;
+#pragma unroll
+    for (uint32_t j = 0; j < CW; ++j)
+      for (uint32_t x = w[j]; x; x &= x - 1) stage[pos++] = (w0 + j) * 32 + (uint32_t)(__ffs(x) - 1);
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) out[carry + i] = stage[i];
-    carry += total;
-    __syncthreads();
+  } else {
+    uint32_t pos = carry + ex;
+#pragma unroll
+    for (uint32_t j = 0; j < CW; ++j)
+      for (uint32_t x = w[j]; x; x &= x - 1) out[pos++] = (w0 + j) * 32 + (uint32_t)(__ffs(x) - 1);
   }
-  if (sg == nseg - 1 && threadIdx.x == 0) bp[NB - 1] = carry;
+  if (sg == nseg - 1 && threadIdx.x == 0) bp[NB - 1] = carry + total;
 }
 
 // Reduce-then-scan for generic u32 arrays (wrapping u32 values, u64 total).
