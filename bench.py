@@ -167,7 +167,7 @@ def main():
     # the same batched path replayed through the context's cached hipGraphs
     # (captured phases updated in place), alternating with plain launches
     gcmp = {"graph": 0.0, "launches": 0.0}
-    i0, u0 = ctx.graph_cache_stats()
+    i0, u0, r0 = ctx.graph_cache_stats()
     for _ in range(args.steps):
         for name, ug in (("graph", True), ("launches", False)):
             t0 = time.perf_counter()
@@ -175,11 +175,12 @@ def main():
             gcmp[name] += time.perf_counter() - t0
             if ug:
                 assert [(x.count, x.hash) for x in r] == [(x.count, x.hash) for x in res]
-    i1, u1 = ctx.graph_cache_stats()
+    i1, u1, r1 = ctx.graph_cache_stats()
     extra["graph_replay_qps"] = len(qs) * args.steps / gcmp["graph"]
     extra["graph_over_launches"] = gcmp["launches"] / gcmp["graph"]
     extra["graph_instantiations"] = i1 - i0
     extra["graph_updates"] = u1 - u0
+    extra["graph_replays"] = r1 - r0
     if not args.no_plain:
         rg.match(qs[0])
         t0 = time.perf_counter()

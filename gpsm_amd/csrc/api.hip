@@ -232,10 +232,11 @@ int gpsm_profile_read(gpsm_ctx* c, gpsm_kstat* out, uint32_t max, uint32_t* n) {
   return GPSM_OK;
 }
 
-int gpsm_graph_cache_stats(const gpsm_ctx* c, uint64_t* instantiations, uint64_t* updates) {
+int gpsm_graph_cache_stats(const gpsm_ctx* c, uint64_t* instantiations, uint64_t* updates, uint64_t* replays) {
   if (!c) return GPSM_EINVAL;
   if (instantiations) *instantiations = c->graph_instantiations;
   if (updates) *updates = c->graph_updates;
+  if (replays) *replays = c->graph_replays;
   return GPSM_OK;
 }
 
@@ -605,8 +606,9 @@ struct QState {
 // context's cached executable graph of that phase: updated in place when the
 // captured topology matches, else (re)instantiated.  The captured graph is
 // destroyed on every path.
-int launch_captured(gpsm_ctx* c, int phase, hipStream_t s) {
+int launch_captured(gpsm_ctx* c, int phase, hipStream_t s, const std::vector<uint64_t>& sig) {
   hipGraph_t graph = nullptr;
+  c->gsig[phase].clear();
   if (hipStreamEndCapture(s, &graph) != hipSuccess || !graph) {
     (void)hipGetLastError();
     if (graph) hipGraphDestroy(graph);
@@ -640,7 +642,38 @@ int launch_captured(gpsm_ctx* c, int phase, hipStream_t s) {
     (void)hipGetLastError();
     return GPSM_EDEVICE;
   }
+  c->gsig[phase] = sig;
   return GPSM_OK;
+}
+
+// The argument list of a phase's launches, word by word: every value a
+// kernel argument or a launch shape of the phase is computed from (buffer
+// addresses, the data graph's fields, sizes, counts, tuning).  A batch whose
+// list equals the one the cached executable graph was captured with
+// relaunches that graph as it is: its kernels read the batch's own data from
+// the same buffers, uploaded before the phase.
+struct PhaseSig {
+  std::vector<uint64_t> w;
+  void add(uint64_t v) { w.push_back(v); }
+  void add(const void* p) { w.push_back((uint64_t)(uintptr_t)p); }
+  void add_graph(const DevGraph& g) {
+    add(g.n), add(g.adj), add(g.W), add(g.NB), add(g.row), add(g.col), add(g.key), add(g.ckey), add(g.fchunk),
+        add(g.nfchunk), add(g.wrange), add(g.hub_of), add(g.hub_bits), add(g.nhub), add(g.hub_degree);
+  }
+};
+
+// Relaunches phase's cached graph when sig matches its capture; false when
+// the phase has to be captured (or launched) afresh.
+bool replay_cached(gpsm_ctx* c, int phase, const std::vector<uint64_t>& sig, hipStream_t s, int* rc) {
+  *rc = GPSM_OK;
+  if (!c->gexec[phase] || c->gsig[phase].empty() || c->gsig[phase] != sig) return false;
+  if (hipGraphLaunch(c->gexec[phase], s) != hipSuccess) {
+    (void)hipGetLastError();
+    *rc = GPSM_EDEVICE;
+  } else {
+    ++c->graph_replays;
+  }
+  return true;
 }
 
 int predict_with_ctx(void* user, const float* x, uint32_t B, float* y) {
@@ -772,7 +805,19 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
   if ((rc = c->h_stage[0].ensure(sizeof(uint32_t) * std::max<uint32_t>(nslots, 1)))) return rc;
   uint32_t* bitmaps = dptr<uint32_t>(c, B_BITMAPS);
   unsigned long long* fstats = dptr<unsigned long long>(c, B_FSTATS);
-  {
+  PhaseSig sig_a;
+  if (o.use_graph) {
+    sig_a.add(uint64_t{0});
+    sig_a.add_graph(dg);
+    for (int i : {B_BITMAPS, B_FNITEMS, B_RFLAGS, B_INIT, B_FOPS, B_NBR_SLOT, B_NBR_LO, B_NBR_HI, B_FITEMS, B_SEGC,
+                  B_COUNTS})
+      sig_a.add(c->bufs[i].ptr);
+    sig_a.add(nslots), sig_a.add(maxsteps), sig_a.add(nq), sig_a.add(init.size()), sig_a.add(fblock);
+    for (uint32_t v : step_off) sig_a.add(v);
+  }
+  bool replayed = o.use_graph && replay_cached(c, 0, sig_a.w, s, &rc);
+  if (replayed && rc) return rc;
+  if (!replayed) {
     bool captured = false;
     if (o.use_graph && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) captured = true;
     launch_fill_u32(bitmaps, 0u, (uint64_t)nslots * W, s);
@@ -796,7 +841,9 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       pend(K_FILTER, a, 4.0 * W * nops_s, counters ? (int)sidx : -1);
     }
     launch_row_popcount(dg, bitmaps, nslots, dptr<uint32_t>(c, B_SEGC), dptr<uint32_t>(c, B_COUNTS), s);
-    if (captured && (rc = launch_captured(c, 0, s))) return rc;
+    if (captured && (rc = launch_captured(c, 0, s, sig_a.w))) return rc;
+  }
+  {
     HIPCHK(hipGetLastError());
     std::vector<uint32_t>& counts = c->h_counts;
     counts.assign(std::max<uint32_t>(nslots, 1), 0);
@@ -1092,7 +1139,23 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     emit_cap = o.tuple_cap;
     if ((rc = c->bufs[B_EMIT].ensure(sizeof(uint32_t) * std::max<uint64_t>(emit_cap * qmax, 1)))) return rc;
   }
-  {
+  PhaseSig sig_c;
+  if (o.use_graph) {
+    sig_c.add(uint64_t{1});
+    sig_c.add_graph(dg);
+    for (int i : {B_BITMAPS, B_CLIST, B_CDESC, B_OFFSETS, B_CHUNKS, B_JDESC, B_LISTS, B_TERM_DESC, B_TERM_START,
+                  B_TERMS, B_JOUT, B_CTR, B_BPREFIX, B_PLANS, B_TASKS, B_LEVELS, B_EMIT})
+      sig_c.add(c->bufs[i].ptr);
+    for (uint64_t v : {(uint64_t)cdesc.size(), (uint64_t)cgrid, (uint64_t)jdesc.size(), (uint64_t)term_desc.size(),
+                       (uint64_t)term_words, (uint64_t)nplans, (uint64_t)tasks.size(), (uint64_t)ntree,
+                       (uint64_t)ncyc, (uint64_t)nwg, (uint64_t)nwg_cyc, (uint64_t)lwg, (uint64_t)lcap,
+                       (uint64_t)qmax, (uint64_t)lw, (uint64_t)W, (uint64_t)NB, (uint64_t)emit_cap,
+                       (uint64_t)o.work_budget})
+      sig_c.add(v);
+  }
+  bool replayed_c = o.use_graph && replay_cached(c, 1, sig_c.w, s, &rc);
+  if (replayed_c && rc) return rc;
+  if (!replayed_c) {
     bool captured = false;
     if (o.use_graph && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) captured = true;
     hipEvent_t pa = pbegin(K_CET_COUNT);
@@ -1157,7 +1220,9 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       launch_join(ja, std::min(lwg, std::min(nwg_cyc, ncyc)), true, s);
     }
     pend(K_JOIN, pa, 0.0, -1);
-    if (captured && (rc = launch_captured(c, 1, s))) return rc;
+    if (captured && (rc = launch_captured(c, 1, s, sig_c.w))) return rc;
+  }
+  {
     HIPCHK(hipGetLastError());
     std::vector<JoinOut> jo(std::max<uint32_t>(nplans, 1));
     HIPCHK(hipMemcpyAsync(c->h_stage[2].ptr, dptr<JoinOut>(c, B_JOUT), sizeof(JoinOut) * nplans,

@@ -171,14 +171,17 @@ struct gpsm_ctx {
   gpsm::HostBuf h_stage[3];  // pinned readbacks of a batch: counts, |CET| descriptors, join results
   gpsm::PinnedArena arena;   // pinned sources of a batch's uploads
   // Instantiated hipGraphs of a batch's two captured phases (filter; tables +
-  // join), kept across batches: a new capture updates them in place when its
-  // topology is the same (hipGraphExecUpdate) and is instantiated afresh
-  // only when it is not.  graph_instantiations counts the latter.
+  // join), kept across batches: a batch whose launches have the argument
+  // list a graph was captured with (gsig) relaunches it without a capture
+  // (graph_replays); otherwise a new capture updates it in place when its
+  // topology is the same (hipGraphExecUpdate, graph_updates) and is
+  // instantiated afresh when it is not (graph_instantiations).
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  std::vector<uint64_t> gsig[2];
   // this device's copy of the allocation-free scan's static scratch array
   // (hipGetSymbolAddress at gpsm_init)
   uint32_t* noalloc_scratch = nullptr;
-  uint64_t graph_instantiations = 0, graph_updates = 0;
+  uint64_t graph_instantiations = 0, graph_updates = 0, graph_replays = 0;
   // hub adjacency bitmaps of later uploads: vertices of degree > the hub
   // degree, starting at hub_min_degree and doubled until they fit
   // hub_max_bytes (gpsm_init: 48 GiB or a quarter of the device's memory,

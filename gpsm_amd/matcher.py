@@ -144,10 +144,12 @@ class Context:
         check(_lib.lib().gpsm_set_hub_policy(self.h, min_degree, max_bytes), "gpsm_set_hub_policy")
 
     def graph_cache_stats(self):
-        """(instantiations, in-place updates) of the context's cached hipGraphs."""
-        i, u = C.c_uint64(), C.c_uint64()
-        check(_lib.lib().gpsm_graph_cache_stats(self.h, C.byref(i), C.byref(u)), "gpsm_graph_cache_stats")
-        return i.value, u.value
+        """(instantiations, in-place updates, relaunches without a capture) of
+        the context's cached hipGraphs."""
+        i, u, r = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(_lib.lib().gpsm_graph_cache_stats(self.h, C.byref(i), C.byref(u), C.byref(r)),
+              "gpsm_graph_cache_stats")
+        return i.value, u.value, r.value
 
     def tuning(self):
         """The context's launch tuning (defaults until gpsm_autotune runs)."""

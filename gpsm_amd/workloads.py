@@ -275,6 +275,26 @@ def run(name, steps=5, autotune=False, plain=True, seed=None, budget=EMBEDDING_B
                 % (label, len(qs) / dt, emb / dt, 1e3 * dt))
         out["embeddings"] = sum(c for c, _, _ in ref)
         out["statuses"] = sorted({s for _, _, s in ref})
+        # the last order's batch replayed through the context's cached
+        # hipGraphs, alternating with plain launches batch by batch
+        label, mode = modes[-1]
+        rg.match_batch(qs, order_mode=mode, use_graph=True)
+        i0, u0, r0 = ctx.graph_cache_stats()
+        tg = {True: 0.0, False: 0.0}
+        for _ in range(steps):
+            for ug in (True, False):
+                t0 = time.perf_counter()
+                r = rg.match_batch(qs, order_mode=mode, use_graph=ug)
+                tg[ug] += time.perf_counter() - t0
+                assert [(x.count, x.hash, x.status) for x in r] == ref, "hipGraph replay changed a result"
+        i1, u1, r1 = ctx.graph_cache_stats()
+        out["graph_replay"] = {"order": label, "graph_ms_per_batch": 1e3 * tg[True] / steps,
+                               "launches_ms_per_batch": 1e3 * tg[False] / steps,
+                               "graph_over_launches": tg[False] / tg[True],
+                               "instantiations": i1 - i0, "updates": u1 - u0, "replays": r1 - r0}
+        log("hipGraph replay %.3f ms/batch, plain launches %.3f ms/batch (%.2fx; %d instantiations, %d updates, "
+            "%d relaunches)" % (1e3 * tg[True] / steps, 1e3 * tg[False] / steps, tg[False] / tg[True], i1 - i0,
+                                u1 - u0, r1 - r0))
         if plain:
             rg.match(qs[0])
             t0 = time.perf_counter()

@@ -333,11 +333,14 @@ int gpsm_profile_enable(gpsm_ctx* ctx, int mode);
 /* Copies up to max kernel-family statistics; *n receives the number. */
 int gpsm_profile_read(gpsm_ctx* ctx, gpsm_kstat* out, uint32_t max, uint32_t* n);
 /* With opts.use_graph, a batch's two captured phases run through executable
- * hipGraphs that the context keeps: a later batch's capture updates them in
- * place when its topology is the same and instantiates them again when it is
- * not.  These counters (since gpsm_init) say which happened.  No counterpart
- * in the reference (no graphs). */
-int gpsm_graph_cache_stats(const gpsm_ctx* ctx, uint64_t* instantiations, uint64_t* updates);
+ * hipGraphs that the context keeps.  A later batch whose launches have the
+ * same arguments (buffer addresses, sizes, counts: e.g. the same queries
+ * again, or queries of the same shapes) relaunches them without capturing;
+ * otherwise its capture updates them in place when its topology is the same
+ * and instantiates them again when it is not.  These counters (since
+ * gpsm_init, per phase launch) say which happened; any pointer may be NULL.
+ * No counterpart in the reference (no graphs). */
+int gpsm_graph_cache_stats(const gpsm_ctx* ctx, uint64_t* instantiations, uint64_t* updates, uint64_t* replays);
 
 /* --- set hash ------------------------------------------------------------- */
 

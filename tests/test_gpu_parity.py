@@ -178,26 +178,40 @@ def test_batch_equals_single_and_oracle(ctx):
 
 def test_cached_graphs_replay_and_update(ctx):
     """The captured phases run through the context's cached executable
-    graphs: the same batch again updates them in place (no new
-    instantiation), eager batches and synchronisations in between change
-    nothing, and a batch of another shape re-instantiates them; every replay
-    equals the eager result."""
+    graphs: the same batch again relaunches them without a capture, eager
+    batches and synchronisations in between change nothing, a batch of
+    another shape is captured again (an in-place update or a new
+    instantiation), and a batch of the same shapes with other labels (whose
+    launches may have the same arguments as the last one's, reading other
+    data) gives its own results; every run equals the eager result."""
     g = G.small_graph(21, n=4000, m=20000, labels=3, kind="cl")
     rg = ctx.upload(g)
     qa = G.queries(g, 16, 3, 7, _lib.Q_WALK_HALF, 31, 10 ** 7)
     qb = G.queries(g, 9, 4, 9, _lib.Q_WALK_INDUCED, 32, 10 ** 7)
+    # qa's shapes with every label moved to the next one of the graph's three
+    qc = [G.Query(labels=np.array([l % 3 + 1 for l in q.labels], dtype=np.uint32), src=q.src.copy(),
+                  dst=q.dst.copy()) for q in qa]
     want_a = [(r.count, r.hash, r.join_work) for r in rg.match_batch(qa)]
     want_b = [(r.count, r.hash, r.join_work) for r in rg.match_batch(qb)]
-    i0, u0 = ctx.graph_cache_stats()
+    want_c = [(r.count, r.hash, r.join_work) for r in rg.match_batch(qc)]
+    i0, u0, r0 = ctx.graph_cache_stats()
     for rep in range(4):
         got = [(r.count, r.hash, r.join_work) for r in rg.match_batch(qa, use_graph=True)]
         assert got == want_a, rep
         assert [(r.count, r.hash, r.join_work) for r in rg.match_batch(qa)] == want_a  # eager between replays
-    i1, u1 = ctx.graph_cache_stats()
-    assert i1 - i0 <= 2 and u1 - u0 >= 6  # two phases: instantiated once, then updated
+    i1, u1, r1 = ctx.graph_cache_stats()
+    # two phases: captured once (instantiated or updated), then relaunched
+    assert i1 - i0 + u1 - u0 == 2 and r1 - r0 == 6
     for rep in range(2):
         assert [(r.count, r.hash, r.join_work) for r in rg.match_batch(qb, use_graph=True)] == want_b
+    i2, u2, r2 = ctx.graph_cache_stats()
+    assert i2 - i1 + u2 - u1 >= 2 and r2 - r1 >= 2
     assert [(r.count, r.hash, r.join_work) for r in rg.match_batch(qa, use_graph=True)] == want_a
+    _, _, r3 = ctx.graph_cache_stats()
+    for rep in range(2):
+        assert [(r.count, r.hash, r.join_work) for r in rg.match_batch(qc, use_graph=True)] == want_c, rep
+    _, _, r4 = ctx.graph_cache_stats()
+    assert r4 - r3 >= 2  # at least the second run relaunches both phases
 
 
 def test_small_join_buffers_force_depth_first_spills(ctx):
