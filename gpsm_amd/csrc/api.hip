@@ -281,7 +281,10 @@ int gpsm_init(int device, gpsm_ctx** out) {
     c->ncu = 256;
   }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+      hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
     delete c;
     return GPSM_EDEVICE;
   }
@@ -296,7 +299,10 @@ int gpsm_init(int device, gpsm_ctx** out) {
     set_error("hipGetSymbolAddress(scratch)", hipGetLastError());
     hipEventDestroy(c->ev0);
     hipEventDestroy(c->ev1);
+    hipEventDestroy(c->ev_fork);
+    hipEventDestroy(c->ev_join);
     hipStreamDestroy(c->stream);
+    hipStreamDestroy(c->stream2);
     c->bufs[B_SCRATCH].release();
     delete c;
     return GPSM_EDEVICE;
@@ -309,6 +315,7 @@ void gpsm_shutdown(gpsm_ctx* c) {
   if (!c) return;
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
+  hipStreamSynchronize(c->stream2);
   for (auto& e : c->gexec)
     if (e) hipGraphExecDestroy(e);
   for (auto& b : c->bufs) b.release();
@@ -316,7 +323,10 @@ void gpsm_shutdown(gpsm_ctx* c) {
   c->arena.release();
   hipEventDestroy(c->ev0);
   hipEventDestroy(c->ev1);
+  hipEventDestroy(c->ev_fork);
+  hipEventDestroy(c->ev_join);
   hipStreamDestroy(c->stream);
+  hipStreamDestroy(c->stream2);
   delete c;
 }
 
@@ -1125,15 +1135,21 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       (rc = c->h_stage[2].ensure(sizeof(JoinOut) * std::max<uint32_t>(nplans, 1))))
     return rc;
   if ((rc = c->bufs[B_CTR].ensure(256))) return rc;
-  // Level buffers of the workgroups launched, within kLevelBytes: rows per
-  // level halve (to 2048) and then workgroups are dropped until they fit.
-  const uint32_t nlaunch = std::max<uint32_t>(
-      1, std::max(std::min<uint32_t>(nwg, ntree), std::min<uint32_t>(nwg_cyc, ncyc)));
-  uint32_t lcap = cap, lwg = nlaunch;
-  while (lcap > 2048 && (double)join_level_words(qmax, lcap) * 4.0 * lwg > kLevelBytes) lcap /= 2;
+  // Level buffers of the workgroups launched (the tree plans' launch, then
+  // the cyclic plans', which run at the same time), within kLevelBytes: rows
+  // per level halve (to 2048) and then workgroups are dropped, from both
+  // launches in proportion, until they fit.
+  uint32_t gt = ntree ? std::min<uint32_t>(nwg, ntree) : 0, gc = ncyc ? std::min<uint32_t>(nwg_cyc, ncyc) : 0;
+  uint32_t lcap = cap;
+  const uint64_t gsum = std::max<uint64_t>(1, (uint64_t)gt + gc);
+  while (lcap > 2048 && (double)join_level_words(qmax, lcap) * 4.0 * gsum > kLevelBytes) lcap /= 2;
   const uint64_t lw = join_level_words(qmax, lcap);
-  if (lw && (double)lw * 4.0 * lwg > kLevelBytes) lwg = std::max<uint32_t>(1, (uint32_t)(kLevelBytes / (4.0 * lw)));
-  if ((rc = c->bufs[B_LEVELS].ensure(sizeof(uint32_t) * std::max<uint64_t>(lw * lwg, 1)))) return rc;
+  if (lw && (double)lw * 4.0 * gsum > kLevelBytes) {
+    const uint64_t fit = std::max<uint64_t>(2, (uint64_t)(kLevelBytes / (4.0 * lw)));
+    if (gt) gt = (uint32_t)std::max<uint64_t>(1, (uint64_t)gt * fit / gsum);
+    if (gc) gc = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)gc * fit / gsum + 1, fit - gt));
+  }
+  if ((rc = c->bufs[B_LEVELS].ensure(sizeof(uint32_t) * std::max<uint64_t>(lw * ((uint64_t)gt + gc), 1)))) return rc;
   uint64_t emit_cap = 0;
   if (o.want_tuples && nq == 1) {
     emit_cap = o.tuple_cap;
@@ -1148,7 +1164,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       sig_c.add(c->bufs[i].ptr);
     for (uint64_t v : {(uint64_t)cdesc.size(), (uint64_t)cgrid, (uint64_t)jdesc.size(), (uint64_t)term_desc.size(),
                        (uint64_t)term_words, (uint64_t)nplans, (uint64_t)tasks.size(), (uint64_t)ntree,
-                       (uint64_t)ncyc, (uint64_t)nwg, (uint64_t)nwg_cyc, (uint64_t)lwg, (uint64_t)lcap,
+                       (uint64_t)ncyc, (uint64_t)nwg, (uint64_t)nwg_cyc, (uint64_t)gt, (uint64_t)gc, (uint64_t)lcap,
                        (uint64_t)qmax, (uint64_t)lw, (uint64_t)W, (uint64_t)NB, (uint64_t)emit_cap,
                        (uint64_t)o.work_budget})
       sig_c.add(v);
@@ -1209,16 +1225,28 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     // tree plans, then cyclic plans: one launch each, its own task counter;
     // timed together as one join
     pa = pbegin(K_JOIN);
+    // the cyclic plans on stream2 (own task counter and level buffers),
+    // forked after everything before on stream and joined back before what
+    // follows; the tree plans on stream
+    if (ncyc) {
+      JoinArgs jc = ja;
+      jc.tasks = dptr<JoinTask>(c, B_TASKS) + ntree;
+      jc.ntasks = ncyc;
+      jc.task_counter = dptr<unsigned int>(c, B_CTR) + 1;
+      jc.levels = dptr<uint32_t>(c, B_LEVELS) + lw * gt;
+      hipStream_t s2 = ntree ? c->stream2 : s;
+      if (ntree) {
+        HIPCHK(hipEventRecord(c->ev_fork, s));
+        HIPCHK(hipStreamWaitEvent(s2, c->ev_fork, 0));
+      }
+      launch_join(jc, gc, true, s2);
+      if (ntree) HIPCHK(hipEventRecord(c->ev_join, s2));
+    }
     if (ntree) {
       ja.ntasks = ntree;
-      launch_join(ja, std::min(lwg, std::min(nwg, ntree)), false, s);
+      launch_join(ja, gt, false, s);
     }
-    if (ncyc) {
-      ja.tasks = dptr<JoinTask>(c, B_TASKS) + ntree;
-      ja.ntasks = ncyc;
-      ja.task_counter = dptr<unsigned int>(c, B_CTR) + 1;
-      launch_join(ja, std::min(lwg, std::min(nwg_cyc, ncyc)), true, s);
-    }
+    if (ncyc && ntree) HIPCHK(hipStreamWaitEvent(s, c->ev_join, 0));
     pend(K_JOIN, pa, 0.0, -1);
     if (captured && (rc = launch_captured(c, 1, s, sig_c.w))) return rc;
   }

@@ -163,6 +163,12 @@ struct gpsm_ctx {
   int ncu = 256;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // the cyclic plans' join runs on stream2 beside the tree plans' (forked
+  // from and joined back into stream with ev_fork / ev_join), so that its
+  // workgroups fill the CUs the tree launch's last tasks leave idle; both
+  // streams at the default priority
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   char name[256] = {0};
   gpsm::DevBuf bufs[gpsm::B_COUNT_];
   gpsm::Tuning tune;

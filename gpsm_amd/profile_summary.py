@@ -63,6 +63,21 @@ def add_join_total(per_kernel, key, count_key):
     per_kernel["join_kernel"] = tot
 
 
+def join_pair_us(trace_csv):
+    """Per batch, first start to last end of its join_tree_kernel and
+    join_cyclic_kernel launches (counting variants left out), paired in
+    launch order; a batch with one of them counts that one."""
+    ev = {"join_tree_kernel": [], "join_cyclic_kernel": []}
+    for r in csv.DictReader(open(trace_csv)):
+        k = short(r["Kernel_Name"])
+        if k in ev:
+            ev[k].append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+    t, c = sorted(ev["join_tree_kernel"]), sorted(ev["join_cyclic_kernel"])
+    if not t or not c or len(t) != len(c):
+        return [(e - b) * 1e-3 for b, e in (t or c)]
+    return [(max(a[1], b[1]) - min(a[0], b[0])) * 1e-3 for a, b in zip(t, c)]
+
+
 def find(d, suffix):
     for f in os.listdir(d):
         if f.endswith(suffix):
@@ -82,7 +97,8 @@ def main():
     ap.add_argument("--prof", required=True)
     ap.add_argument("--out", required=True)
     ap.add_argument("--tag", default="chunglu1M_mix256")
-    ap.add_argument("--command", default="")
+    ap.add_argument("--command", default="bash gpsm_amd/profile_bench.sh gpurun_out/prof && "
+                    "python3 -m gpsm_amd.profile_summary --prof gpurun_out/prof --out profiles")
     args = ap.parse_args()
     P = args.prof
     copy_bytes = 1 << 30
@@ -161,6 +177,12 @@ def main():
     for r in rows[:16]:
         lines.append("| %s | %s | %.1f | %.2f | %.2f |" % (short(r["Name"]), r["Calls"], float(r["AverageNs"]) / 1e3,
                                                          float(r["TotalDurationNs"]) / 1e6, float(r["Percentage"])))
+    pair = join_pair_us(find(os.path.join(P, "trace"), "kernel_trace.csv"))
+    if pair:
+        lines += ["", "A batch's tree-plan and cyclic-plan join launches run at the same time on two streams, so "
+                  "their durations overlap (and the %% column counts both): from their first start to their last "
+                  "end they take %.1f us per batch on average (%d batches of the trace)." % (sum(pair) / len(pair),
+                                                                                             len(pair))]
     if bl:
         lines += ["", "The traced run's bench line: join average launch %.1f us over its %d timed launches (HIP "
                   "events), %s queries/s." % (bl["roofline"]["avg_launch_us"], bl["roofline"]["launches"], bl["value"])]
