@@ -176,12 +176,12 @@ struct gpsm_ctx {
   std::vector<uint32_t> h_counts;
   gpsm::HostBuf h_stage[3];  // pinned readbacks of a batch: counts, |CET| descriptors, join results
   gpsm::PinnedArena arena;   // pinned sources of a batch's uploads
-  // Instantiated hipGraphs of a batch's two captured phases (filter; tables +
-  // join), kept across batches: a batch whose launches have the argument
-  // list a graph was captured with (gsig) relaunches it without a capture
-  // (graph_replays); otherwise a new capture updates it in place when its
-  // topology is the same (hipGraphExecUpdate, graph_updates) and is
-  // instantiated afresh when it is not (graph_instantiations).
+  // Instantiated hipGraphs of a batch's two captured phases (filter;
+  // candidate-edge tables), kept across batches: a batch whose launches
+  // have the argument list a graph was captured with (gsig) relaunches it
+  // without a capture (graph_replays); otherwise a new capture updates it in
+  // place when its topology is the same (hipGraphExecUpdate, graph_updates)
+  // and is instantiated afresh when it is not (graph_instantiations).
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   std::vector<uint64_t> gsig[2];
   // this device's copy of the allocation-free scan's static scratch array

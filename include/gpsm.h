@@ -332,8 +332,9 @@ typedef struct gpsm_kstat {
 int gpsm_profile_enable(gpsm_ctx* ctx, int mode);
 /* Copies up to max kernel-family statistics; *n receives the number. */
 int gpsm_profile_read(gpsm_ctx* ctx, gpsm_kstat* out, uint32_t max, uint32_t* n);
-/* With opts.use_graph, a batch's two captured phases run through executable
- * hipGraphs that the context keeps.  A later batch whose launches have the
+/* With opts.use_graph, a batch's two captured phases (the filter; the
+ * candidate-edge tables) run through executable hipGraphs that the context
+ * keeps; the join is launched directly.  A later batch whose launches have the
  * same arguments (buffer addresses, sizes, counts: e.g. the same queries
  * again, or queries of the same shapes) relaunches them without capturing;
  * otherwise its capture updates them in place when its topology is the same
