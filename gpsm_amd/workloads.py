@@ -31,6 +31,7 @@ rule's parameters and the outcome of every slot.
 """
 import json
 import os
+import sys
 
 import numpy as np
 
@@ -360,6 +361,56 @@ def loaders(name, tmpdir, text=True, log=print):
     return out
 
 
+PARTS = {"rmat22_shapes": ("rmat22_cycles4_8", "rmat22_cliques4_8", "rmat22_walks8_50"),
+         "chunglu1m_mix4_32": ("chunglu1m_mix4_32",)}
+
+
+def markdown(out, command):
+    """A Markdown summary of a run() result: the stored sets' feasibility
+    records, rates per path, the hipGraph comparison and kernel times."""
+    L = ["# Workload %s" % out["workload"], "", "Command (from the repository root, one MI355X):", "", "```", command,
+         "```", "", "Graph: %d vertices, %d undirected edges; %d queries (sizes %s)."
+         % (out["n"], out["undirected_edges"], out["queries"], out["query_sizes"]), ""]
+    parts = PARTS.get(out["workload"])
+    if parts:
+        L += ["Stored sets (gpsm_amd/data; a draw is kept when its exact join with the heuristic order finishes "
+              "within the work budget; fresh size per draw, never shrunk):", "",
+              "| set | slots | draws | kept | sizes kept | dropped slots | embeddings found before abandoning a "
+              "rejected draw (min / median / max) |", "|---|---|---|---|---|---|---|"]
+        for name in parts:
+            _, rec = load_set(name)
+            qs = rec.get("rejected_count_lower_bound_quantiles") or []
+            qtxt = " / ".join("%.3g" % qs[i] for i in (0, len(qs) // 2, -1)) if qs else "-"
+            L.append("| %s | %d | %d | %d | %s | %d | %s |" % (name, rec["rule"]["slots"], rec["draws"], rec["kept"],
+                                                              rec["query_sizes"], rec["dropped_slots"], qtxt))
+        L.append("")
+    L += ["%d embeddings per batch; every path agrees on every count and hash (checked by the runner)."
+          % out["embeddings"], "", "| path | queries/s | embeddings/s | ms per batch | examined candidates |",
+          "|---|---|---|---|---|"]
+    paths = [k for k in ("batched_heuristic", "batched_learned") if k in out]
+    for k in paths:
+        v = out[k]
+        L.append("| %s | %.2f | %.3g | %.1f | %d |" % (k.replace("_", ", "), v["queries_per_s"], v["embeddings_per_s"],
+                                                        v["ms_per_batch"], v["join_work"]))
+    if "plain_heuristic" in out:
+        v = out["plain_heuristic"]
+        L.append("| plain per query, heuristic | %.2f | %.3g | | |" % (v["queries_per_s"], v["embeddings_per_s"]))
+    gr = out.get("graph_replay")
+    if gr:
+        L += ["", "Cached hipGraphs (%s order), alternating with plain launches: %.3f against %.3f ms per batch "
+              "(%.2fx; %d instantiations, %d updates, %d relaunches without a capture)."
+              % (gr["order"], gr["graph_ms_per_batch"], gr["launches_ms_per_batch"], gr["graph_over_launches"],
+                 gr["instantiations"], gr["updates"], gr["replays"])]
+    if paths:
+        L += ["", "Kernel time per batch (ms, HIP events around every launch):", "",
+              "| kernel | " + " | ".join(k.split("_", 1)[1] for k in paths) + " |",
+              "|---|" + "---|" * len(paths)]
+        names = [n for n in out[paths[0]]["kernels_ms"] if not n.startswith("wall:")]
+        for n in names:
+            L.append("| %s | %s |" % (n, " | ".join("%.3f" % out[k]["kernels_ms"].get(n, 0.0) for k in paths)))
+    return "\n".join(L) + "\n"
+
+
 def main():
     import argparse
     import json
@@ -377,6 +428,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--budget", type=float, default=EMBEDDING_BUDGET)
     ap.add_argument("--json", default=None)
+    ap.add_argument("--markdown", default=None, help="write a Markdown summary of the run to this file")
     ap.add_argument("--loaders", metavar="TMPDIR", default=None,
                     help="time text parsing, the binary cache and the device CSR build instead")
     ap.add_argument("--no-text", action="store_true", help="with --loaders: skip the text format")
@@ -395,6 +447,9 @@ def main():
     if args.json:
         with open(args.json, "w") as fh:
             json.dump(out, fh, indent=1)
+    if args.markdown and not args.loaders:
+        with open(args.markdown, "w") as fh:
+            fh.write(markdown(out, "python3 -m gpsm_amd.workloads " + " ".join(sys.argv[1:])))
 
 
 if __name__ == "__main__":
