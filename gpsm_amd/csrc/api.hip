@@ -802,9 +802,8 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     return rc;
   uint32_t max_ops = 0;  // ops of the largest superstep
   for (uint32_t sidx = 0; sidx < maxsteps; ++sidx) max_ops = std::max(max_ops, step_off[sidx + 1] - step_off[sidx]);
-  if ((rc = c->bufs[B_FITEMS].ensure(sizeof(uint2) * std::max<uint64_t>(
-                                         (uint64_t)FILTER_XCDS * filter_list_cap(max_ops, dg.nfchunk), 1))) ||
-      (rc = c->bufs[B_FNITEMS].ensure(sizeof(uint32_t) * FILTER_XCDS * std::max<uint32_t>(maxsteps, 1))) ||
+  if ((rc = c->bufs[B_FITEMS].ensure(sizeof(uint2) * std::max<uint64_t>((uint64_t)max_ops * dg.nfchunk, 1))) ||
+      (rc = c->bufs[B_FNITEMS].ensure(sizeof(uint32_t) * std::max<uint32_t>(maxsteps, 1))) ||
       (rc = c->bufs[B_RFLAGS].ensure(sizeof(uint32_t) * 4 * (uint64_t)std::max<uint32_t>(nq, 1))))
     return rc;
   // work counters (profile mode 2): one slot per filter superstep, then the
@@ -832,7 +831,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
     bool captured = false;
     if (o.use_graph && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) captured = true;
     launch_fill_u32(bitmaps, 0u, (uint64_t)nslots * W, s);
-    if (maxsteps) launch_fill_u32(dptr<uint32_t>(c, B_FNITEMS), 0u, (uint64_t)FILTER_XCDS * maxsteps, s);
+    if (maxsteps) launch_fill_u32(dptr<uint32_t>(c, B_FNITEMS), 0u, maxsteps, s);
     launch_fill_u32(dptr<uint32_t>(c, B_RFLAGS), 0u, 4ull * std::max<uint32_t>(nq, 1), s);
     if (counters) launch_fill_u32(reinterpret_cast<uint32_t*>(fstats), 0u, 2ull * fstat_step * nstat_slots, s);
     {
@@ -846,7 +845,7 @@ int run_batch(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_query* q
       hipEvent_t a = pbegin(K_FILTER);
       launch_filter_step(dg, dptr<FilterOp>(c, B_FOPS) + step_off[sidx], nops_s, dptr<uint32_t>(c, B_NBR_SLOT),
                          dptr<uint32_t>(c, B_NBR_LO), dptr<uint32_t>(c, B_NBR_HI), bitmaps, fblock,
-                         65536, dptr<uint2>(c, B_FITEMS), dptr<uint32_t>(c, B_FNITEMS) + FILTER_XCDS * sidx,
+                         65536, dptr<uint2>(c, B_FITEMS), dptr<uint32_t>(c, B_FNITEMS) + sidx,
                          dptr<uint32_t>(c, B_RFLAGS),
                          counters ? fstats + fstat_step * sidx : nullptr, s);
       pend(K_FILTER, a, 4.0 * W * nops_s, counters ? (int)sidx : -1);

@@ -78,7 +78,7 @@ enum BufId {
   B_TEXT_WG,     // ... scan scratch, per-workgroup lengths and offsets
   B_TEXT_OUT,    // ... the chunk's text
   B_FITEMS,      // filter superstep: (op, vertex range) pairs with candidates
-  B_FNITEMS,     // ... the lengths of their FILTER_XCDS lists, per superstep
+  B_FNITEMS,     // ... their number, one counter per superstep
   B_CDESC,       // candidate-edge tables of reversed plan edges (counted again)
   B_RFLAGS,      // refine rounds that removed a candidate, 4 per query
   B_COUNT_

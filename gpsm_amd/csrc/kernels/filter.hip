@@ -73,8 +73,7 @@ __global__ void __launch_bounds__(256) init_kernel(const uint32_t* __restrict__ 
 // each word's first vertex (wrange), and a nonzero word marks its range
 // (and, when it runs past that range's end, the next ones it has bits in).
 // The four waves of a workgroup take consecutive range groups of one op,
-// and the pairs go to that op's list (op % FILTER_XCDS, the XCD its filter
-// waves run on), appended to with one atomic per workgroup: appends to a
+// and the list is appended to with one atomic per workgroup: appends to a
 // single counter serialise, and one per wave cost as much as the scan.
 constexpr uint32_t AR_GROUP = 4 * WAVE;  // vertex ranges per wave
 constexpr int AR_UNROLL = 8;             // bitmap words in flight per lane
@@ -84,8 +83,7 @@ __global__ void __launch_bounds__(256) active_ranges_kernel(const uint32_t* __re
                                                              const FilterOp* __restrict__ ops, uint32_t nops,
                                                              const uint32_t* __restrict__ bitmaps, uint32_t W,
                                                              const uint32_t* __restrict__ rflags,
-                                                             uint2* __restrict__ items, uint32_t list_cap,
-                                                             uint32_t* __restrict__ nitems) {
+                                                             uint2* __restrict__ items, uint32_t* __restrict__ nitems) {
   __shared__ uint32_t s_lo[4][AR_GROUP + 1];
   __shared__ uint32_t s_act[4][AR_GROUP];
   __shared__ uint32_t s_wtot[4], s_wbase[4];
@@ -158,15 +156,14 @@ __global__ void __launch_bounds__(256) active_ranges_kernel(const uint32_t* __re
       s_wbase[x] = tot;
       tot += s_wtot[x];
     }
-    const uint32_t base = tot ? atomicAdd(nitems + o % FILTER_XCDS, tot) : 0u;
+    const uint32_t base = tot ? atomicAdd(nitems, tot) : 0u;
     for (uint32_t x = 0; x < blockDim.x / WAVE; ++x) s_wbase[x] += base;
   }
   __syncthreads();
   uint32_t pos = s_wbase[wv] + incl - mine;
-  uint2* list = items + (size_t)(o % FILTER_XCDS) * list_cap;
 #pragma unroll
   for (uint32_t j = 0; j < AR_GROUP / WAVE; ++j)
-    if (act[j * WAVE + lane]) list[pos++] = make_uint2(o, c0 + j * WAVE + lane);
+    if (act[j * WAVE + lane]) items[pos++] = make_uint2(o, c0 + j * WAVE + lane);
 }
 
 constexpr int UNROLL = 4;  // adjacency entries in flight per lane
@@ -183,7 +180,7 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
                                                           const uint32_t* __restrict__ col,
                                                           const uint32_t* __restrict__ ckey, uint32_t W,
                                                           const uint32_t* __restrict__ fchunk,
-                                                          const uint2* __restrict__ lists, uint32_t list_cap,
+                                                          const uint2* __restrict__ list,
                                                           const uint32_t* __restrict__ nlist,
                                                           const FilterOp* __restrict__ ops,
                                                           const uint32_t* __restrict__ nbr_slot,
@@ -195,16 +192,9 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
   extern __shared__ unsigned char lds_raw[];
   WaveLds& L = reinterpret_cast<WaveLds*>(lds_raw)[threadIdx.x / WAVE];
   const uint32_t lane = lane_id();
-  // XCD-aware: workgroups go to the XCDs round-robin, so blockIdx %
-  // FILTER_XCDS names this workgroup's XCD, and its waves stride over that
-  // XCD's list (the grid is a multiple of FILTER_XCDS): an op's probes and
-  // bit sets into its query's bitmaps then come from one XCD, whose L2
-  // holds the bitmaps of an eighth of the step's ops.
-  const uint32_t xcd = blockIdx.x % FILTER_XCDS;
-  const uint32_t items = uniform(nlist[xcd]);
-  const uint2* __restrict__ list = lists + (size_t)xcd * list_cap;
-  const uint32_t wave = (blockIdx.x / FILTER_XCDS) * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-  const uint32_t nwaves = (gridDim.x / FILTER_XCDS) * (blockDim.x / WAVE);
+  const uint32_t items = uniform(*nlist);
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) / WAVE;
   // per-lane tallies for the optional byte accounting, over all items
   unsigned long long t_cand = 0;
   uint32_t t_ent = 0, t_key = 0, t_probe = 0, t_set = 0;
@@ -432,13 +422,18 @@ void launch_filter_step(const DevGraph& g, const FilterOp* ops, uint32_t nops, c
   if (block != 64 && block != 128 && block != 256) block = 256;
   const uint64_t pairs = (uint64_t)nops * g.nfchunk;
   const uint32_t wgs_per_op = ((g.nfchunk + AR_GROUP - 1) / AR_GROUP + 3) / 4;
-  const uint32_t list_cap = filter_list_cap(nops, g.nfchunk);
   // four waves per workgroup: the kernel's LDS and its per-workgroup append assume 256 threads
   hipLaunchKernelGGL(active_ranges_kernel, dim3(nops * wgs_per_op), dim3(256), 0, s, g.fchunk, g.wrange, g.nfchunk,
-                     ops, nops, bitmaps, g.W, rflags, items, list_cap, nitems);
+                     ops, nops, bitmaps, g.W, rflags, items, nitems);
   // the list's length is known on the device only: a grid of at most
-  // max_wg workgroups, and at most as many as are resident at once, strides
-  // over it (waves past its end leave at once); a multiple of FILTER_XCDS
+  // max_wg workgroups, and at most FILTER_OVERSUB times as many as are
+  // resident at once, strides over it (waves past its end leave at once).
+  // Workgroups beyond the resident ones start as earlier ones finish, which
+  // evens out uneven items (R-MAT hub ranges); with far more, the waves in
+  // flight touch more bitmaps at a time.  Measured per batch: 1x resident
+  // 10.9 ms (1M Chung-Lu mix) / 24.0 ms (R-MAT 22 shapes), 8x 11.1 / 20.9,
+  // 65,536 workgroups 13.3 / 19.5; items taken from a shared counter (16
+  // per atomic) 14.1 / 28.4, the atomics serialising across the XCDs.
   const uint32_t wpb = block / WAVE;
   const size_t lds = sizeof(WaveLds) * wpb;
   uint64_t grid = (pairs + wpb - 1) / wpb;
@@ -448,10 +443,9 @@ void launch_filter_step(const DevGraph& g, const FilterOp* ops, uint32_t nops, c
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
       hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, filter_step_kernel, (int)block, lds) == hipSuccess &&
       ncu > 0 && per_cu > 0)
-    grid = std::min<uint64_t>(grid, (uint64_t)ncu * per_cu);
-  grid = (grid + FILTER_XCDS - 1) / FILTER_XCDS * FILTER_XCDS;
+    grid = std::min<uint64_t>(grid, (uint64_t)ncu * per_cu * FILTER_OVERSUB);
   hipLaunchKernelGGL(filter_step_kernel, dim3((uint32_t)grid), dim3(block), lds, s, g.row, g.col, g.ckey, g.W,
-                     g.fchunk, items, list_cap, nitems, ops, nbr_slot, nbr_lo, nbr_hi, bitmaps, rflags, stats);
+                     g.fchunk, items, nitems, ops, nbr_slot, nbr_lo, nbr_hi, bitmaps, rflags, stats);
 }
 
 void filter_chunks(uint32_t n, const uint32_t* row_offset, std::vector<uint32_t>& bounds) {

@@ -66,19 +66,14 @@ void launch_init(const DevGraph& g, const InitOp* ops, uint32_t nops, uint32_t* 
 // bitmap probes, bits set, added by the workgroups with blockIdx % copies
 // == c (the sum over copies is the launch's total).
 // The superstep visits only the (op, vertex range) pairs whose range holds a
-// candidate of C(u): a first kernel lists them in items (FILTER_XCDS lists
-// of filter_list_cap pairs) and counts them in nitems[0..FILTER_XCDS)
-// (zeroed by the caller), the second processes the lists with at most max_wg
-// workgroups, at most as many as are resident.  rflags: the refine
+// candidate of C(u): a first kernel lists them in items (nops x g.nfchunk
+// capacity) and counts them in *nitems (zeroed by the caller), the second
+// processes the list with at most max_wg workgroups, and at most
+// FILTER_OVERSUB times as many as are resident.  rflags: the refine
 // rounds' change flags (FilterOp::mark / need), zeroed by the caller once
 // per batch; an op whose need flag is clear lists no range.
 constexpr uint32_t STAT_COPIES = 32, STAT_STRIDE = 8;
-// XCDs of an MI355X: workgroup b of a launch runs on XCD b % FILTER_XCDS.
-// Op o's pairs go to list o % FILTER_XCDS, processed on that XCD.
-constexpr uint32_t FILTER_XCDS = 8;
-inline uint32_t filter_list_cap(uint32_t nops, uint32_t nfchunk) {
-  return ((nops + FILTER_XCDS - 1) / FILTER_XCDS) * nfchunk;
-}
+constexpr uint32_t FILTER_OVERSUB = 8;
 void launch_filter_step(const DevGraph& g, const FilterOp* ops, uint32_t nops, const uint32_t* nbr_slot,
                         const uint32_t* nbr_lo, const uint32_t* nbr_hi, uint32_t* bitmaps, uint32_t block,
                         uint32_t max_wg, uint2* items, uint32_t* nitems, uint32_t* rflags,
