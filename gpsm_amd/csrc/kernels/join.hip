@@ -34,7 +34,10 @@ namespace {
 
 constexpr uint32_t WROWS = 256;  // rows per window = threads per workgroup
 static_assert((WROWS & (WROWS - 1)) == 0, "row indices are masked with WROWS - 1");
-constexpr int EXP_UNROLL = 4;    // candidates in flight per thread
+// candidates in flight per thread: cyclic plans, tree plans (with the
+// waves per SIMD of the two kernels, below)
+constexpr int EXP_UNROLL = 4;
+constexpr int TREE_UNROLL = 3;
 // At the last level a row with at least this many candidates is expanded by
 // one wave on its own (no row search per candidate, the row's values
 // uniform); shorter rows share lanes through the flattened expansion.
@@ -92,27 +95,27 @@ __device__ __forceinline__ bool has_edge(const uint32_t* __restrict__ row, const
 // candidate and edge).  Otherwise the candidates' own hub indices are read
 // together and a pair with no hub falls back to has_edge's binary search.
 // A candidate whose test fails skips the later ones.  oks[u] in/out.
-template <bool BYTES>
+template <bool BYTES, int U>
 __device__ __forceinline__ void back_edges_of_row(const JoinArgs& a, const JoinPlan& P,
                                                   const uint32_t* __restrict__ val, uint32_t bo, uint32_t be,
-                                                  uint32_t i, const uint32_t (&mvs)[EXP_UNROLL],
-                                                  bool (&oks)[EXP_UNROLL], unsigned long long& bytes) {
+                                                  uint32_t i, const uint32_t (&mvs)[U],
+                                                  bool (&oks)[U], unsigned long long& bytes) {
   for (uint32_t b = bo; b < be; ++b) {
     const uint32_t bv = uniform(val[P.backs[b] * WROWS + i]);
     const uint32_t hb = uniform(a.hub_of[bv]);
     if (hb != ~0u) {
       const uint32_t* bm = a.hub_bits + (size_t)hb * a.W;
-      uint32_t w[EXP_UNROLL];
+      uint32_t w[U];
 #pragma unroll
-      for (int u = 0; u < EXP_UNROLL; ++u) w[u] = oks[u] ? bm[mvs[u] >> 5] >> (mvs[u] & 31) : 0u;
+      for (int u = 0; u < U; ++u) w[u] = oks[u] ? bm[mvs[u] >> 5] >> (mvs[u] & 31) : 0u;
 #pragma unroll
-      for (int u = 0; u < EXP_UNROLL; ++u) {
+      for (int u = 0; u < U; ++u) {
         if (BYTES && oks[u]) bytes += 12;  // two hub indices, one bitmap word
         oks[u] = oks[u] && (w[u] & 1u);
       }
     } else {
 #pragma unroll
-      for (int u = 0; u < EXP_UNROLL; ++u)
+      for (int u = 0; u < U; ++u)
         if (oks[u]) oks[u] = has_edge<BYTES>(a.row, a.col, a.hub_of, a.hub_bits, a.W, mvs[u], bv, bytes);
     }
   }
@@ -145,6 +148,7 @@ struct JoinShared {
 // per SIMD, or fewer waves).
 template <bool BYTES, bool CYCLIC>
 __device__ __forceinline__ void join_body(const JoinArgs& a) {
+  constexpr int XU = CYCLIC ? EXP_UNROLL : TREE_UNROLL;
   extern __shared__ unsigned char lds_raw[];
   JoinShared& S = *reinterpret_cast<JoinShared*>(lds_raw);
   uint32_t* val = reinterpret_cast<uint32_t*>(lds_raw + ((sizeof(JoinShared) + 15) & ~(size_t)15));
@@ -292,7 +296,7 @@ __device__ __forceinline__ void join_body(const JoinArgs& a) {
         }
         __syncthreads();
 
-        // expand: EXP_UNROLL candidates per thread per step, list loads in
+        // expand: XU candidates per thread per step, list loads in
         // flight together; rows of the next level are appended per wave (one
         // LDS atomic per wave; row order inside a level does not matter).
         // The last level and the others are separate instantiations.
@@ -303,71 +307,73 @@ __device__ __forceinline__ void join_body(const JoinArgs& a) {
           constexpr bool FINAL = decltype(fin)::value;
           const unsigned long long* trm_n = FINAL ? a.terms + dn.term_base : nullptr;
           uint32_t f = 0;
-          for (; f < T; f += WROWS * EXP_UNROLL) {
+          for (; f < T; f += WROWS * XU) {
             if constexpr (!FINAL) {
               // every thread reads fill before any wave appends in this step
               const uint32_t fl = S.fill[nk];
               __syncthreads();
-              if (fl + WROWS * EXP_UNROLL > cap) break;  // uniform
+              if (fl + WROWS * XU > cap) break;  // uniform
             }
-            uint32_t mvs[EXP_UNROLL], is[EXP_UNROLL];
-            unsigned long long trm[EXP_UNROLL];
+            uint32_t mvs[XU], is[XU];
+            unsigned long long trm[XU];
             // wave w takes the step's candidates g0 .. g0 + 255 (lane l: g0 +
             // 64u + l), so one pair of searches per wave brackets the rows they
             // fall in and each candidate searches only inside the bracket
-            const uint32_t g0 = f + (tid / WAVE) * (WAVE * EXP_UNROLL);
-            uint32_t rlo = 0, rn = 1;
-            if (g0 < T) {  // uniform per wave
-              const uint32_t glast = min(g0 + WAVE * EXP_UNROLL - 1, T - 1);
-              uint32_t r = lane == 0 ? g0 : glast;
+            // (T > 0 here; indices past T are clamped to T - 1 throughout, so
+            // the bracket holds the row of every clamped index)
+            const uint32_t g0 = f + (tid / WAVE) * (WAVE * XU);
+            uint32_t rlo, rn;
+            {
+              const uint32_t glast = min(g0 + WAVE * XU - 1, T - 1);
+              uint32_t r = lane == 0 ? min(g0, T - 1) : glast;
               halving_search<1>(S.pre, uniform(nw), &r);
               rlo = uniform(__shfl(r, 0, WAVE));
               rn = uniform(__shfl(r, 1, WAVE)) - rlo + 1;
             }
-            // each candidate's row inside the bracket, the EXP_UNROLL
+            // each candidate's row inside the bracket, the XU
             // searches of a lane stepping together
-            uint32_t rb[EXP_UNROLL];
+            // a candidate index past T searches for T - 1 (its loads then
+            // re-read the last candidate's entry, in bounds); the candidate
+            // itself is masked out (is = WROWS)
+            uint32_t rb[XU];
 #pragma unroll
-            for (int u = 0; u < EXP_UNROLL; ++u) rb[u] = g0 + u * WAVE + lane;
-            halving_search<EXP_UNROLL>(S.pre + rlo, rn, rb);
-            // LDS reads of the EXP_UNROLL candidates are issued together
+            for (int u = 0; u < XU; ++u) rb[u] = min(g0 + u * WAVE + lane, T - 1);
+            halving_search<XU>(S.pre + rlo, rn, rb);
+            // LDS reads of the XU candidates are issued together
             // (row offsets, then the bound vertices the checks compare
             // with), so a lane waits once per kind rather than once per
             // candidate
-            uint32_t es[EXP_UNROLL];
+            uint32_t es[XU];
 #pragma unroll
-            for (int u = 0; u < EXP_UNROLL; ++u) {
+            for (int u = 0; u < XU; ++u) {
               const uint32_t g = g0 + u * WAVE + lane;
-              const uint32_t i = rlo + rb[u];  // a row of the window even when g >= T
+              const uint32_t i = rlo + rb[u];
               is[u] = g < T ? i : WROWS;
-              es[u] = S.delta[i] + g;
-              mvs[u] = 0;
+              es[u] = S.delta[i] + min(g, T - 1);
               trm[u] = 0;
             }
 #pragma unroll
-            for (int u = 0; u < EXP_UNROLL; ++u) {
-              if (is[u] != WROWS) {
-                mvs[u] = lst[es[u]];
-                // last level: the entry's hash term, computed once per list entry
-                if constexpr (FINAL) trm[u] = trm_n[es[u]];
-              }
+            for (int u = 0; u < XU; ++u) {
+              mvs[u] = lst[es[u]];
+              // last level: the entry's hash term, computed once per list entry
+              if constexpr (FINAL) trm[u] = trm_n[es[u]];
             }
-            bool oks[EXP_UNROLL];
+            bool oks[XU];
 #pragma unroll
-            for (int u = 0; u < EXP_UNROLL; ++u) oks[u] = is[u] != WROWS;
+            for (int u = 0; u < XU; ++u) oks[u] = is[u] != WROWS;
             // injectivity: only the positions bound to a vertex of the same
             // label that no edge test already separates
             for (uint64_t m = inj; m; m &= m - 1) {
               const uint32_t c = (uint32_t)(__ffsll((long long)m) - 1);
 #pragma unroll
-              for (int u = 0; u < EXP_UNROLL; ++u) oks[u] &= val[c * WROWS + (is[u] & (WROWS - 1))] != mvs[u];
+              for (int u = 0; u < XU; ++u) oks[u] &= val[c * WROWS + (is[u] & (WROWS - 1))] != mvs[u];
             }
             if constexpr (!FINAL) {
               // join-mode edges, then one append to the next level for the
-              // lane's EXP_UNROLL candidates together: one LDS atomic per
+              // lane's XU candidates together: one LDS atomic per
               // wave and step instead of one per candidate
 #pragma unroll
-              for (int u = 0; u < EXP_UNROLL; ++u) {
+              for (int u = 0; u < XU; ++u) {
                 if (oks[u]) {
                   if (BYTES) my_bytes += 4;  // the list entry
                   for (uint32_t b = bo; b < be && oks[u]; ++b)
@@ -375,10 +381,10 @@ __device__ __forceinline__ void join_body(const JoinArgs& a) {
                                              val[P.backs[b] * WROWS + is[u]], my_bytes);
                 }
               }
-              unsigned long long bal[EXP_UNROLL];
+              unsigned long long bal[XU];
               uint32_t nbal = 0;
 #pragma unroll
-              for (int u = 0; u < EXP_UNROLL; ++u) {
+              for (int u = 0; u < XU; ++u) {
                 bal[u] = __ballot(oks[u]);
                 nbal += (uint32_t)__popcll(bal[u]);
               }
@@ -386,7 +392,7 @@ __device__ __forceinline__ void join_body(const JoinArgs& a) {
               if (lane == 0 && nbal) wbase = atomicAdd(&S.fill[nk], nbal);
               wbase = __shfl(wbase, 0, WAVE);
 #pragma unroll
-              for (int u = 0; u < EXP_UNROLL; ++u) {
+              for (int u = 0; u < XU; ++u) {
                 if (oks[u]) {
                   const uint64_t r = wbase + __popcll(bal[u] & below);
                   for (uint32_t c = 0; c <= k; ++c) Ln[(uint64_t)c * cap + r] = val[c * WROWS + is[u]];
@@ -398,7 +404,7 @@ __device__ __forceinline__ void join_body(const JoinArgs& a) {
             }
             if constexpr (FINAL) {
 #pragma unroll
-              for (int u = 0; u < EXP_UNROLL; ++u) {
+              for (int u = 0; u < XU; ++u) {
                 const uint32_t i = is[u], mv = mvs[u];
                 bool ok = oks[u];
                 if (ok) {
@@ -435,44 +441,44 @@ __device__ __forceinline__ void join_body(const JoinArgs& a) {
             const unsigned long long srow = S.srow[i];
             unsigned long long rsum = 0;  // this lane's factors of the row's valid candidates
             if (lane == 0) atomicAdd(&S.long_cands, n_r);
-            for (uint32_t g0 = 0; g0 < n_r; g0 += WAVE * EXP_UNROLL) {
-              uint32_t mvs[EXP_UNROLL];
-              unsigned long long trm[EXP_UNROLL];
-              bool oks[EXP_UNROLL];
+            for (uint32_t g0 = 0; g0 < n_r; g0 += WAVE * XU) {
+              uint32_t mvs[XU];
+              unsigned long long trm[XU];
+              bool oks[XU];
+              // every lane loads (a lane past the row's end re-reads its
+              // last entry, n_r >= LONG_ROW > 0): no branch per candidate
 #pragma unroll
-              for (int u = 0; u < EXP_UNROLL; ++u) {
+              for (int u = 0; u < XU; ++u) {
                 const uint32_t g = g0 + u * WAVE + lane;
                 oks[u] = g < n_r;
-                mvs[u] = 0;
+                const uint32_t e = st_r + min(g, n_r - 1);
+                mvs[u] = lst[e];
+                // without back edges the hash factors load beside the entries
                 trm[u] = 0;
-                if (oks[u]) {
-                  mvs[u] = lst[st_r + g];
-                  // without back edges the hash factors load beside the entries
-                  if (bo == be) trm[u] = trm_n[st_r + g];
-                }
+                if (!CYCLIC || bo == be) trm[u] = trm_n[e];
               }
               for (uint64_t m = inj; m; m &= m - 1) {
                 const uint32_t vc = val[(uint32_t)(__ffsll((long long)m) - 1) * WROWS + i];
 #pragma unroll
-                for (int u = 0; u < EXP_UNROLL; ++u) oks[u] &= mvs[u] != vc;
+                for (int u = 0; u < XU; ++u) oks[u] &= mvs[u] != vc;
               }
 #pragma unroll
-              for (int u = 0; u < EXP_UNROLL; ++u)
+              for (int u = 0; u < XU; ++u)
                 if (BYTES && oks[u]) my_bytes += 4;  // the list entry
               if (CYCLIC && bo != be) {
                 // with back edges: the tests first, then the factors of the
                 // candidates that passed them
-                back_edges_of_row<BYTES>(a, P, val, bo, be, i, mvs, oks, my_bytes);
+                back_edges_of_row<BYTES, XU>(a, P, val, bo, be, i, mvs, oks, my_bytes);
 #pragma unroll
-                for (int u = 0; u < EXP_UNROLL; ++u)
+                for (int u = 0; u < XU; ++u)
                   if (oks[u]) trm[u] = trm_n[st_r + g0 + u * WAVE + lane];
               }
 #pragma unroll
-              for (int u = 0; u < EXP_UNROLL; ++u) {
+              for (int u = 0; u < XU; ++u) {
                 const bool ok = oks[u];
                 wave_cnt += __popcll(__ballot(ok));
+                rsum += ok ? trm[u] : 0ull;
                 if (ok) {
-                  rsum += trm[u];
                   if (a.emit) {
                     const unsigned long long pos = atomicAdd(a.emit_ctr, 1ULL);
                     if (pos < a.emit_cap) {
@@ -557,12 +563,17 @@ __device__ __forceinline__ void join_body(const JoinArgs& a) {
   }
 }
 
-// Seven waves per SIMD for tree plans: 72 VGPRs (on the earlier benchmark
-// faster than six waves with no spill); five for cyclic plans, whose
-// batched back-edge tests spill heavily at seven (R-MAT cycles and cliques
-// 1.7x faster than with the per-candidate tests).
+// Eight waves per SIMD for tree plans, three candidates per thread: 64
+// VGPRs with 4 spilled.  On the 1M Chung-Lu mix (join per batch, learned
+// order) 438 ms, against 445 ms with four candidates at eight waves (7
+// spilled), 457 ms with four at seven waves (no spill), 452 ms with two at
+// eight and 483-571 ms with six or eight candidates at five to seven waves:
+// the waves in flight hide the list loads better than more loads per wave.
+// Five waves for cyclic plans, whose batched back-edge tests spill heavily
+// beyond (R-MAT 22 cycles and cliques 2.05 s per batch at five, 2.07 s at
+// six, 2.33 s at eight).
 template <bool BYTES>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) join_tree_kernel(JoinArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) join_tree_kernel(JoinArgs a) {
   join_body<BYTES, false>(a);
 }
 
