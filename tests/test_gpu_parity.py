@@ -214,6 +214,32 @@ def test_cached_graphs_replay_and_update(ctx):
     assert r4 - r3 >= 2  # at least the second run relaunches both phases
 
 
+def test_wide_tree_and_cyclic_queries_share_the_level_budget(ctx):
+    """64-vertex paths (tree plans) and cycles (cyclic plans) in one batch on
+    a 64-vertex cycle: the two join launches run side by side, and at 2048
+    rows per level their level buffers (16.5 MB per workgroup at 64 query
+    vertices, 2048 workgroups' worth of tasks) exceed the 16 GiB budget, so
+    both launches drop workgroups in proportion. Closed forms: a path or the
+    whole cycle has 2n placements, a shorter cycle none; every hash equals
+    the enumerator's, and the cut launches equal the batch without it."""
+    g = G.cycle(64)
+    rg = ctx.upload(g)
+    qs = [G.path_query(64)] * 8 + [G.cycle_query(64)] * 8 + [G.path_query(33), G.cycle_query(32)]
+    res = rg.match_batch(qs, join_cap=2048)
+    assert [r.count for r in res] == [128] * 17 + [0]
+    want = {}
+    for q, r in zip(qs, res):
+        key = (len(q.labels), len(q.src))
+        if key not in want:
+            o = O.o1_enumerate(g, q)
+            want[key] = (o["count"], o["hash"])
+        assert (r.count, r.hash) == want[key]
+    small = rg.match_batch(qs[16:], join_cap=2048)
+    assert [(r.count, r.hash) for r in small] == [(r.count, r.hash) for r in res[16:]]
+    assert [(r.count, r.hash) for r in rg.match_batch(qs, join_cap=2048, use_graph=True)] == \
+        [(r.count, r.hash) for r in res]
+
+
 def test_small_join_buffers_force_depth_first_spills(ctx):
     """A join buffer of 512 rows forces many partial windows and re-entries."""
     g = G.small_graph(10, n=2000, m=16000, labels=1, kind="er")
