@@ -4,11 +4,12 @@
 // compaction (util/prefix_scan.cu:158-427, util/compact_scan.cu:7-70).
 //
 // Candidate sets are bitmaps here, so "collect" is a popcount scan.  Each
-// bitmap row is cut into segments of SEG_WORDS words, one workgroup per
-// (row, segment), so that a batch of a few hundred rows still fills 256 CUs:
-// a first pass popcounts every segment (and sums a row's segments into its
-// candidate count), the second gives a segment its row-relative base from
-// the counts of the segments before it and compacts it in one pass (each
+// bitmap row is cut into segments of SEG_WORDS words, so that a batch of a
+// few hundred rows still fills 256 CUs: a first pass popcounts every
+// segment (one workgroup per (row, segment); a row's segments summed into
+// its candidate count), the second (one workgroup per four consecutive
+// segments of a row) takes the group's row-relative base from the counts of
+// the segments before it and compacts the four one after the other (each
 // thread owns eight consecutive words, one block scan places them), ids
 // staged in LDS and written back coalesced.  The same pass writes the rank
 // index (set bits before each 256-bit block, row-relative) the join uses to
@@ -47,15 +48,29 @@ __global__ void row_totals_kernel(const uint32_t* __restrict__ segc, uint32_t ns
   counts[r] = t;
 }
 
-// A workgroup compacts one (row, segment) in one pass: thread t owns the
-// eight consecutive words 8t..8t+7 of the segment (eight loads in flight),
-// one block scan of the threads' popcounts gives every thread its output
-// position and its 256-bit block's rank-index entry, and the ids go through
-// LDS when the segment's fit (written back coalesced), else straight from
-// the threads.
+// A workgroup compacts COMPACT_GROUP consecutive segments of one row, one
+// after the other: thread t owns the eight consecutive words 8t..8t+7 of a
+// segment (eight loads in flight, and the next segment's eight issued
+// before the current one is scanned), one block scan of the threads'
+// popcounts gives every thread its output position and its 256-bit block's
+// rank-index entry, and the ids go through LDS (in rounds of STAGE_IDS
+// output positions when the segment holds more) and are written back
+// coalesced.  The row's set bits before the group come from the segment
+// popcounts; within the group the workgroup carries them.  Measured per
+// batch on the 1M Chung-Lu mix: 0.29 ms for one segment per workgroup with
+// dense segments written by the threads, 0.27 ms for four segments, 0.23 ms
+// for four with dense segments staged in rounds (stages of 4096 and 2048
+// ids 0.25 and 0.41 ms; sixteen segments per workgroup 0.31 ms).
 constexpr uint32_t CW = 8;             // words per thread (one rank-index block)
-constexpr uint32_t STAGE_IDS = 8192;   // ids staged in LDS per segment
+constexpr uint32_t STAGE_IDS = 8192;   // ids staged in LDS per round
+constexpr uint32_t COMPACT_GROUP = 4;  // segments per workgroup
 static_assert(SEG_WORDS == 256 * CW, "a segment is one word block per thread");
+
+__device__ __forceinline__ void load_words(const uint32_t* __restrict__ bm, uint32_t W, uint32_t w0,
+                                           uint32_t (&w)[CW]) {
+#pragma unroll
+  for (uint32_t j = 0; j < CW; ++j) w[j] = w0 + j < W ? bm[w0 + j] : 0u;
+}
 
 __global__ void __launch_bounds__(256) compact_rows_kernel(const uint32_t* __restrict__ bitmaps, uint32_t W,
                                                            uint32_t NB, uint32_t nseg,
@@ -66,41 +81,70 @@ __global__ void __launch_bounds__(256) compact_rows_kernel(const uint32_t* __res
   __shared__ uint32_t stage[STAGE_IDS];
   __shared__ uint32_t tmp[8];
   __shared__ uint32_t rel;
-  const uint32_t r = blockIdx.x / nseg, sg = blockIdx.x % nseg;
+  const uint32_t ngrp = (nseg + COMPACT_GROUP - 1) / COMPACT_GROUP;
+  const uint32_t r = blockIdx.x / ngrp, sg0 = (blockIdx.x % ngrp) * COMPACT_GROUP;
+  const uint32_t sg1 = min(nseg, sg0 + COMPACT_GROUP);
   const uint32_t* bm = bitmaps + (size_t)r * W;
   uint32_t* bp = bprefix + (size_t)r * NB;
   uint32_t* out = clist + coff[r];
-  if (threadIdx.x < WAVE) {  // set bits of the row's earlier segments
+  if (threadIdx.x < WAVE) {  // set bits of the row's segments before the group
     uint32_t x = 0;
-    for (uint32_t i = threadIdx.x; i < sg; i += WAVE) x += segc[(size_t)r * nseg + i];
+    for (uint32_t i = threadIdx.x; i < sg0; i += WAVE) x += segc[(size_t)r * nseg + i];
     x = wave_sum(x);
     if (threadIdx.x == 0) rel = x;
   }
-  const uint32_t w0 = sg * SEG_WORDS + threadIdx.x * CW;
-  uint32_t w[CW], c = 0;
+  uint32_t w[CW];
+  load_words(bm, W, sg0 * SEG_WORDS + threadIdx.x * CW, w);
+  for (uint32_t sg = sg0; sg < sg1; ++sg) {
+    const uint32_t w0 = sg * SEG_WORDS + threadIdx.x * CW;
+    uint32_t nx[CW] = {};
+    if (sg + 1 < sg1) load_words(bm, W, w0 + SEG_WORDS, nx);
+    uint32_t c = 0;
 #pragma unroll
-  for (uint32_t j = 0; j < CW; ++j) {
-    w[j] = w0 + j < W ? bm[w0 + j] : 0u;
-    c += __popc(w[j]);
+    for (uint32_t j = 0; j < CW; ++j) c += __popc(w[j]);
+    uint32_t total;
+    const uint32_t ex = block_excl_scan(c, tmp, &total);  // its barriers also publish rel
+    const uint32_t carry = rel;
+    if (w0 < W) bp[w0 >> 3] = carry + ex;
+    if (total <= STAGE_IDS) {
+      uint32_t pos = ex;
+#pragma unroll
+      for (uint32_t j = 0; j < CW; ++j)
+        for (uint32_t x = w[j]; x; x &= x - 1) stage[pos++] = (w0 + j) * 32 + (uint32_t)(__ffs(x) - 1);
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) out[carry + i] = stage[i];
+    } else {
+      // a dense segment: its ids go through LDS in rounds of STAGE_IDS
+      // output positions, each thread writing the ids of its words that
+      // fall in the round
+      for (uint32_t base = 0; base < total; base += STAGE_IDS) {
+        const uint32_t lim = base + STAGE_IDS;
+        if (ex < lim && ex + c > base) {
+          uint32_t pos = ex;
+#pragma unroll
+          for (uint32_t j = 0; j < CW; ++j) {
+            const uint32_t pc = __popc(w[j]);
+            if (pos + pc > base && pos < lim) {
+              for (uint32_t x = w[j]; x; x &= x - 1, ++pos) {
+                if (pos >= base && pos < lim) stage[pos - base] = (w0 + j) * 32 + (uint32_t)(__ffs(x) - 1);
+              }
+            } else {
+              pos += pc;
+            }
+          }
+        }
+        __syncthreads();
+        const uint32_t n = min(STAGE_IDS, total - base);
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) out[carry + base + i] = stage[i];
+        __syncthreads();
+      }
+    }
+    if (sg == nseg - 1 && threadIdx.x == 0) bp[NB - 1] = carry + total;
+    __syncthreads();  // stage and rel are read; the next segment may reuse them
+    if (threadIdx.x == 0) rel = carry + total;
+#pragma unroll
+    for (uint32_t j = 0; j < CW; ++j) w[j] = nx[j];
   }
-  uint32_t total;
-  const uint32_t ex = block_excl_scan(c, tmp, &total);  // its barriers also publish rel
-  const uint32_t carry = rel;
-  if (w0 < W) bp[w0 >> 3] = carry + ex;
-  if (total <= STAGE_IDS) {
-    uint32_t pos = ex;
-#pragma unroll
-    for (uint32_t j = 0; j < CW; ++j)
-      for (uint32_t x = w[j]; x; x &= x - 1) stage[pos++] = (w0 + j) * 32 + (uint32_t)(__ffs(x) - 1);
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) out[carry + i] = stage[i];
-  } else {
-    uint32_t pos = carry + ex;
-#pragma unroll
-    for (uint32_t j = 0; j < CW; ++j)
-      for (uint32_t x = w[j]; x; x &= x - 1) out[pos++] = (w0 + j) * 32 + (uint32_t)(__ffs(x) - 1);
-  }
-  if (sg == nseg - 1 && threadIdx.x == 0) bp[NB - 1] = carry + total;
 }
 
 // Reduce-then-scan for generic u32 arrays (wrapping u32 values, u64 total).
@@ -183,7 +227,8 @@ void launch_compact_rows(const DevGraph& g, const uint32_t* bitmaps, uint32_t nr
                          const uint64_t* coff, uint32_t* clist, uint32_t* bprefix, hipStream_t s) {
   if (!nrows) return;
   const uint32_t nseg = segs_of(g.W);
-  hipLaunchKernelGGL(compact_rows_kernel, dim3(nrows * nseg), dim3(256), 0, s, bitmaps, g.W, g.NB, nseg, segc, coff,
+  const uint32_t ngrp = (nseg + COMPACT_GROUP - 1) / COMPACT_GROUP;
+  hipLaunchKernelGGL(compact_rows_kernel, dim3(nrows * ngrp), dim3(256), 0, s, bitmaps, g.W, g.NB, nseg, segc, coff,
                      clist, bprefix);
 }
 
