@@ -155,11 +155,20 @@ __global__ void __launch_bounds__(256) scan_reduce_kernel(const uint32_t* __rest
                                                           unsigned long long* __restrict__ sums) {
   __shared__ unsigned long long part[4];
   const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
-  unsigned long long s = 0;
-  for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
-    const uint64_t i = base + (uint64_t)j * 256 + threadIdx.x;
-    if (i < n) s += in[i];
+  uint32_t v[SCAN_ITEMS];
+  if (base + SCAN_TILE <= n) {  // a whole tile: every load unconditional, all in flight together
+#pragma unroll
+    for (uint32_t j = 0; j < SCAN_ITEMS; ++j) v[j] = in[base + (uint64_t)j * 256 + threadIdx.x];
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
+      const uint64_t i = base + (uint64_t)j * 256 + threadIdx.x;
+      v[j] = i < n ? in[i] : 0u;
+    }
   }
+  unsigned long long s = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < SCAN_ITEMS; ++j) s += v[j];
   s = wave_sum64(s);
   if (lane_id() == 0) part[threadIdx.x / WAVE] = s;
   __syncthreads();
@@ -179,23 +188,52 @@ __global__ void scan_sums_kernel(unsigned long long* __restrict__ sums, uint32_t
   }
 }
 
+// A tile in striped order: thread t holds values j * 256 + t (every load
+// and store instruction moves 256 contiguous bytes per wave).  Each wave
+// scans its 64 values of every row j with shuffles, one wave scans the 64
+// (row, wave) totals in element order, and each value gets the tile's base,
+// its (row, wave) offset and its place in its wave's scan: two barriers per
+// tile.
 __global__ void __launch_bounds__(256) scan_tile_kernel(const uint32_t* in, uint32_t* out, uint64_t n,
                                                         const unsigned long long* __restrict__ sums) {
-  __shared__ uint32_t tmp[8];
-  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+  __shared__ uint32_t wt[SCAN_ITEMS * 4];  // (row j, wave w) totals at j * 4 + w, then their offsets
+  const uint32_t lane = lane_id(), w = threadIdx.x / WAVE;
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+  const bool whole = base + SCAN_TILE <= n;
   uint32_t v[SCAN_ITEMS];
-  uint32_t s = 0;
-  for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
-    const uint64_t i = base + j;
-    v[j] = i < n ? in[i] : 0u;
-    s += v[j];
+  if (whole) {
+#pragma unroll
+    for (uint32_t j = 0; j < SCAN_ITEMS; ++j) v[j] = in[base + (uint64_t)j * 256 + threadIdx.x];
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
+      const uint64_t i = base + (uint64_t)j * 256 + threadIdx.x;
+      v[j] = i < n ? in[i] : 0u;
+    }
   }
-  uint32_t total;
-  uint32_t ex = block_excl_scan(s, tmp, &total) + (uint32_t)sums[blockIdx.x];
+#pragma unroll
   for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
-    const uint64_t i = base + j;
-    if (i < n) out[i] = ex;
-    ex += v[j];
+    const uint32_t incl = wave_incl_scan(v[j]);
+    if (lane == WAVE - 1) wt[j * 4 + w] = incl;
+    v[j] = incl - v[j];  // exclusive within the wave's 64 values of row j
+  }
+  __syncthreads();
+  static_assert(SCAN_ITEMS * 4 == WAVE, "one wave scans the (row, wave) totals");
+  if (w == 0) {
+    const uint32_t x = wt[lane];
+    wt[lane] = wave_incl_scan(x) - x;
+  }
+  __syncthreads();
+  const uint32_t off = (uint32_t)sums[blockIdx.x];
+  if (whole) {
+#pragma unroll
+    for (uint32_t j = 0; j < SCAN_ITEMS; ++j) out[base + (uint64_t)j * 256 + threadIdx.x] = off + wt[j * 4 + w] + v[j];
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
+      const uint64_t i = base + (uint64_t)j * 256 + threadIdx.x;
+      if (i < n) out[i] = off + wt[j * 4 + w] + v[j];
+    }
   }
 }
 
