@@ -49,27 +49,39 @@ __global__ void row_totals_kernel(const uint32_t* __restrict__ segc, uint32_t ns
 }
 
 // A workgroup compacts COMPACT_GROUP consecutive segments of one row, one
-// after the other: thread t owns the eight consecutive words 8t..8t+7 of a
-// segment (eight loads in flight, and the next segment's eight issued
-// before the current one is scanned), one block scan of the threads'
-// popcounts gives every thread its output position and its 256-bit block's
-// rank-index entry, and the ids go through LDS (in rounds of STAGE_IDS
-// output positions when the segment holds more) and are written back
-// coalesced.  The row's set bits before the group come from the segment
-// popcounts; within the group the workgroup carries them.  Measured per
-// batch on the 1M Chung-Lu mix: 0.29 ms for one segment per workgroup with
-// dense segments written by the threads, 0.27 ms for four segments, 0.23 ms
-// for four with dense segments staged in rounds (stages of 4096 and 2048
-// ids 0.25 and 0.41 ms; sixteen segments per workgroup 0.31 ms).
-constexpr uint32_t CW = 8;             // words per thread (one rank-index block)
+// after the other: thread t holds the words j * 256 + t of a segment (eight
+// loads in flight, and the next segment's eight issued before the current
+// one is scanned); a wave scan per row j of its words' popcounts and one
+// scan of the (row, wave) totals give every word its output position, the
+// first word of each 256-bit block writes its rank-index entry, and the ids
+// go through LDS (in rounds of STAGE_IDS output positions when the segment
+// holds more) and are written back coalesced.  The row's set bits before
+// the group come from the segment popcounts; within the group the
+// workgroup carries them.  Measured per batch on the 1M Chung-Lu mix: 0.29
+// ms for one segment per workgroup, eight consecutive words per thread and
+// dense segments written by the threads; 0.27 ms for four segments; 0.23 ms
+// with dense segments staged in rounds (stages of 4096 and 2048 ids 0.25
+// and 0.41 ms; sixteen segments per workgroup 0.31 ms); 0.19 ms with the
+// words in striped order.
+constexpr uint32_t CW = 8;             // words per thread
 constexpr uint32_t STAGE_IDS = 8192;   // ids staged in LDS per round
 constexpr uint32_t COMPACT_GROUP = 4;  // segments per workgroup
-static_assert(SEG_WORDS == 256 * CW, "a segment is one word block per thread");
+static_assert(SEG_WORDS == 256 * CW && CW * 4 < WAVE, "a segment is CW words per thread; one wave scans the totals");
 
-__device__ __forceinline__ void load_words(const uint32_t* __restrict__ bm, uint32_t W, uint32_t w0,
+// Words of a segment in striped order: thread t holds words j * 256 + t,
+// j < CW (every load instruction reads 256 contiguous bytes per wave).
+__device__ __forceinline__ void load_words(const uint32_t* __restrict__ bm, uint32_t W, uint32_t s0,
                                            uint32_t (&w)[CW]) {
+  if (s0 + SEG_WORDS <= W) {
 #pragma unroll
-  for (uint32_t j = 0; j < CW; ++j) w[j] = w0 + j < W ? bm[w0 + j] : 0u;
+    for (uint32_t j = 0; j < CW; ++j) w[j] = bm[s0 + j * 256 + threadIdx.x];
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < CW; ++j) {
+      const uint32_t i = s0 + j * 256 + threadIdx.x;
+      w[j] = i < W ? bm[i] : 0u;
+    }
+  }
 }
 
 __global__ void __launch_bounds__(256) compact_rows_kernel(const uint32_t* __restrict__ bitmaps, uint32_t W,
@@ -79,8 +91,9 @@ __global__ void __launch_bounds__(256) compact_rows_kernel(const uint32_t* __res
                                                            uint32_t* __restrict__ clist,
                                                            uint32_t* __restrict__ bprefix) {
   __shared__ uint32_t stage[STAGE_IDS];
-  __shared__ uint32_t tmp[8];
+  __shared__ uint32_t wt[CW * 4 + 1];  // (row j, wave) popcount totals, then their offsets; [CW * 4] = segment total
   __shared__ uint32_t rel;
+  const uint32_t lane = lane_id(), wv = threadIdx.x / WAVE;
   const uint32_t ngrp = (nseg + COMPACT_GROUP - 1) / COMPACT_GROUP;
   const uint32_t r = blockIdx.x / ngrp, sg0 = (blockIdx.x % ngrp) * COMPACT_GROUP;
   const uint32_t sg1 = min(nseg, sg0 + COMPACT_GROUP);
@@ -94,54 +107,60 @@ __global__ void __launch_bounds__(256) compact_rows_kernel(const uint32_t* __res
     if (threadIdx.x == 0) rel = x;
   }
   uint32_t w[CW];
-  load_words(bm, W, sg0 * SEG_WORDS + threadIdx.x * CW, w);
+  load_words(bm, W, sg0 * SEG_WORDS, w);
   for (uint32_t sg = sg0; sg < sg1; ++sg) {
-    const uint32_t w0 = sg * SEG_WORDS + threadIdx.x * CW;
+    const uint32_t s0 = sg * SEG_WORDS;
     uint32_t nx[CW] = {};
-    if (sg + 1 < sg1) load_words(bm, W, w0 + SEG_WORDS, nx);
-    uint32_t c = 0;
+    if (sg + 1 < sg1) load_words(bm, W, s0 + SEG_WORDS, nx);
+    // positions: per row j a wave scan of the 64 words' popcounts, then one
+    // scan of the (row, wave) totals in word order
+    uint32_t pos[CW];
 #pragma unroll
-    for (uint32_t j = 0; j < CW; ++j) c += __popc(w[j]);
-    uint32_t total;
-    const uint32_t ex = block_excl_scan(c, tmp, &total);  // its barriers also publish rel
-    const uint32_t carry = rel;
-    if (w0 < W) bp[w0 >> 3] = carry + ex;
-    if (total <= STAGE_IDS) {
-      uint32_t pos = ex;
-#pragma unroll
-      for (uint32_t j = 0; j < CW; ++j)
-        for (uint32_t x = w[j]; x; x &= x - 1) stage[pos++] = (w0 + j) * 32 + (uint32_t)(__ffs(x) - 1);
-      __syncthreads();
-      for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) out[carry + i] = stage[i];
-    } else {
-      // a dense segment: its ids go through LDS in rounds of STAGE_IDS
-      // output positions, each thread writing the ids of its words that
-      // fall in the round
-      for (uint32_t base = 0; base < total; base += STAGE_IDS) {
-        const uint32_t lim = base + STAGE_IDS;
-        if (ex < lim && ex + c > base) {
-          uint32_t pos = ex;
-#pragma unroll
-          for (uint32_t j = 0; j < CW; ++j) {
-            const uint32_t pc = __popc(w[j]);
-            if (pos + pc > base && pos < lim) {
-              for (uint32_t x = w[j]; x; x &= x - 1, ++pos) {
-                if (pos >= base && pos < lim) stage[pos - base] = (w0 + j) * 32 + (uint32_t)(__ffs(x) - 1);
-              }
-            } else {
-              pos += pc;
-            }
-          }
-        }
-        __syncthreads();
-        const uint32_t n = min(STAGE_IDS, total - base);
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) out[carry + base + i] = stage[i];
-        __syncthreads();
-      }
+    for (uint32_t j = 0; j < CW; ++j) {
+      const uint32_t c = __popc(w[j]);
+      const uint32_t incl = wave_incl_scan(c);
+      if (lane == WAVE - 1) wt[j * 4 + wv] = incl;
+      pos[j] = incl - c;
     }
+    __syncthreads();  // also publishes rel
+    if (wv == 0) {
+      const uint32_t x = lane < CW * 4 ? wt[lane] : 0u;
+      const uint32_t incl = wave_incl_scan(x);
+      if (lane < CW * 4) wt[lane] = incl - x;
+      if (lane == CW * 4 - 1) wt[CW * 4] = incl;
+    }
+    __syncthreads();
+    const uint32_t carry = rel, total = wt[CW * 4];
+#pragma unroll
+    for (uint32_t j = 0; j < CW; ++j) {
+      pos[j] += wt[j * 4 + wv];
+      const uint32_t wi = s0 + j * 256 + threadIdx.x;
+      if ((threadIdx.x & 7) == 0 && wi < W) bp[wi >> 3] = carry + pos[j];  // first word of its 256-bit block
+    }
+    // ids through LDS, in rounds of STAGE_IDS output positions when the
+    // segment holds more, written back coalesced
+    for (uint32_t base = 0; base < total; base += STAGE_IDS) {
+      const uint32_t lim = base + STAGE_IDS;
+#pragma unroll
+      for (uint32_t j = 0; j < CW; ++j) {
+        const uint32_t pc = __popc(w[j]);
+        if (pos[j] + pc > base && pos[j] < lim) {
+          const uint32_t id0 = (s0 + j * 256 + threadIdx.x) * 32;
+          uint32_t p = pos[j];
+          for (uint32_t x = w[j]; x; x &= x - 1, ++p)
+            if (p >= base && p < lim) stage[p - base] = id0 + (uint32_t)(__ffs(x) - 1);
+        }
+      }
+      __syncthreads();
+      const uint32_t n = min(STAGE_IDS, total - base);
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) out[carry + base + i] = stage[i];
+      __syncthreads();
+    }
+    // every thread has read wt (and stage): the last round ends with a
+    // barrier, and a segment without set bits has none
+    if (total == 0) __syncthreads();
     if (sg == nseg - 1 && threadIdx.x == 0) bp[NB - 1] = carry + total;
-    __syncthreads();  // stage and rel are read; the next segment may reuse them
-    if (threadIdx.x == 0) rel = carry + total;
+    if (threadIdx.x == 0) rel = carry + total;  // read after the next segment's first barrier
 #pragma unroll
     for (uint32_t j = 0; j < CW; ++j) w[j] = nx[j];
   }
