@@ -27,12 +27,20 @@ __global__ void __launch_bounds__(256) seg_popcount_kernel(const uint32_t* __res
   const uint32_t r = blockIdx.x / nseg, sg = blockIdx.x % nseg;
   const uint32_t* bm = bitmaps + (size_t)r * W;
   const uint32_t base = sg * SEG_WORDS;
+  uint32_t v[SEG_WORDS / 256];
+  if (base + SEG_WORDS <= W) {  // a whole segment: the loads unconditional, all in flight together
+#pragma unroll
+    for (uint32_t j = 0; j < SEG_WORDS / 256; ++j) v[j] = bm[base + j * 256 + threadIdx.x];
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < SEG_WORDS / 256; ++j) {
+      const uint32_t i = base + j * 256 + threadIdx.x;
+      v[j] = i < W ? bm[i] : 0u;
+    }
+  }
   uint32_t c = 0;
 #pragma unroll
-  for (uint32_t j = 0; j < SEG_WORDS / 256; ++j) {
-    const uint32_t i = base + j * 256 + threadIdx.x;
-    if (i < W) c += __popc(bm[i]);
-  }
+  for (uint32_t j = 0; j < SEG_WORDS / 256; ++j) c += __popc(v[j]);
   c = wave_sum(c);
   if (lane_id() == 0) part[threadIdx.x / WAVE] = c;
   __syncthreads();
