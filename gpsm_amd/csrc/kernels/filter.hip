@@ -260,16 +260,18 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
         // refine needs the neighbour ids, explore only their keys
         const uint32_t* __restrict__ src = type == OP_REFINE ? col : ckey;
         uint32_t mvs[UNROLL], cis[UNROLL], es[UNROLL];
+        // an entry index past T reads entry T - 1 (in bounds) and is masked
+        // out (cis = WAVE): every lane loads, no branch per entry
+        uint32_t gi[UNROLL];
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) cis[u] = u * WAVE + lane;
+        for (int u = 0; u < UNROLL; ++u) cis[u] = gi[u] = min(u * WAVE + lane, T - 1);
         halving_search<UNROLL>(L.pre, WAVE, cis);
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) es[u] = L.base[cis[u]] + (u * WAVE + lane - L.pre[cis[u]]);
+        for (int u = 0; u < UNROLL; ++u) es[u] = L.base[cis[u]] + (gi[u] - L.pre[cis[u]]);
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
-          mvs[u] = 0;
-          if (u * WAVE + lane < T) mvs[u] = src[es[u]];
-          else cis[u] = WAVE;
+          mvs[u] = src[es[u]];
+          if (u * WAVE + lane >= T) cis[u] = WAVE;
         }
         // constraints each entry meets, the UNROLL entries of a lane side by
         // side: their candidates' known bits read together, then per
@@ -336,20 +338,17 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
         wave_sync();
         // pass 2: every matching data neighbour of a surviving prefix enters C(v_i)
         for (uint32_t f = 0; f < T; f += UNROLL * WAVE) {
-          uint32_t mvs[UNROLL], ks[UNROLL], cis[UNROLL], es[UNROLL];
+          uint32_t mvs[UNROLL], ks[UNROLL], cis[UNROLL], es[UNROLL], gi[UNROLL];
 #pragma unroll
-          for (int u = 0; u < UNROLL; ++u) cis[u] = f + u * WAVE + lane;
+          for (int u = 0; u < UNROLL; ++u) cis[u] = gi[u] = min(f + u * WAVE + lane, T - 1);
           halving_search<UNROLL>(L.pre, WAVE, cis);
 #pragma unroll
-          for (int u = 0; u < UNROLL; ++u) es[u] = L.base[cis[u]] + (f + u * WAVE + lane - L.pre[cis[u]]);
+          for (int u = 0; u < UNROLL; ++u) es[u] = L.base[cis[u]] + (gi[u] - L.pre[cis[u]]);
 #pragma unroll
           for (int u = 0; u < UNROLL; ++u) {
-            if (f + u * WAVE + lane < T) {
-              mvs[u] = col[es[u]];
-              ks[u] = ckey[es[u]];
-            } else {
-              cis[u] = WAVE;
-            }
+            mvs[u] = col[es[u]];
+            ks[u] = ckey[es[u]];
+            if (f + u * WAVE + lane >= T) cis[u] = WAVE;
           }
           // the UNROLL entries of a lane side by side: their prefix limits
           // read together, then per neighbour v_i the bitmap words of the
