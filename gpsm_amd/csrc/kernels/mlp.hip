@@ -3,10 +3,14 @@
 // counterpart (the reference orders joins with a fixed heuristic,
 // match/Match.cu:1599-1661).
 //
-// One wave handles 32 samples.  Both hidden layers and all three weight
-// gradients that are GEMMs run on v_mfma_f32_32x32x16_bf16; the output
-// layer (a 128-wide dot product per sample) and the bias gradients are FP32
-// FMAs.  BF16 rounding points (round to nearest even): inputs, weights,
+// One wave handles 32 samples of the forward pass and of the backward pass
+// down to the deltas; the two weight gradients that are GEMMs over the
+// batch (W2's and W1's) run in a second kernel as split-K GEMMs from
+// operands the first writes feature-major, so that each workgroup adds its
+// partial gradients to memory once instead of once per 32 samples.  Both
+// hidden layers, the backward product d2 W2 and the weight gradients run on
+// v_mfma_f32_32x32x16_bf16; the output layer (a 128-wide dot product per
+// sample) and the bias gradients are FP32 FMAs.  BF16 rounding points (round to nearest even): inputs, weights,
 // hidden activations h1 (operand of layer 2), and the deltas d2, d1 where
 // they are GEMM operands.  Accumulation order differs from a sequential sum,
 // so results agree with an exact evaluation to FP32 rounding.
@@ -17,6 +21,7 @@
 //   C (32 x 32): lane l, register r holds row 8*(r/4) + 4*(l/32) + r%4, col l%32
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -34,6 +39,7 @@ constexpr int OW1 = 0, OB1 = H * F, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3 = OB2 
 constexpr int NP = OB3 + 1;
 constexpr int XS = F + 8;  // LDS row strides (bf16), padded against bank conflicts
 constexpr int HS = H + 8;
+static_assert(XS % 8 == 0 && HS % 8 == 0 && (32 * XS) % 8 == 0 && (32 * HS) % 8 == 0, "16-byte aligned operand runs");
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -56,6 +62,11 @@ __device__ __forceinline__ f32x16 mfma(const bf16x8& a, const bf16x8& b, const f
 
 __device__ __forceinline__ uint32_t c_row(uint32_t r, uint32_t lane) { return 8 * (r / 4) + 4 * (lane / 32) + (r % 4); }
 
+// Eight consecutive BF16 values (an MFMA operand's k run) in one 16-byte
+// read: every use is 16-byte aligned (row strides XS, HS, F and H are
+// multiples of 8 values, k offsets multiples of 8).
+__device__ __forceinline__ bf16x8 ld8(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
 struct Weights {
   const __bf16* w1;   // [H][F]
   const __bf16* w2;   // [H][H]
@@ -64,7 +75,7 @@ struct Weights {
   const float* norm;  // input standardisation: mu[F], inv_sd[F], applied in FP32 before BF16 rounding
 };
 
-struct TileLds {
+struct alignas(16) TileLds {  // 16-byte aligned rows for ld8 (each member a multiple of 16 bytes)
   __bf16 xs[32 * XS];
   __bf16 h1[32 * HS];
   __bf16 d[32 * HS];
@@ -86,14 +97,8 @@ __device__ void forward_tile(const Weights& W, const float* __restrict__ x, uint
   // layer 1: z1 = x W1^T + b1, h1 = bf16(relu(z1))
   for (int t = 0; t < H / 32; ++t) {
     f32x16 acc = {};
-    for (int s = 0; s < F / 16; ++s) {
-      bf16x8 a, b;
-      for (int j = 0; j < 8; ++j) {
-        a[j] = L.xs[rl * XS + 16 * s + kh + j];
-        b[j] = W.w1[(32 * t + rl) * F + 16 * s + kh + j];
-      }
-      acc = mfma(a, b, acc);
-    }
+    for (int s = 0; s < F / 16; ++s)
+      acc = mfma(ld8(&L.xs[rl * XS + 16 * s + kh]), ld8(&W.w1[(32 * t + rl) * F + 16 * s + kh]), acc);
     const uint32_t colh = 32 * t + rl;
     const float bias = W.master[OB1 + colh];
     for (int r = 0; r < 16; ++r) L.h1[c_row(r, lane) * HS + colh] = to_bf(fmaxf(acc[r] + bias, 0.0f));
@@ -102,14 +107,8 @@ __device__ void forward_tile(const Weights& W, const float* __restrict__ x, uint
   // layer 2: z2 = h1 W2^T + b2
   for (int t = 0; t < H / 32; ++t) {
     f32x16 acc = {};
-    for (int s = 0; s < H / 16; ++s) {
-      bf16x8 a, b;
-      for (int j = 0; j < 8; ++j) {
-        a[j] = L.h1[rl * HS + 16 * s + kh + j];
-        b[j] = W.w2[(32 * t + rl) * H + 16 * s + kh + j];
-      }
-      acc = mfma(a, b, acc);
-    }
+    for (int s = 0; s < H / 16; ++s)
+      acc = mfma(ld8(&L.h1[rl * HS + 16 * s + kh]), ld8(&W.w2[(32 * t + rl) * H + 16 * s + kh]), acc);
     const float bias = W.master[OB2 + 32 * t + rl];
     for (int r = 0; r < 16; ++r) acc[r] += bias;
     z2[t] = acc;
@@ -139,11 +138,32 @@ __global__ void __launch_bounds__(64) mlp_forward_kernel(Weights W, const float*
   if (lane < 32 && b0 + lane < B) y[b0 + lane] = L.y[lane];
 }
 
-// Forward + backward of 32 samples of the loss 0.5*mean((y-t)^2); gradients
-// are added atomically into grad (FP32, parameter order).
+// The operands of the weight-gradient GEMMs, feature-major over the
+// batch padded to Bp samples (a multiple of 32): D2T[H][Bp] (output-layer
+// deltas d2), H1T[H][Bp] (hidden activations h1), D1T[H][Bp] (first-layer
+// deltas d1), XT[F][Bp] (standardised inputs), all BF16, one after the
+// other in act.
+__device__ __forceinline__ void put_rows(__bf16* __restrict__ dst, uint32_t Bp, uint32_t b0, const __bf16* src,
+                                         uint32_t stride, uint32_t nrows) {
+  // row i of the tile's feature-major block: the 32 samples of feature i,
+  // four 16-byte stores
+  for (uint32_t i = lane_id(); i < nrows; i += WAVE)
+    for (uint32_t c = 0; c < 4; ++c) {
+      bf16x8 v;
+      for (uint32_t j = 0; j < 8; ++j) v[j] = src[(8 * c + j) * stride + i];
+      *reinterpret_cast<bf16x8*>(dst + (size_t)i * Bp + b0 + 8 * c) = v;
+    }
+}
+
+// Forward + backward of 32 samples of the loss 0.5*mean((y-t)^2): the loss,
+// the bias gradients and W3's are added atomically into grad (FP32,
+// parameter order); the operands of W2's and W1's gradients are written to
+// act for mlp_wgrad_kernel.  (A wave looping over several tiles with these
+// sums in registers needs 256 VGPRs, one wave per SIMD, and was slower.)
 __global__ void __launch_bounds__(64) mlp_grad_kernel(Weights W, const float* __restrict__ x,
-                                                      const float* __restrict__ tgt, uint32_t B,
-                                                      float* __restrict__ grad, float* __restrict__ loss) {
+                                                      const float* __restrict__ tgt, uint32_t B, uint32_t Bp,
+                                                      float* __restrict__ grad, float* __restrict__ loss,
+                                                      __bf16* __restrict__ act) {
   __shared__ TileLds L;
   __shared__ float dys[32];
   const uint32_t lane = lane_id(), rl = lane % 32, kh = 8 * (lane / 32);
@@ -191,33 +211,16 @@ __global__ void __launch_bounds__(64) mlp_grad_kernel(Weights W, const float* __
     }
   }
   wave_sync();
-  // gW2[i][j] += sum_b d2[b][i] h1[b][j]   (K = 32 samples)
-  for (int it = 0; it < H / 32; ++it)
-    for (int jt = 0; jt < H / 32; ++jt) {
-      f32x16 acc = {};
-      for (int s = 0; s < 2; ++s) {
-        bf16x8 a, b;
-        for (int j = 0; j < 8; ++j) {
-          const uint32_t bs = 16 * s + kh + j;
-          a[j] = L.d[bs * HS + 32 * it + rl];
-          b[j] = L.h1[bs * HS + 32 * jt + rl];
-        }
-        acc = mfma(a, b, acc);
-      }
-      for (int r = 0; r < 16; ++r) atomicAdd(&grad[OW2 + (32 * it + c_row(r, lane)) * H + 32 * jt + rl], acc[r]);
-    }
+  // the operands of gW2 = d2^T h1 and gW1 = d1^T x (mlp_wgrad_kernel)
+  put_rows(act, Bp, b0, L.d, HS, H);
+  put_rows(act + (size_t)H * Bp, Bp, b0, L.h1, HS, H);
+  put_rows(act + (size_t)3 * H * Bp, Bp, b0, L.xs, XS, F);
   // dh1[b][j] = sum_i d2[b][i] W2[i][j];  d1 = [h1 > 0] dh1
   f32x16 dh[4];
   for (int t = 0; t < H / 32; ++t) {
     f32x16 acc = {};
-    for (int s = 0; s < H / 16; ++s) {
-      bf16x8 a, b;
-      for (int j = 0; j < 8; ++j) {
-        a[j] = L.d[rl * HS + 16 * s + kh + j];
-        b[j] = W.w2t[(32 * t + rl) * H + 16 * s + kh + j];
-      }
-      acc = mfma(a, b, acc);
-    }
+    for (int s = 0; s < H / 16; ++s)
+      acc = mfma(ld8(&L.d[rl * HS + 16 * s + kh]), ld8(&W.w2t[(32 * t + rl) * H + 16 * s + kh]), acc);
     dh[t] = acc;
   }
   wave_sync();
@@ -234,20 +237,36 @@ __global__ void __launch_bounds__(64) mlp_grad_kernel(Weights W, const float* __
     if (lane < 32) atomicAdd(&grad[OB1 + colh], gb1);
   }
   wave_sync();
-  // gW1[i][j] += sum_b d1[b][i] x[b][j]
-  for (int it = 0; it < H / 32; ++it) {
-    f32x16 acc = {};
-    for (int s = 0; s < 2; ++s) {
-      bf16x8 a, b;
-      for (int j = 0; j < 8; ++j) {
-        const uint32_t bs = 16 * s + kh + j;
-        a[j] = L.d[bs * HS + 32 * it + rl];
-        b[j] = L.xs[bs * XS + rl];
-      }
-      acc = mfma(a, b, acc);
-    }
-    for (int r = 0; r < 16; ++r) atomicAdd(&grad[OW1 + (32 * it + c_row(r, lane)) * F + rl], acc[r]);
+  put_rows(act + (size_t)2 * H * Bp, Bp, b0, L.d, HS, H);
+}
+
+// Weight gradients from the operands mlp_grad_kernel wrote: gW2[i][j] = sum_b d2[b][i] h1[b][j] and gW1[i][j] =
+// sum_b d1[b][i] x[b][j] over the samples [k0, k1) of this workgroup (split
+// K): wave w owns rows 32w..32w+31 of both, i.e. four 32x32 tiles of gW2
+// and one of gW1, in MFMA accumulators over the chunk, 16 samples per step
+// (each operand one 16-byte load per lane), then added to grad once.
+__global__ void __launch_bounds__(256) mlp_wgrad_kernel(const __bf16* __restrict__ act, uint32_t Bp, uint32_t KC,
+                                                        float* __restrict__ grad) {
+  const uint32_t lane = lane_id(), w = threadIdx.x / WAVE, rl = lane % 32, kh = 8 * (lane / 32);
+  const uint32_t k0 = blockIdx.x * KC, k1 = min(Bp, k0 + KC);
+  const __bf16* d2t = act;
+  const __bf16* h1t = act + (size_t)H * Bp;
+  const __bf16* d1t = act + (size_t)2 * H * Bp;
+  const __bf16* xt = act + (size_t)3 * H * Bp;
+  f32x16 acc2[4] = {}, acc1 = {};
+  for (uint32_t k = k0; k < k1; k += 16) {
+    const size_t kk = k + kh;
+    const bf16x8 a2 = *reinterpret_cast<const bf16x8*>(d2t + (size_t)(32 * w + rl) * Bp + kk);
+    const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(d1t + (size_t)(32 * w + rl) * Bp + kk);
+    const bf16x8 bx = *reinterpret_cast<const bf16x8*>(xt + (size_t)rl * Bp + kk);
+    bf16x8 bh[4];
+    for (int jt = 0; jt < 4; ++jt) bh[jt] = *reinterpret_cast<const bf16x8*>(h1t + (size_t)(32 * jt + rl) * Bp + kk);
+    for (int jt = 0; jt < 4; ++jt) acc2[jt] = mfma(a2, bh[jt], acc2[jt]);
+    acc1 = mfma(a1, bx, acc1);
   }
+  for (int jt = 0; jt < 4; ++jt)
+    for (int r = 0; r < 16; ++r) atomicAdd(&grad[OW2 + (32 * w + c_row(r, lane)) * H + 32 * jt + rl], acc2[jt][r]);
+  for (int r = 0; r < 16; ++r) atomicAdd(&grad[OW1 + (32 * w + c_row(r, lane)) * F + rl], acc1[r]);
 }
 
 // BF16 copies of the weights (and W2 transposed) from the FP32 masters.
@@ -291,6 +310,7 @@ struct gpsm_mlp {
   float* loss = nullptr;
   float* norm = nullptr;  // mu[F], inv_sd[F]
   DevBuf x, t, y;
+  DevBuf act;  // the weight-gradient GEMMs' operands (mlp_grad_kernel)
   HostBuf hx, hy;  // pinned staging of forward's features and outputs
   int step = 0;
 };
@@ -319,10 +339,17 @@ int stage(gpsm_mlp* mm, const float* x, const float* t, uint32_t B) {
 
 int run_grad(gpsm_mlp* mm, uint32_t B) {
   hipStream_t s = mm->ctx->stream;
+  const uint32_t Bp = (B + 31) / 32 * 32;
+  if (mm->act.ensure(sizeof(__bf16) * (size_t)(3 * H + F) * Bp)) return GPSM_ENOMEM;
+  __bf16* act = reinterpret_cast<__bf16*>(mm->act.ptr);
   hipMemsetAsync(mm->loss, 0, sizeof(float), s);
-  hipLaunchKernelGGL(mlp_grad_kernel, dim3((B + 31) / 32), dim3(64), 0, s, weights_of(mm),
-                     reinterpret_cast<const float*>(mm->x.ptr), reinterpret_cast<const float*>(mm->t.ptr), B, mm->g,
-                     mm->loss);
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3(Bp / 32), dim3(64), 0, s, weights_of(mm),
+                     reinterpret_cast<const float*>(mm->x.ptr), reinterpret_cast<const float*>(mm->t.ptr), B, Bp,
+                     mm->g, mm->loss, act);
+  // split K: at most 64 workgroups of at least 256 samples (a multiple of 16)
+  const uint32_t chunks = std::min<uint32_t>(64, std::max<uint32_t>(1, Bp / 256));
+  const uint32_t KC = ((Bp + chunks - 1) / chunks + 15) / 16 * 16;
+  hipLaunchKernelGGL(mlp_wgrad_kernel, dim3((Bp + KC - 1) / KC), dim3(256), 0, s, act, Bp, KC, mm->g);
   return hipGetLastError() == hipSuccess ? GPSM_OK : GPSM_EDEVICE;
 }
 
@@ -383,6 +410,7 @@ void gpsm_mlp_free(gpsm_mlp* mm) {
   mm->hy.release();
   mm->t.release();
   mm->y.release();
+  mm->act.release();
   delete mm;
 }
 

@@ -49,7 +49,7 @@ def test_forward_matches_oracle(ctx, B):
     m.free()
 
 
-@pytest.mark.parametrize("B", [32, 77, 256])
+@pytest.mark.parametrize("B", [32, 77, 256, 5000])
 def test_gradients_match_oracle(ctx, B):
     m = MLP(ctx)
     p = params(100 + B)
