@@ -138,20 +138,24 @@ __global__ void __launch_bounds__(64) mlp_forward_kernel(Weights W, const float*
   if (lane < 32 && b0 + lane < B) y[b0 + lane] = L.y[lane];
 }
 
-// The operands of the weight-gradient GEMMs, feature-major over the
-// batch padded to Bp samples (a multiple of 32): D2T[H][Bp] (output-layer
-// deltas d2), H1T[H][Bp] (hidden activations h1), D1T[H][Bp] (first-layer
-// deltas d1), XT[F][Bp] (standardised inputs), all BF16, one after the
-// other in act.
-__device__ __forceinline__ void put_rows(__bf16* __restrict__ dst, uint32_t Bp, uint32_t b0, const __bf16* src,
-                                         uint32_t stride, uint32_t nrows) {
-  // row i of the tile's feature-major block: the 32 samples of feature i,
-  // four 16-byte stores
+// The operands of the weight-gradient GEMMs over the batch padded to Bp
+// samples (a multiple of 32): d2 (output-layer deltas, H features), h1
+// (hidden activations, H), d1 (first-layer deltas, H) and x (standardised
+// inputs, F), all BF16, one after the other in act, each in blocks of 16
+// samples: value (feature i, sample b) of an n-feature matrix at
+// opnd(n, i, b) = ((b / 16) * n + i) * 16 + b % 16.  One MFMA operand (32
+// features x 16 samples) is then 1 KB contiguous, one 16-byte load per lane.
+__device__ __forceinline__ size_t opnd(uint32_t n, uint32_t i, uint32_t b) {
+  return ((size_t)(b >> 4) * n + i) * 16 + (b & 15);
+}
+__device__ __forceinline__ void put_rows(__bf16* __restrict__ dst, uint32_t b0, const __bf16* src, uint32_t stride,
+                                         uint32_t nrows) {
+  // feature i's 32 samples of the tile: four runs of 8, one 16-byte store each
   for (uint32_t i = lane_id(); i < nrows; i += WAVE)
     for (uint32_t c = 0; c < 4; ++c) {
       bf16x8 v;
       for (uint32_t j = 0; j < 8; ++j) v[j] = src[(8 * c + j) * stride + i];
-      *reinterpret_cast<bf16x8*>(dst + (size_t)i * Bp + b0 + 8 * c) = v;
+      *reinterpret_cast<bf16x8*>(dst + opnd(nrows, i, b0 + 8 * c)) = v;
     }
 }
 
@@ -212,9 +216,9 @@ __global__ void __launch_bounds__(64) mlp_grad_kernel(Weights W, const float* __
   }
   wave_sync();
   // the operands of gW2 = d2^T h1 and gW1 = d1^T x (mlp_wgrad_kernel)
-  put_rows(act, Bp, b0, L.d, HS, H);
-  put_rows(act + (size_t)H * Bp, Bp, b0, L.h1, HS, H);
-  put_rows(act + (size_t)3 * H * Bp, Bp, b0, L.xs, XS, F);
+  put_rows(act, b0, L.d, HS, H);
+  put_rows(act + (size_t)H * Bp, b0, L.h1, HS, H);
+  put_rows(act + (size_t)3 * H * Bp, b0, L.xs, XS, F);
   // dh1[b][j] = sum_i d2[b][i] W2[i][j];  d1 = [h1 > 0] dh1
   f32x16 dh[4];
   for (int t = 0; t < H / 32; ++t) {
@@ -237,7 +241,7 @@ __global__ void __launch_bounds__(64) mlp_grad_kernel(Weights W, const float* __
     if (lane < 32) atomicAdd(&grad[OB1 + colh], gb1);
   }
   wave_sync();
-  put_rows(act + (size_t)2 * H * Bp, Bp, b0, L.d, HS, H);
+  put_rows(act + (size_t)2 * H * Bp, b0, L.d, HS, H);
 }
 
 // Weight gradients from the operands mlp_grad_kernel wrote: gW2[i][j] = sum_b d2[b][i] h1[b][j] and gW1[i][j] =
@@ -255,12 +259,11 @@ __global__ void __launch_bounds__(256) mlp_wgrad_kernel(const __bf16* __restrict
   const __bf16* xt = act + (size_t)3 * H * Bp;
   f32x16 acc2[4] = {}, acc1 = {};
   for (uint32_t k = k0; k < k1; k += 16) {
-    const size_t kk = k + kh;
-    const bf16x8 a2 = *reinterpret_cast<const bf16x8*>(d2t + (size_t)(32 * w + rl) * Bp + kk);
-    const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(d1t + (size_t)(32 * w + rl) * Bp + kk);
-    const bf16x8 bx = *reinterpret_cast<const bf16x8*>(xt + (size_t)rl * Bp + kk);
+    const bf16x8 a2 = ld8(d2t + opnd(H, 32 * w + rl, k + kh));
+    const bf16x8 a1 = ld8(d1t + opnd(H, 32 * w + rl, k + kh));
+    const bf16x8 bx = ld8(xt + opnd(F, rl, k + kh));
     bf16x8 bh[4];
-    for (int jt = 0; jt < 4; ++jt) bh[jt] = *reinterpret_cast<const bf16x8*>(h1t + (size_t)(32 * jt + rl) * Bp + kk);
+    for (int jt = 0; jt < 4; ++jt) bh[jt] = ld8(h1t + opnd(H, 32 * jt + rl, k + kh));
     for (int jt = 0; jt < 4; ++jt) acc2[jt] = mfma(a2, bh[jt], acc2[jt]);
     acc1 = mfma(a1, bx, acc1);
   }
