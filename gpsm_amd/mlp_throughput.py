@@ -57,33 +57,52 @@ def run(sizes, reps):
 
 
 def summarise(trace_csv, run_json, out_md, command):
-    per = collections.defaultdict(list)
+    disp = []
     for r in csv.DictReader(open(trace_csv)):
         k = re.search(r"(mlp_\w+_kernel|adam_kernel|refresh_kernel)", r["Kernel_Name"])
-        if not k:
-            continue
-        grid = int(r.get("Grid_Size_X") or r.get("Grid_Size") or 0)
-        per[(k.group(1), grid)].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3)
+        if k:
+            disp.append((int(r["Start_Timestamp"]), k.group(1), int(r.get("Grid_Size_X") or r.get("Grid_Size") or 0),
+                         (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3))
+    disp.sort()
+    # forward launches by samples (one 64-lane wave per 32 samples); a
+    # gradient is mlp_grad_kernel (one wave per 32 samples) and the
+    # mlp_wgrad_kernel launch after it, timed together
+    fwd, grad, parts = collections.defaultdict(list), collections.defaultdict(list), collections.defaultdict(list)
+    pending = None
+    for _, k, grid, us in disp:
+        if k == "mlp_forward_kernel":
+            fwd[grid // 64 * TILE].append(us)
+        elif k == "mlp_grad_kernel":
+            pending = (grid // 64 * TILE, us)
+            parts[("mlp_grad_kernel", pending[0])].append(us)
+        elif k == "mlp_wgrad_kernel" and pending:
+            grad[pending[0]].append(pending[1] + us)
+            parts[("mlp_wgrad_kernel", pending[0])].append(us)
+            pending = None
+        else:
+            parts[(k, 0)].append(us)
     lines = ["# Planner network kernels: samples/s and MFMA rate", "",
              "Command (from the repository root, one MI355X):", "", "```", command, "```", "",
-             "Network 32 -> 128 -> 128 -> 1 (ReLU), BF16 operands, FP32 accumulation; one wave per %d samples. "
-             "MFMA FLOP per sample: forward %d, forward + backward %d (v_mfma_f32_32x32x16_bf16). "
-             "Peak: %.0f TFLOP/s dense BF16." % (TILE, MFMA_FLOP_FWD, MFMA_FLOP_GRAD, PEAK_BF16_TFLOPS), "",
-             "Kernel durations from the trace (per launch, averaged over the launches of one grid size):", "",
-             "| kernel | samples per launch | launches | average us | samples/s | MFMA TFLOP/s | % of peak |",
+             "Network 32 -> 128 -> 128 -> 1 (ReLU), BF16 operands, FP32 accumulation. Forward: one wave per %d "
+             "samples. Gradient: `mlp_grad_kernel` (one wave per %d samples: forward, backward to the deltas, bias "
+             "and W3 gradients) writes the operands of W2's and W1's gradients in 16-sample blocks, and "
+             "`mlp_wgrad_kernel` forms those two as split-K GEMMs (at most 64 workgroups of four waves, one flush "
+             "of atomics per workgroup). MFMA FLOP per sample: forward %d, forward + backward %d "
+             "(v_mfma_f32_32x32x16_bf16). Peak: %.0f TFLOP/s dense BF16."
+             % (TILE, TILE, MFMA_FLOP_FWD, MFMA_FLOP_GRAD, PEAK_BF16_TFLOPS), "",
+             "Kernel durations from the trace (per call, averaged over the calls of one batch size):", "",
+             "| pass | samples per call | calls | average us | samples/s | MFMA TFLOP/s | % of peak |",
              "|---|---|---|---|---|---|---|"]
-    for (k, grid), ds in sorted(per.items()):
-        if not k.startswith("mlp_"):
-            continue
-        samples = grid // 64 * TILE  # grid size in threads: one 64-lane wave per tile
-        us = sum(ds) / len(ds)
-        flop = (MFMA_FLOP_GRAD if "grad" in k else MFMA_FLOP_FWD) * samples
-        tf = flop / (us * 1e-6) / 1e12
-        lines.append("| %s | %d | %d | %.1f | %.3g | %.3f | %.4f |" % (k, samples, len(ds), us, samples / (us * 1e-6),
-                                                                        tf, 100 * tf / PEAK_BF16_TFLOPS))
-    other = {k: ds for (k, g), ds in per.items() if not k.startswith("mlp_")}
-    for k, ds in sorted(other.items()):
-        lines.append("| %s | - | %d | %.1f | | | |" % (k, len(ds), sum(ds) / len(ds)))
+    for name, table, flop in (("forward", fwd, MFMA_FLOP_FWD), ("gradient (both kernels)", grad, MFMA_FLOP_GRAD)):
+        for samples, ds in sorted(table.items()):
+            us = sum(ds) / len(ds)
+            tf = flop * samples / (us * 1e-6) / 1e12
+            lines.append("| %s | %d | %d | %.1f | %.3g | %.3f | %.4f |" % (name, samples, len(ds), us,
+                                                                        samples / (us * 1e-6), tf,
+                                                                        100 * tf / PEAK_BF16_TFLOPS))
+    lines += ["", "| kernel | samples per call | launches | average us |", "|---|---|---|---|"]
+    for (k, samples), ds in sorted(parts.items()):
+        lines.append("| %s | %s | %d | %.1f |" % (k, samples or "-", len(ds), sum(ds) / len(ds)))
     if run_json:
         lines += ["", "Whole C-ABI calls timed on the host (inputs copied from host memory each call; the training "
                   "loop uses 256-sample mini-batches):", "",
