@@ -77,7 +77,7 @@ def main():
         print(json.dumps({"copy_bytes": a.numel() * 4, "copies": 3}))
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on exactly one GPU (the matcher shards by query across GPUs only "
+        print("bench.py: the matcher runs on exactly one GPU (the matcher shards by query across GPUs only "
               "in design); --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 

@@ -1402,7 +1402,7 @@ int gpsm_autotune(gpsm_ctx* c, const gpsm_graph* g, uint32_t nq, const gpsm_quer
   o.join_cap = 0;
   o.use_graph = 0;  // the trials time launches with HIP events, which captured graphs do not record
   std::vector<gpsm_result> ref(nq), out(nq);
-  // The join's settings are judged by the join kernel's own time (HIP
+  // The join's settings are compared by the join kernel's own time (HIP
   // events, profile mode 3) and the filter's by the batch's wall time: the
   // other phases' run-to-run variation is as large as the difference
   // between neighbouring join settings.  The caller's profiling state is

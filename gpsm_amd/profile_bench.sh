@@ -2,13 +2,13 @@
 # rocprofv3 runs of bench.py on one MI355X whose summaries go to profiles/
 # (gpsm_amd/profile_summary.py): a kernel trace with statistics, FETCH_SIZE
 # and WRITE_SIZE of the bench and of its 1 GiB copy calibration, and SQ /
-# GRBM counters.  Raw output goes to $1 (default gpurun_out/prof).
+# GRBM counters.  Raw output goes to $1 (default prof_out).
 #
-#   bash gpsm_amd/profile_bench.sh gpurun_out/prof
-#   python3 -m gpsm_amd.profile_summary --prof gpurun_out/prof --out profiles
+#   bash gpsm_amd/profile_bench.sh prof_out
+#   python3 -m gpsm_amd.profile_summary --prof prof_out --out profiles
 set -u
 R=$(cd "$(dirname "$0")/.." && pwd)
-O=$(cd "$R" && mkdir -p "${1:-gpurun_out/prof}" && cd "${1:-gpurun_out/prof}" && pwd)
+O=$(cd "$R" && mkdir -p "${1:-prof_out}" && cd "${1:-prof_out}" && pwd)
 cd /tmp && export TMPDIR=/tmp
 B="--steps 2 --warmup 1 --no-cpu --no-plain --no-autotune"
 echo "step: kernel trace"

@@ -14,7 +14,7 @@ the traced bench run printed.  Output:
   candidate and lane, and the instruction mix;
 * kernel-trace Markdown table (calls, average and total duration).
 
-    python -m gpsm_amd.profile_summary --prof gpurun_out/prof --out profiles
+    python -m gpsm_amd.profile_summary --prof prof_out --out profiles
 """
 import argparse
 import collections
@@ -97,8 +97,8 @@ def main():
     ap.add_argument("--prof", required=True)
     ap.add_argument("--out", required=True)
     ap.add_argument("--tag", default="chunglu1M_mix256")
-    ap.add_argument("--command", default="bash gpsm_amd/profile_bench.sh gpurun_out/prof && "
-                    "python3 -m gpsm_amd.profile_summary --prof gpurun_out/prof --out profiles")
+    ap.add_argument("--command", default="bash gpsm_amd/profile_bench.sh prof_out && "
+                    "python3 -m gpsm_amd.profile_summary --prof prof_out --out profiles")
     args = ap.parse_args()
     P = args.prof
     copy_bytes = 1 << 30

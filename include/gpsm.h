@@ -196,10 +196,11 @@ void gpsm_result_free(gpsm_result* r);
 /* Autotuner (no reference counterpart; the reference hard-codes block sizes
  * 1024/256/32, match/Match.cu:455-1618): times gpsm_match_batch on the given
  * sample for every candidate filter workgroup size (64/128/256), join level
- * buffer (2048/4096/8192 rows), persistent join workgroups per CU (4/8/12/16)
- * and join tasks per workgroup (8/16/32/64), one parameter at a time (best
- * of three runs each; the filter workgroup size judged by the batch's wall
- * time, the join settings by the join kernel's HIP-event time), and keeps
+ * buffer (2048/4096/8192 rows), persistent join workgroups per CU (2/4/6/8,
+ * never more than are resident) and join tasks per workgroup (8/16/32/64),
+ * one parameter at a time (best of three runs each; the filter workgroup
+ * size timed by the batch's wall time, the join settings by the join
+ * kernel's HIP-event time), and keeps
  * the fastest in the context.  The caller's profiling statistics are left
  * as they were.  Results of every trial are checked to agree.  *best_ms
  * receives the join kernel's time (ms) with the chosen setting.  Later
