@@ -199,8 +199,11 @@ __global__ void __launch_bounds__(256) filter_step_kernel(const uint32_t* __rest
   unsigned long long t_cand = 0;
   uint32_t t_ent = 0, t_key = 0, t_probe = 0, t_set = 0;
 
+  // the next item's list entry is loaded while the current one runs
+  uint2 nxt = wave < items ? list[wave] : make_uint2(0u, 0u);
   for (uint32_t it = wave; it < items; it += nwaves) {
-    const uint2 item = list[it];
+    const uint2 item = nxt;
+    if (it + nwaves < items) nxt = list[it + nwaves];
     const uint32_t o = uniform(item.x);
     const uint32_t c = uniform(item.y);
     const uint32_t type = uniform(ops[o].type);
