@@ -84,8 +84,8 @@ void launch_filter_step(const DevGraph& g, const FilterOp* ops, uint32_t nops, c
 // per (row, segment).
 constexpr uint32_t SEG_WORDS = 2048;
 inline uint32_t segs_of(uint32_t W) { return (W + SEG_WORDS - 1) / SEG_WORDS; }
-// segc[r * segs_of(W) + i] = popcount of segment i of bitmap row r;
-// counts[r] = popcount of row r.
+// segc[r * segs_of(W) + i] = set bits of bitmap row r before segment i
+// (exclusive prefix of the segments' popcounts); counts[r] = popcount of row r.
 void launch_row_popcount(const DevGraph& g, const uint32_t* bitmaps, uint32_t nrows, uint32_t* segc,
                          uint32_t* counts, hipStream_t s);
 // Compaction of bitmap rows to ascending id lists (row r at clist + coff[r])

@@ -6,10 +6,11 @@
 // Candidate sets are bitmaps here, so "collect" is a popcount scan.  Each
 // bitmap row is cut into segments of SEG_WORDS words, so that a batch of a
 // few hundred rows still fills 256 CUs: a first pass popcounts every
-// segment (one workgroup per (row, segment); a row's segments summed into
-// its candidate count), the second (one workgroup per four consecutive
-// segments of a row) takes the group's row-relative base from the counts of
-// the segments before it and compacts the four one after the other (each
+// segment (one workgroup per (row, segment)), a row's segment counts are
+// scanned in place into row-relative offsets (one workgroup per row; the
+// total is the row's candidate count), and the second pass (one workgroup
+// per four consecutive segments of a row) takes the group's base from them
+// and compacts the four one after the other (each
 // thread owns eight consecutive words, one block scan places them), ids
 // staged in LDS and written back coalesced.  The same pass writes the rank
 // index (set bits before each 256-bit block, row-relative) the join uses to
@@ -47,13 +48,31 @@ __global__ void __launch_bounds__(256) seg_popcount_kernel(const uint32_t* __res
   if (threadIdx.x == 0) segc[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
-__global__ void row_totals_kernel(const uint32_t* __restrict__ segc, uint32_t nseg, uint32_t nrows,
-                                  uint32_t* __restrict__ counts) {
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrows) return;
-  uint32_t t = 0;
-  for (uint32_t i = 0; i < nseg; ++i) t += segc[(size_t)r * nseg + i];
-  counts[r] = t;
+// Row r's segment popcounts become their exclusive prefix sums (in place),
+// and counts[r] their total: one workgroup per row, 256 segments per step
+// (a wave scan each, then the four wave totals).  The compaction then reads
+// a group's row-relative base with one load.  (One thread per row adding
+// its counts took 0.5 ms for the 8192 segments of a 2^29-bit row, and each
+// compaction workgroup re-added the counts before its group.)
+__global__ void __launch_bounds__(256) seg_offsets_kernel(uint32_t* __restrict__ segc, uint32_t nseg,
+                                                          uint32_t* __restrict__ counts) {
+  __shared__ uint32_t part[4];
+  uint32_t* p = segc + (size_t)blockIdx.x * nseg;
+  const uint32_t lane = lane_id(), w = threadIdx.x / WAVE;
+  uint32_t carry = 0;
+  for (uint32_t b = 0; b < nseg; b += 256) {
+    const uint32_t i = b + threadIdx.x;
+    const uint32_t x = i < nseg ? p[i] : 0u;
+    const uint32_t incl = wave_incl_scan(x);
+    if (lane == WAVE - 1) part[w] = incl;
+    __syncthreads();
+    uint32_t before = carry;
+    for (uint32_t k = 0; k < w; ++k) before += part[k];
+    if (i < nseg) p[i] = before + incl - x;
+    carry += part[0] + part[1] + part[2] + part[3];
+    __syncthreads();  // the next step rewrites part
+  }
+  if (threadIdx.x == 0) counts[blockIdx.x] = carry;
 }
 
 // A workgroup compacts COMPACT_GROUP consecutive segments of one row, one
@@ -108,12 +127,7 @@ __global__ void __launch_bounds__(256) compact_rows_kernel(const uint32_t* __res
   const uint32_t* bm = bitmaps + (size_t)r * W;
   uint32_t* bp = bprefix + (size_t)r * NB;
   uint32_t* out = clist + coff[r];
-  if (threadIdx.x < WAVE) {  // set bits of the row's segments before the group
-    uint32_t x = 0;
-    for (uint32_t i = threadIdx.x; i < sg0; i += WAVE) x += segc[(size_t)r * nseg + i];
-    x = wave_sum(x);
-    if (threadIdx.x == 0) rel = x;
-  }
+  if (threadIdx.x == 0) rel = segc[(size_t)r * nseg + sg0];  // set bits of the row's segments before the group
   uint32_t w[CW];
   load_words(bm, W, sg0 * SEG_WORDS, w);
   for (uint32_t sg = sg0; sg < sg1; ++sg) {
@@ -202,17 +216,47 @@ __global__ void __launch_bounds__(256) scan_reduce_kernel(const uint32_t* __rest
   if (threadIdx.x == 0) sums[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
-__global__ void scan_sums_kernel(unsigned long long* __restrict__ sums, uint32_t nb,
-                                 unsigned long long* __restrict__ total) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    unsigned long long acc = 0;
-    for (uint32_t i = 0; i < nb; ++i) {
-      unsigned long long t = sums[i];
-      sums[i] = acc;
-      acc += t;
+// The tile sums' exclusive scan, in place, and their total: one workgroup
+// takes them in chunks of 4096 in the tile pass's striped order (wave scans
+// per row, one wave over the 64 (row, wave) totals) with a running carry.
+// A single thread's serial loop over them took 3.8 ms for 32,768 sums.
+constexpr uint32_t SUMS_ITEMS = 16;
+static_assert(SUMS_ITEMS * 4 == WAVE, "one wave scans the (row, wave) totals");
+__global__ void __launch_bounds__(256) scan_sums_kernel(unsigned long long* __restrict__ sums, uint32_t nb,
+                                                        unsigned long long* __restrict__ total) {
+  __shared__ unsigned long long wt[SUMS_ITEMS * 4 + 1];  // (row j, wave w) at j * 4 + w, then the chunk total
+  const uint32_t lane = lane_id(), w = threadIdx.x / WAVE;
+  unsigned long long carry = 0;
+  for (uint32_t base = 0; base < nb; base += 256 * SUMS_ITEMS) {
+    unsigned long long v[SUMS_ITEMS];
+#pragma unroll
+    for (uint32_t j = 0; j < SUMS_ITEMS; ++j) {
+      const uint32_t i = base + j * 256 + threadIdx.x;
+      v[j] = i < nb ? sums[i] : 0ull;
     }
-    if (total) *total = acc;
+#pragma unroll
+    for (uint32_t j = 0; j < SUMS_ITEMS; ++j) {
+      const unsigned long long incl = wave_incl_scan64(v[j]);
+      if (lane == WAVE - 1) wt[j * 4 + w] = incl;
+      v[j] = incl - v[j];  // exclusive within the wave's 64 sums of row j
+    }
+    __syncthreads();
+    if (w == 0) {
+      const unsigned long long x = wt[lane];
+      const unsigned long long incl = wave_incl_scan64(x);
+      wt[lane] = incl - x;
+      if (lane == WAVE - 1) wt[SUMS_ITEMS * 4] = incl;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < SUMS_ITEMS; ++j) {
+      const uint32_t i = base + j * 256 + threadIdx.x;
+      if (i < nb) sums[i] = carry + wt[j * 4 + w] + v[j];
+    }
+    carry += wt[SUMS_ITEMS * 4];
+    __syncthreads();  // the next chunk rewrites wt
   }
+  if (threadIdx.x == 0 && total) *total = carry;
 }
 
 // A tile in striped order: thread t holds values j * 256 + t (every load
@@ -285,7 +329,7 @@ void launch_row_popcount(const DevGraph& g, const uint32_t* bitmaps, uint32_t nr
   if (!nrows) return;
   const uint32_t nseg = segs_of(g.W);
   hipLaunchKernelGGL(seg_popcount_kernel, dim3(nrows * nseg), dim3(256), 0, s, bitmaps, g.W, nseg, segc);
-  hipLaunchKernelGGL(row_totals_kernel, dim3((nrows + 255) / 256), dim3(256), 0, s, segc, nseg, nrows, counts);
+  hipLaunchKernelGGL(seg_offsets_kernel, dim3(nrows), dim3(256), 0, s, segc, nseg, counts);
 }
 
 void launch_compact_rows(const DevGraph& g, const uint32_t* bitmaps, uint32_t nrows, const uint32_t* segc,
@@ -305,7 +349,7 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, unsign
   if (nb == 0 || scratch_words < 2 * nb) return;
   unsigned long long* sums = reinterpret_cast<unsigned long long*>(scratch);
   hipLaunchKernelGGL(scan_reduce_kernel, dim3((uint32_t)nb), dim3(256), 0, s, in, n, sums);
-  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(64), 0, s, sums, (uint32_t)nb, total);
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(256), 0, s, sums, (uint32_t)nb, total);
   hipLaunchKernelGGL(scan_tile_kernel, dim3((uint32_t)nb), dim3(256), 0, s, in, out, n, sums);
 }
 

@@ -32,6 +32,16 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
   return x;
 }
 
+__device__ __forceinline__ unsigned long long wave_incl_scan64(unsigned long long x) {
+  const uint32_t l = lane_id();
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    unsigned long long y = __shfl_up(x, d, WAVE);
+    if (l >= (uint32_t)d) x += y;
+  }
+  return x;
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
 #pragma unroll
   for (int d = WAVE / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, WAVE);

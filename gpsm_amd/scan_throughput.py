@@ -49,8 +49,10 @@ def run(mode, log2n, density, seed=3):
 
 def kernel_bytes(mode, n, ids):
     if mode == "scan":
-        return {"scan_reduce_kernel": 4 * n, "scan_tile_kernel": 8 * n}
-    return {"seg_popcount_kernel": n // 8, "compact_rows_kernel": n // 8 + 4 * ids + 4 * (n // 256)}
+        return {"scan_reduce_kernel": 4 * n, "scan_sums_kernel": 16 * ((n + 4095) // 4096), "scan_tile_kernel": 8 * n}
+    nseg = (n // 32 + 2047) // 2048  # 2048-word segments
+    return {"seg_popcount_kernel": n // 8, "seg_offsets_kernel": 8 * nseg + 4,
+            "compact_rows_kernel": n // 8 + 4 * ids + 4 * (n // 256)}
 
 
 def summarise(trace_csv, rec, out_md, command, append):
@@ -76,6 +78,14 @@ def summarise(trace_csv, rec, out_md, command, append):
         us = sum(ds) / len(ds)
         gbs = b / (us * 1e-6) / 1e9
         lines.append("| %s | %s | %.1f | %d | %.0f | %.1f |" % (cfg, k, us, b, gbs, 100 * gbs / PEAK_GBS))
+    # the whole operation: its kernels one after another, against the bytes
+    # of the operation itself (the scan reads and writes each value once;
+    # the compaction reads the bitmap once and writes the ids and the index)
+    if all(dur.get(k) for k in byt):
+        us = sum(sum(dur[k]) / len(dur[k]) for k in byt)
+        b = 8 * rec["n"] if rec["mode"] == "scan" else rec["n"] // 8 + 4 * rec.get("ids", 0) + 4 * (rec["n"] // 256)
+        gbs = b / (us * 1e-6) / 1e9
+        lines.append("| %s | all of the above | %.1f | %d | %.0f | %.1f |" % (cfg, us, b, gbs, 100 * gbs / PEAK_GBS))
     with open(out_md, "a" if append else "w") as fh:
         fh.write("\n".join(lines) + "\n")
     if command:

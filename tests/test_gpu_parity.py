@@ -20,7 +20,7 @@ def bits_to_sets(bm, n):
     return out
 
 
-@pytest.mark.parametrize("n", [1, 63, 64, 65, 1023, 1024, 1025, 4097, (1 << 20) + 3])
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 1023, 1024, 1025, 4097, (1 << 20) + 3, (1 << 24) + 4099])
 def test_scan(ctx, n):
     rng = np.random.default_rng(n)
     a = rng.integers(0, 1 << 20, size=n).astype(np.uint32)
@@ -45,6 +45,19 @@ def test_compact(ctx, nbits, density):
     for i in on:
         words[i >> 5] |= np.uint32(1 << (i & 31))
     words[-1] |= np.uint32(0xFFFFFFFF) << np.uint32(nbits & 31) if nbits & 31 else np.uint32(0)  # garbage past nbits
+    assert np.array_equal(ctx.compact(words, nbits), O.compact_bitmap(words, nbits))
+
+
+@pytest.mark.parametrize("density", [0.01, 0.5])
+def test_compact_many_segments(ctx, density):
+    """A bitmap of 260 segments of 2048 words (the per-row scan of segment
+    counts takes two steps), with set bits past nbits that must be ignored."""
+    nbits = (1 << 24) + 3 * 65536 + 37
+    rng = np.random.default_rng(5)
+    by = np.packbits(rng.random(nbits) < density, bitorder="little")
+    words = np.concatenate([by, np.zeros((-len(by)) % 4, dtype=np.uint8)]).view(np.uint32).copy()
+    assert len(words) == (nbits + 31) // 32 and (len(words) + 2047) // 2048 == 260
+    words[-1] |= np.uint32(0xFFFFFFFF) << np.uint32(nbits & 31)
     assert np.array_equal(ctx.compact(words, nbits), O.compact_bitmap(words, nbits))
 
 
