@@ -48,31 +48,64 @@ __global__ void __launch_bounds__(256) seg_popcount_kernel(const uint32_t* __res
   if (threadIdx.x == 0) segc[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
+// Exclusive scan of p[0..n) in place by one workgroup of NT threads, in
+// chunks of NT * ITEMS values in striped order (thread t holds p[c + j * NT
+// + t]): a wave scan per row j, one wave scan of the (row, wave) totals in
+// element order, a running carry across chunks.  Returns the total (in
+// every thread).  Contains __syncthreads.
+template <typename T, uint32_t NT, uint32_t ITEMS>
+__device__ __forceinline__ T block_scan_in_place(T* __restrict__ p, uint32_t n) {
+  constexpr uint32_t NW = NT / WAVE;
+  static_assert(NT % WAVE == 0 && ITEMS * NW == WAVE, "one wave scans the (row, wave) totals");
+  __shared__ T wt[ITEMS * NW + 1];  // (row j, wave w) at j * NW + w, then the chunk total
+  const uint32_t lane = lane_id(), w = threadIdx.x / WAVE;
+  T carry = 0;
+  for (uint32_t base = 0; base < n; base += NT * ITEMS) {
+    T v[ITEMS];
+#pragma unroll
+    for (uint32_t j = 0; j < ITEMS; ++j) {
+      const uint32_t i = base + j * NT + threadIdx.x;
+      v[j] = i < n ? p[i] : T(0);
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < ITEMS; ++j) {
+      T incl;
+      if constexpr (sizeof(T) == 8) incl = wave_incl_scan64(v[j]);
+      else incl = wave_incl_scan(v[j]);
+      if (lane == WAVE - 1) wt[j * NW + w] = incl;
+      v[j] = incl - v[j];  // exclusive within the wave's 64 values of row j
+    }
+    __syncthreads();
+    if (w == 0) {
+      const T x = wt[lane];
+      T incl;
+      if constexpr (sizeof(T) == 8) incl = wave_incl_scan64(x);
+      else incl = wave_incl_scan(x);
+      wt[lane] = incl - x;
+      if (lane == WAVE - 1) wt[ITEMS * NW] = incl;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < ITEMS; ++j) {
+      const uint32_t i = base + j * NT + threadIdx.x;
+      if (i < n) p[i] = carry + wt[j * NW + w] + v[j];
+    }
+    carry += wt[ITEMS * NW];
+    __syncthreads();  // the next chunk rewrites wt
+  }
+  return carry;
+}
+
 // Row r's segment popcounts become their exclusive prefix sums (in place),
-// and counts[r] their total: one workgroup per row, 256 segments per step
-// (a wave scan each, then the four wave totals).  The compaction then reads
-// a group's row-relative base with one load.  (One thread per row adding
-// its counts took 0.5 ms for the 8192 segments of a 2^29-bit row, and each
-// compaction workgroup re-added the counts before its group.)
+// and counts[r] their total: one workgroup per row, 4096 segments per step.
+// The compaction then reads a group's row-relative base with one load.
+// (One thread per row adding its counts took 0.5 ms for the 8192 segments
+// of a 2^29-bit row, and each compaction workgroup re-added the counts
+// before its group.)
 __global__ void __launch_bounds__(256) seg_offsets_kernel(uint32_t* __restrict__ segc, uint32_t nseg,
                                                           uint32_t* __restrict__ counts) {
-  __shared__ uint32_t part[4];
-  uint32_t* p = segc + (size_t)blockIdx.x * nseg;
-  const uint32_t lane = lane_id(), w = threadIdx.x / WAVE;
-  uint32_t carry = 0;
-  for (uint32_t b = 0; b < nseg; b += 256) {
-    const uint32_t i = b + threadIdx.x;
-    const uint32_t x = i < nseg ? p[i] : 0u;
-    const uint32_t incl = wave_incl_scan(x);
-    if (lane == WAVE - 1) part[w] = incl;
-    __syncthreads();
-    uint32_t before = carry;
-    for (uint32_t k = 0; k < w; ++k) before += part[k];
-    if (i < nseg) p[i] = before + incl - x;
-    carry += part[0] + part[1] + part[2] + part[3];
-    __syncthreads();  // the next step rewrites part
-  }
-  if (threadIdx.x == 0) counts[blockIdx.x] = carry;
+  const uint32_t t = block_scan_in_place<uint32_t, 256, 16>(segc + (size_t)blockIdx.x * nseg, nseg);
+  if (threadIdx.x == 0) counts[blockIdx.x] = t;
 }
 
 // A workgroup compacts COMPACT_GROUP consecutive segments of one row, one
@@ -217,46 +250,12 @@ __global__ void __launch_bounds__(256) scan_reduce_kernel(const uint32_t* __rest
 }
 
 // The tile sums' exclusive scan, in place, and their total: one workgroup
-// takes them in chunks of 4096 in the tile pass's striped order (wave scans
-// per row, one wave over the 64 (row, wave) totals) with a running carry.
-// A single thread's serial loop over them took 3.8 ms for 32,768 sums.
-constexpr uint32_t SUMS_ITEMS = 16;
-static_assert(SUMS_ITEMS * 4 == WAVE, "one wave scans the (row, wave) totals");
-__global__ void __launch_bounds__(256) scan_sums_kernel(unsigned long long* __restrict__ sums, uint32_t nb,
-                                                        unsigned long long* __restrict__ total) {
-  __shared__ unsigned long long wt[SUMS_ITEMS * 4 + 1];  // (row j, wave w) at j * 4 + w, then the chunk total
-  const uint32_t lane = lane_id(), w = threadIdx.x / WAVE;
-  unsigned long long carry = 0;
-  for (uint32_t base = 0; base < nb; base += 256 * SUMS_ITEMS) {
-    unsigned long long v[SUMS_ITEMS];
-#pragma unroll
-    for (uint32_t j = 0; j < SUMS_ITEMS; ++j) {
-      const uint32_t i = base + j * 256 + threadIdx.x;
-      v[j] = i < nb ? sums[i] : 0ull;
-    }
-#pragma unroll
-    for (uint32_t j = 0; j < SUMS_ITEMS; ++j) {
-      const unsigned long long incl = wave_incl_scan64(v[j]);
-      if (lane == WAVE - 1) wt[j * 4 + w] = incl;
-      v[j] = incl - v[j];  // exclusive within the wave's 64 sums of row j
-    }
-    __syncthreads();
-    if (w == 0) {
-      const unsigned long long x = wt[lane];
-      const unsigned long long incl = wave_incl_scan64(x);
-      wt[lane] = incl - x;
-      if (lane == WAVE - 1) wt[SUMS_ITEMS * 4] = incl;
-    }
-    __syncthreads();
-#pragma unroll
-    for (uint32_t j = 0; j < SUMS_ITEMS; ++j) {
-      const uint32_t i = base + j * 256 + threadIdx.x;
-      if (i < nb) sums[i] = carry + wt[j * 4 + w] + v[j];
-    }
-    carry += wt[SUMS_ITEMS * 4];
-    __syncthreads();  // the next chunk rewrites wt
-  }
-  if (threadIdx.x == 0 && total) *total = carry;
+// of 1024 threads, 4096 sums per step.  (A single thread's serial loop over
+// them took 3.8 ms for 32,768 sums; 256 threads with 16 sums each, 45 us.)
+__global__ void __launch_bounds__(1024) scan_sums_kernel(unsigned long long* __restrict__ sums, uint32_t nb,
+                                                         unsigned long long* __restrict__ total) {
+  const unsigned long long t = block_scan_in_place<unsigned long long, 1024, 4>(sums, nb);
+  if (threadIdx.x == 0 && total) *total = t;
 }
 
 // A tile in striped order: thread t holds values j * 256 + t (every load
@@ -349,7 +348,7 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, unsign
   if (nb == 0 || scratch_words < 2 * nb) return;
   unsigned long long* sums = reinterpret_cast<unsigned long long*>(scratch);
   hipLaunchKernelGGL(scan_reduce_kernel, dim3((uint32_t)nb), dim3(256), 0, s, in, n, sums);
-  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(256), 0, s, sums, (uint32_t)nb, total);
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, sums, (uint32_t)nb, total);
   hipLaunchKernelGGL(scan_tile_kernel, dim3((uint32_t)nb), dim3(256), 0, s, in, out, n, sums);
 }
 
