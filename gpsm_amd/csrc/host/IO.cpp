@@ -1,11 +1,60 @@
 #include "IO.h"
 
+#include <sys/stat.h>
+
+#include <algorithm>
 #include <cstring>
+
+namespace {
+
+bool is_space(char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\v' || c == '\f'; }
+
+// A number as scanf's %d reads it: leading white space, an optional sign, at
+// least one digit; s advances past it.  (One sscanf per line took most of
+// the 4 s that a 135 MB graph file took to read.)
+bool read_int(const char*& s, int& out) {
+  while (is_space(*s)) ++s;
+  bool neg = false;
+  if (*s == '-' || *s == '+') neg = *s++ == '-';
+  if (*s < '0' || *s > '9') return false;
+  long long v = 0;
+  for (; *s >= '0' && *s <= '9'; ++s)
+    if (v < (1LL << 40)) v = v * 10 + (*s - '0');
+  out = (int)(neg ? -v : v);
+  return true;
+}
+
+// A graph's edges in file order, added to its vertices' in/out lists when
+// the graph ends, each list reserved at its final size first: one
+// allocation per list instead of growth by doubling (about 2.3 s of 2.7 s
+// for 8M edges).  The lists hold what Graph::addEdge in file order gives.
+struct EdgeBuf {
+  std::vector<int> a, b, lb;
+};
+
+void add_edges(Graph* g, const EdgeBuf& e) {
+  const size_t n = g->vertices.size(), m = e.a.size();
+  std::vector<unsigned> nout(n, 0), nin(n, 0);
+  for (size_t i = 0; i < m; ++i) {
+    ++nout[e.a[i]];
+    ++nin[e.b[i]];
+  }
+  for (size_t v = 0; v < n; ++v) {
+    g->vertices[v].out.reserve(nout[v]);
+    g->vertices[v].in.reserve(nin[v]);
+  }
+  for (size_t i = 0; i < m; ++i) g->addEdge(e.a[i], e.b[i], e.lb[i]);
+}
+
+}  // namespace
 
 IO::IO(const std::string& query, const std::string& data, const std::string& file) {
   qfp = std::fopen(query.c_str(), "r");
   dfp = std::fopen(data.c_str(), "r");
   ofp = std::fopen(file.c_str(), "w+");
+  // 1 MiB stdio buffers for the graph streams (fewer read calls)
+  if (qfp) std::setvbuf(qfp, nullptr, _IOFBF, 1 << 20);
+  if (dfp) std::setvbuf(dfp, nullptr, _IOFBF, 1 << 20);
   if (!qfp || !dfp || !ofp) std::fprintf(stderr, "IO: cannot open %s\n", !qfp ? query.c_str() : !dfp ? data.c_str() : file.c_str());
 }
 
@@ -22,39 +71,59 @@ Graph* IO::input(FILE* fp) {
     return nullptr;
   }
   Graph* g = nullptr;
+  EdgeBuf edges;
   char buf[512];
-  long pos = std::ftell(fp);
   while (std::fgets(buf, sizeof buf, fp)) {
-    char* s = buf;
+    const char* s = buf;
     while (*s == ' ' || *s == '\t') ++s;
-    if (*s == '\n' || *s == '\r' || *s == 0) {
-      pos = std::ftell(fp);
-      continue;
-    }
+    if (*s == '\n' || *s == '\r' || *s == 0) continue;
     if (*s == 't') {
-      if (g) {  // next graph starts: rewind to this line
-        std::fseek(fp, pos, SEEK_SET);
+      if (g) {  // next graph starts: back to the start of this line
+        std::fseek(fp, -(long)std::strlen(buf), SEEK_CUR);
+        add_edges(g, edges);
         return g;
       }
+      // "t # id" (scanf's "t # %d": white space optional around '#')
       int id = 0;
-      if (std::sscanf(s, "t # %d", &id) != 1) break;
+      ++s;
+      while (is_space(*s)) ++s;
+      if (*s++ != '#' || !read_int(s, id)) break;
       if (id == -1) return nullptr;
       g = new Graph;
-      // "n m Lv Le" header line
+      // "n m Lv Le" header line: its counts only size the arrays
       if (!std::fgets(buf, sizeof buf, fp)) break;
+      // (at most what the file's size can hold: a "v" or "e" line takes at
+      // least 6 bytes, so a damaged header cannot ask for more)
+      const char* h = buf;
+      int hn, hm;
+      if (read_int(h, hn) && read_int(h, hm) && hn > 0 && hm > 0) {
+        long long cap = 1LL << 26;
+        struct stat st;
+        if (fstat(fileno(fp), &st) == 0 && S_ISREG(st.st_mode)) cap = std::min<long long>(cap, st.st_size / 6 + 1);
+        g->vertices.reserve((size_t)std::min<long long>(hn, cap));
+        const size_t em = (size_t)std::min<long long>(hm, cap);
+        edges.a.reserve(em);
+        edges.b.reserve(em);
+        edges.lb.reserve(em);
+      }
     } else if (*s == 'v') {
       int id, lb;
-      if (!g || std::sscanf(s, "v %d %d", &id, &lb) != 2 || id != (int)g->vertices.size()) break;
+      ++s;
+      if (!g || !read_int(s, id) || !read_int(s, lb) || id != (int)g->vertices.size()) break;
       g->addVertex(lb);
     } else if (*s == 'e') {
+      // "e from to [label]": the label is optional (0 when absent)
       int a, b, lb = 0;
-      if (!g || std::sscanf(s, "e %d %d %d", &a, &b, &lb) < 2) break;
+      ++s;
+      if (!g || !read_int(s, a) || !read_int(s, b)) break;
+      if (!read_int(s, lb)) lb = 0;
       if (a < 0 || b < 0 || a >= (int)g->vertices.size() || b >= (int)g->vertices.size()) break;
-      g->addEdge(a, b, lb);
+      edges.a.push_back(a);
+      edges.b.push_back(b);
+      edges.lb.push_back(lb);
     } else {
       break;
     }
-    pos = std::ftell(fp);
   }
   // malformed line, or end of file before "t # -1"
   delete g;
@@ -132,6 +201,7 @@ extern "C" int gpsm_io_count_graphs(const char* path, uint32_t* ngraphs, uint64_
   if (!path || !ngraphs) return GPSM_EINVAL;
   FILE* fp = std::fopen(path, "r");
   if (!fp) return GPSM_EINVAL;
+  std::setvbuf(fp, nullptr, _IOFBF, 1 << 20);
   IO io;
   uint32_t n = 0;
   uint64_t nv = 0, ne = 0;
