@@ -93,13 +93,15 @@ Graph* IO::input(FILE* fp) {
       // "n m Lv Le" header line: its counts only size the arrays
       if (!std::fgets(buf, sizeof buf, fp)) break;
       // (at most what the file's size can hold: a "v" or "e" line takes at
-      // least 6 bytes, so a damaged header cannot ask for more)
+      // least 6 bytes, so a damaged header cannot ask for more; 2^20 on a
+      // stream of unknown size, which then grows as lines arrive)
       const char* h = buf;
       int hn, hm;
       if (read_int(h, hn) && read_int(h, hm) && hn > 0 && hm > 0) {
-        long long cap = 1LL << 26;
+        long long cap = 1LL << 20;
         struct stat st;
-        if (fstat(fileno(fp), &st) == 0 && S_ISREG(st.st_mode)) cap = std::min<long long>(cap, st.st_size / 6 + 1);
+        if (fstat(fileno(fp), &st) == 0 && S_ISREG(st.st_mode))
+          cap = std::min<long long>(1LL << 26, st.st_size / 6 + 1);
         g->vertices.reserve((size_t)std::min<long long>(hn, cap));
         const size_t em = (size_t)std::min<long long>(hm, cap);
         edges.a.reserve(em);
