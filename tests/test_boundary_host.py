@@ -61,10 +61,11 @@ def _read(driver, tmp_path, name, data):
 
 
 def test_reader_token_forms(driver, tmp_path):
-    """The reader takes the forms its scanf-style rules allow (io/IO.cpp:53-97
-    reads tokens with fscanf): spaces or tabs between tokens, none after
-    't', a sign on a number, CRLF line ends, blank lines, an edge without a
-    label, trailing text after the numbers; the graph read is the same."""
+    """The reader takes tokens by scanf's rules (the reference reads them
+    with fscanf, io/IO.cpp:53-97): spaces or tabs between tokens, none after
+    't', a sign on a number, CRLF line ends, blank lines, trailing text after
+    the numbers; an edge without a label gets label 0.  The graph read is
+    the same."""
     rc, ref = _read(driver, tmp_path, "plain.g", DATA)
     assert rc == 0 and ref[0].startswith("n 5 adj 6 labels 3")
     variant = ("\n  t#0\r\n5 5 3 1\r\n\tv\t0   1\r\nv 1 2 extra\nv 2 +2\n\n   \nv 3 3\nv 4 1\n"
