@@ -1,6 +1,7 @@
 """The profile summarisers on synthetic kernel traces (CPU only): the
 per-kernel rows and the whole-operation row of the scan and compaction
-summary, and the pairing of gradient launches in the planner network's."""
+summary, the pairing of gradient launches in the planner network's, and the
+bench profile's kernel names, counter sums and overlapped join launches."""
 import csv
 import json
 
@@ -69,3 +70,32 @@ def test_mlp_summary_pairs_each_gradient_with_its_weight_kernel(tmp_path):
     assert len(grad) == 1 and grad[0][1] == "256" and grad[0][2] == "2" and float(grad[0][3]) == 12.0
     fwd = [r for r in rows if r[0] == "forward"]
     assert fwd[0][1] == "256" and float(fwd[0][3]) == 5.0
+
+
+def test_profile_summary_helpers(tmp_path):
+    from gpsm_amd import profile_summary as PS
+    assert PS.short("void gpsm::(anonymous namespace)::join_tree_kernel<true>(gpsm::JoinArgs)") == \
+        "join_tree_kernel_counting"
+    assert PS.short("void gpsm::(anonymous namespace)::join_tree_kernel<false>(gpsm::JoinArgs)") == "join_tree_kernel"
+    assert PS.short("gpsm::(anonymous namespace)::scan_tile_kernel(unsigned int const*)") == "scan_tile_kernel"
+    # per-dispatch counters are summed over their rows (one per XCD or SE)
+    p = tmp_path / "pmc.csv"
+    with open(p, "w", newline="") as fh:
+        w = csv.DictWriter(fh, ["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value"])
+        w.writeheader()
+        for d, k, v in ((1, "a_kernel", 2.0), (1, "a_kernel", 3.0), (2, "a_kernel", 7.0), (3, "b_kernel", 1.0)):
+            w.writerow({"Dispatch_Id": d, "Kernel_Name": k, "Counter_Name": "FETCH_SIZE", "Counter_Value": v})
+    c = PS.counters(str(p))
+    assert sorted(c["a_kernel"]["FETCH_SIZE"]) == [5.0, 7.0] and c["b_kernel"]["FETCH_SIZE"] == [1.0]
+    # the tree and cyclic join launches of a batch overlap: first start to last end
+    with open(tmp_path / "j.csv", "w", newline="") as fh:
+        w = csv.DictWriter(fh, ["Kernel_Name", "Start_Timestamp", "End_Timestamp"])
+        w.writeheader()
+        for name, b, e in (("join_tree_kernel<false>", 0, 9000), ("join_cyclic_kernel<false>", 1000, 12000),
+                           ("join_tree_kernel<true>", 20000, 90000),
+                           ("join_tree_kernel<false>", 100000, 104000), ("join_cyclic_kernel<false>", 100500, 103000)):
+            w.writerow({"Kernel_Name": name, "Start_Timestamp": b, "End_Timestamp": e})
+    assert PS.join_pair_us(str(tmp_path / "j.csv")) == [12.0, 4.0]
+    per = {"join_tree_kernel": {"n": 2, "x": 10}, "join_cyclic_kernel": {"n": 2, "x": 4}}
+    PS.add_join_total(per, ["x"], "n")
+    assert per["join_kernel"] == {"n": 2, "x": 14}
