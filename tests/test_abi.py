@@ -215,3 +215,27 @@ def test_cli_directed_refuses_count_and_cache(tmp_path):
     for args in (["--directed", "--count", str(p), str(p)], ["--directed", str(cache), str(p)]):
         r = subprocess.run([exe] + args + [str(tmp_path / "ans.txt")], capture_output=True, text=True, timeout=60)
         assert r.returncode == 2 and "--directed" in r.stderr, r.stderr
+
+
+def test_generators_follow_their_laws():
+    """Chung-Lu: vertex i's expected degree is 2m * min(w_i * s, cap) / sum,
+    w_i = (i + 1)^(-1/(gamma - 1)), s scaling the weights to 2m, cap =
+    sqrt(2m); mean degrees of id slices match it (a wrong exponent or a
+    missing cap fails).  R-MAT: the top-level quadrant of the deduplicated
+    undirected edges follows (a, b, c, d), the dense corner first."""
+    n, m, gamma = 20000, 80000, 2.1
+    c = G.chung_lu(n, m, gamma, 3)
+    deg = np.diff(c.row_offset).astype(np.float64)
+    w = (np.arange(n) + 1.0) ** (-1.0 / (gamma - 1.0))
+    cw = np.minimum(w * (2.0 * m / w.sum()), np.sqrt(2.0 * m))
+    expect = 2.0 * m * cw / cw.sum()
+    for lo, hi in ((100, 200), (1000, 2000), (10000, 20000)):
+        assert abs(deg[lo:hi].mean() / expect[lo:hi].mean() - 1) < 0.1, (lo, hi)
+    assert deg.max() < 2 * np.sqrt(2.0 * m)  # the cap: uncapped, vertex 0 would expect ~9,600
+    r = G.rmat(12, 20000, 0.57, 0.19, 0.19, 5)
+    src = np.repeat(np.arange(r.n), np.diff(r.row_offset).astype(np.int64))
+    dst = r.col.astype(np.int64)
+    keep = src < dst
+    s, d, half = src[keep], dst[keep], r.n // 2
+    both_low, both_high = np.mean((s < half) & (d < half)), np.mean((s >= half) & (d >= half))
+    assert 0.40 < both_low < 0.52 and 0.06 < both_high < 0.12
