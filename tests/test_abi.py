@@ -239,3 +239,17 @@ def test_generators_follow_their_laws():
     s, d, half = src[keep], dst[keep], r.n // 2
     both_low, both_high = np.mean((s < half) & (d < half)), np.mean((s >= half) & (d >= half))
     assert 0.40 < both_low < 0.52 and 0.06 < both_high < 0.12
+
+
+@pytest.mark.parametrize("nl,s", [(16, 1.0), (32, 0.8)])
+def test_label_shares_follow_zipf(nl, s):
+    """Labels 1..L with P(k) proportional to k^-s (16 labels at s = 1.0:
+    P(1) = 0.296, P(16) = 0.018)."""
+    g = G.chung_lu(50000, 100000, 2.1, 11)
+    G.set_labels(g, nl, s, 4)
+    share = np.bincount(g.labels, minlength=nl + 1)[1:] / g.n
+    p = np.arange(1, nl + 1, dtype=np.float64) ** -s
+    p /= p.sum()
+    assert np.all(np.abs(share - p) < 4 * np.sqrt(p * (1 - p) / g.n) + 1e-3)
+    if nl == 16:
+        assert abs(p[0] - 0.296) < 0.001 and abs(p[15] - 0.018) < 0.001
