@@ -253,3 +253,26 @@ def test_label_shares_follow_zipf(nl, s):
     assert np.all(np.abs(share - p) < 4 * np.sqrt(p * (1 - p) / g.n) + 1e-3)
     if nl == 16:
         assert abs(p[0] - 0.296) < 0.001 and abs(p[15] - 0.018) < 0.001
+
+
+def test_walk_query_kinds():
+    """A tree walk keeps the k - 1 first-visit edges; an induced walk keeps
+    every data edge among its k vertices, so some embedding's image spans
+    exactly the query's edges; the half/half mix alternates them by slot."""
+    g = G.small_graph(4, n=3000, m=9000, labels=6, kind="cl")
+    ro, col = g.row_offset, g.col
+
+    def edges_among(vs):
+        s = set(int(v) for v in vs)
+        return sum(int(np.isin(col[ro[v]:ro[v + 1]], list(s)).sum()) for v in s) // 2
+
+    for q in G.queries(g, 6, 5, 7, _lib.Q_WALK_TREE, 21):
+        assert q.m == q.n - 1
+    induced = G.queries(g, 6, 4, 5, _lib.Q_WALK_INDUCED, 22)  # small: O1 enumerates every embedding
+    assert any(q.m > q.n - 1 for q in induced)  # some walk closes a cycle (not only first-visit edges)
+    for q in induced:
+        assert q.m >= q.n - 1
+        r = O.o1_enumerate(g, q, tuple_cap=300000)
+        assert any(edges_among(t) == q.m for t in r["tuples"])
+    half = G.queries(g, 8, 5, 7, _lib.Q_WALK_HALF, 23)
+    assert all(q.m == q.n - 1 for q in half[0::2])
